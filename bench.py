@@ -1,0 +1,183 @@
+"""Benchmark of the MI355X path-tracing hot path (one GPU).
+
+Headline: closest-hit Mrays/s on the config-1 mesh (displaced icosphere, 8
+subdivision levels, + 10K lognormal clutter = 1,320,720 triangles) for the
+2,000,000 incoherent rays (origins uniform on the surface, directions
+uniform over the normal's hemisphere). A step is one closest-hit pass over
+the whole ray batch, rays and BVH resident in HBM. Also reported: coherent
+1920x1080 primaries, the GPU BVH build time, and wavefront path-tracing
+samples/s at 1920x1080 on config 2 (max 16 vertices, NEE + MIS).
+
+Prints one JSON line. `roofline` is for the dominant kernel (closest-hit
+traversal, k_trace): algorithmic bytes per launch = rays x B_ray, with
+B_ray = 32 + 16 + N_node * 32 + N_tri * 36 from the reference SAH BVH's
+mean node visits / triangle tests on the same rays
+(tests/golden/c1_traversal_stats.json), divided by the mean kernel time
+measured with HIP events on the launch stream. `cpu_baseline` times the FP64
+oracle restatement of accel::sahbvh on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'lightmetrica-v3_amd'))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the traversal kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel calibration); None if absent."""
+    p = os.path.join(ROOT, 'profiles', 'trace_c1_incoherent_pmc.json')
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return float(d['hbm_bytes_per_launch'])
+    except Exception:
+        return None
+
+
+def cpu_baseline(flat, rays, n_rays):
+    from oracle import oracle as orc
+    import numpy as np
+    cores = os.cpu_count() or 1
+    t0 = time.time()
+    acc = orc.Accel(flat.verts, cores)
+    build_s = time.time() - t0
+    sub = {k: np.ascontiguousarray(v[:n_rays]) for k, v in rays.items()}
+    acc.intersect(sub, nthreads=cores)  # warm
+    t0 = time.time()
+    acc.intersect(sub, nthreads=cores)
+    dt = time.time() - t0
+    return {'value': n_rays / dt / 1e6, 'unit': 'Mrays/s', 'cores': cores, 'kind': 'port',
+            'sample': f'FP64 accel::sahbvh restatement (oracle), OpenMP over {n_rays} of the 2,000,000 '
+                      f'incoherent config-1 rays; SAH build of 1,320,720 tris {build_s:.1f} s excluded',
+            'build_s': build_s}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--width', type=int, default=4)
+    ap.add_argument('--leaf', type=int, default=4)
+    ap.add_argument('--pt-spp', type=int, default=4)
+    ap.add_argument('--no-pt', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-rays', type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU only; --gpus must be 1 '
+                          '(rays and samples shard by index range with a replicated BVH; see DESIGN.md)'}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lm_amd import scenes
+    from lm_amd.accel import Accel, rays_from_dict
+
+    dev = torch.device('cuda:0')
+    log('generating config-1 scene and rays')
+    scene, rays = scenes.config1()
+    flat = scene.flatten()
+    acc = Accel(args.width, args.leaf)
+    acc.build(flat)
+    acc.build(flat)
+    info = acc.info()
+    log('accel', info)
+
+    def time_rays(label, steps, warmup):
+        r = torch.from_numpy(rays_from_dict(rays[label])).to(dev)
+        h = torch.zeros((r.shape[0], 4), dtype=torch.float32, device=dev)
+        for _ in range(warmup):
+            acc.intersect_device(r, h)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_start.record()
+        for e0, e1 in ev:
+            e0.record()
+            acc.intersect_device(r, h)
+            e1.record()
+        t_end.record()
+        torch.cuda.synchronize()
+        acc.status()
+        total_ms = t_start.elapsed_time(t_end)
+        kern_ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+        return r.shape[0], total_ms, kern_ms
+
+    n_inc, total_ms, kern_ms = time_rays('incoherent', args.steps, args.warmup)
+    value = n_inc * args.steps / (total_ms / 1e3) / 1e6
+    n_coh, tot_c, kern_c = time_rays('coherent', args.steps, args.warmup)
+    coherent = n_coh * args.steps / (tot_c / 1e3) / 1e6
+    log(f'incoherent {value:.1f} Mrays/s (kernel {kern_ms:.3f} ms), coherent {coherent:.1f} Mrays/s')
+
+    with open(os.path.join(ROOT, 'tests', 'golden', 'c1_traversal_stats.json')) as f:
+        ref_stats = json.load(f)
+    b_ray = ref_stats['incoherent']['bytes_per_ray']
+    achieved = n_inc * b_ray / (kern_ms / 1e3) / 1e9
+    traffic = traffic_from_profiles()
+    roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                'kernel': f'k_trace<{args.width},closest> over {n_inc} rays',
+                'bytes_per_ray': round(b_ray, 1), 'kernel_ms': round(kern_ms, 4)}
+
+    extra = {'coherent_mrays_s': round(coherent, 1), 'bvh_build_ms': round(info['build_ms'], 2),
+             'bvh': {k: info[k] for k in ('width', 'leaf_size', 'num_nodes', 'depth', 'sah_cost')}}
+
+    if not args.no_pt:
+        from lm_amd.renderer import GpuScene
+        log('generating config-2 scene (1920x1080)')
+        s2 = scenes.config2().flatten()
+        sc = GpuScene(s2, width=args.width, leaf_size=args.leaf)
+        fs = torch.zeros((s2.h, s2.w, 3), dtype=torch.float64, device=dev)
+        fq = torch.zeros_like(fs)
+        sc.render_raw(spp=1, max_verts=16, seed=1, film_sum=fs, film_sumsq=fq)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        _, _, st = sc.render_raw(spp=args.pt_spp, max_verts=16, seed=1, sample_offset=1, film_sum=fs, film_sumsq=fq)
+        torch.cuda.synchronize()
+        dt = time.time() - t0
+        extra['pt_1080p'] = {'samples_s': round(st['samples'] / dt, 1), 'spp': args.pt_spp,
+                             'secondary_mrays_s': round((st['extension_rays'] + st['shadow_rays']) / dt / 1e6, 1),
+                             'extension_rays': st['extension_rays'], 'shadow_rays': st['shadow_rays'],
+                             'ms_per_spp': round(dt / args.pt_spp * 1e3, 2)}
+        log('pt', extra['pt_1080p'])
+
+    cpu = None
+    if not args.no_cpu:
+        log('cpu baseline (oracle)')
+        cpu = cpu_baseline(flat, rays['incoherent'], min(args.cpu_rays, 2_000_000))
+
+    out = {
+        'metric': 'Mrays/s (incoherent) & samples/s at 1080p; equal-time MSE guided vs plain PT',
+        'value': round(value, 2), 'unit': 'Mrays/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(total_ms / args.steps, 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded procedural scene and rays)',
+        'config': {'workload': 'closest-hit ray cast, config-1 mesh 1,320,720 tris, 2,000,000 incoherent rays',
+                   'bvh_width': args.width, 'leaf_size': args.leaf},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'extra': extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

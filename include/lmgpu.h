@@ -1,0 +1,210 @@
+/*
+ * lmgpu — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * A drop-in replacement for Lightmetrica's `accel::*` (behind lm::Accel) and
+ * `renderer::pt` (behind lm::Renderer), with lm::Film output unchanged.
+ * Plain pointers, sizes and POD structs only; no exceptions cross this
+ * boundary. Every entry point returns an int status (LMGPU_OK = 0, < 0 on
+ * error) and leaves a message in lmgpu_last_error().
+ *
+ * Reference interfaces replaced (paths are relative to the reference repo):
+ *   lm::Accel::build(const Scene&)                  include/lm/accel.h:38
+ *   lm::Accel::intersect(Ray, Float, Float)         include/lm/accel.h:70
+ *   lm::Accel::Hit {t, uv, global_transform, primitive, face}
+ *                                                   include/lm/accel.h:49-55
+ *   lm::Scene::visible(sp1, sp2)                    include/lm/scene.h:343-364
+ *   lm::Renderer::render()  (renderer::pt)          include/lm/renderer.h:25,
+ *                                                   src/renderer/renderer_pt.cpp:80-315
+ *   lm::Film::splat / rescale / buffer              include/lm/film.h:103-205
+ *   error model LM_THROW_EXCEPTION(Error::...)      include/lm/exception.h:142-150
+ */
+#ifndef LMGPU_H
+#define LMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirror lm::Error, include/lm/exception.h) ------------- */
+#define LMGPU_OK 0
+#define LMGPU_ERR_INVALID_ARGUMENT (-1)   /* Error::InvalidArgument */
+#define LMGPU_ERR_UNSUPPORTED (-2)        /* Error::Unsupported */
+#define LMGPU_ERR_FAILED_TO_RENDER (-3)   /* Error::FailedToRender */
+#define LMGPU_ERR_DEVICE (-4)             /* HIP runtime / kernel error */
+#define LMGPU_ERR_OUT_OF_MEMORY (-5)
+#define LMGPU_ERR_STACK_OVERFLOW (-6)     /* traversal stack capacity exceeded */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* lmgpu_last_error(void);
+
+/* Library version string. */
+const char* lmgpu_version(void);
+
+/* Number of visible HIP devices (0 when no GPU). Does not create a context. */
+int lmgpu_device_count(int* count);
+
+/* ---- rays and hits --------------------------------------------------------
+ * One ray = 32 B, one hit = 16 B, both 16-B aligned so a wave of 64 lanes
+ * reads 2 KiB / writes 1 KiB with full-width vector accesses.
+ * Ray: reference lm::Ray {Vec3 o, d} plus the [tmin, tmax] range of
+ * Accel::intersect (include/lm/math.h:67-77, include/lm/accel.h:70), in FP32
+ * (FP64 -> FP32 at the boundary, as plugin/accel_embree/accel_embree.cpp:135-143).
+ * Hit: the global triangle index (flattened order of Accel_SAHBVH::build,
+ * src/accel/accel_sahbvh.cpp:132-151) or -1, t, and the barycentric (u, v)
+ * weights of the 2nd and 3rd vertex (Accel::Hit::uv). The host maps the
+ * triangle index to (primitive node, face) with the tables it flattened. */
+typedef struct lmgpu_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} lmgpu_ray;
+
+typedef struct lmgpu_hit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+} lmgpu_hit;
+
+/* ---- accel ------------------------------------------------------------------ */
+typedef struct lmgpu_accel lmgpu_accel;
+
+typedef struct lmgpu_build_opts {
+    int32_t width;       /* BVH branching factor: 2, 4 or 8 (0 = 4) */
+    int32_t leaf_size;   /* max triangles per leaf, 1..15 (0 = 4) */
+    int32_t reserved[6];
+} lmgpu_build_opts;
+
+typedef struct lmgpu_accel_info {
+    int64_t num_tris;
+    int64_t num_nodes;       /* wide nodes */
+    int64_t num_leaves;
+    int32_t width;
+    int32_t leaf_size;
+    int32_t depth;           /* wide-tree depth (root = 1) */
+    int32_t stack_capacity;  /* traversal stack entries available per ray */
+    int64_t node_bytes;
+    int64_t tri_bytes;
+    double sah_cost;         /* SAH cost of the wide tree (node=1, tri=1 units) */
+    double build_ms;         /* device time of the last build */
+} lmgpu_accel_info;
+
+int lmgpu_accel_create(lmgpu_accel** out);
+int lmgpu_accel_destroy(lmgpu_accel* accel);
+
+/* Builds the BVH over `num_tris` triangles given as [num_tris][9] floats
+ * (p1, p2, p3 world-space positions, FP32), replacing
+ * Accel_SAHBVH::build (src/accel/accel_sahbvh.cpp:127-267).
+ * `verts_on_device` != 0: `verts` is a device pointer, else host memory.
+ * `stream` is a hipStream_t (NULL = default stream). Synchronous on return. */
+int lmgpu_accel_build(lmgpu_accel* accel, const float* verts, int64_t num_tris, int verts_on_device,
+                      const lmgpu_build_opts* opts, void* stream);
+
+int lmgpu_accel_info_get(const lmgpu_accel* accel, lmgpu_accel_info* info);
+
+/* Closest hit for `n` rays, device pointers; enqueued on `stream`, asynchronous.
+ * Batched form of Accel_SAHBVH::intersect (src/accel/accel_sahbvh.cpp:269-299). */
+int lmgpu_accel_intersect(const lmgpu_accel* accel, const lmgpu_ray* rays, lmgpu_hit* hits, int64_t n,
+                          void* stream);
+
+/* Same, host pointers; copies in and out and synchronizes. Serves the
+ * single-ray lm::Accel::intersect of a host component (n = 1). */
+int lmgpu_accel_intersect_host(const lmgpu_accel* accel, const lmgpu_ray* rays, lmgpu_hit* hits,
+                               int64_t n);
+
+/* Any-hit occlusion for `n` rays (device pointers): occluded[i] = 1 when a
+ * triangle is hit in [tmin, tmax], else 0. Equals !Scene::intersect(ray,
+ * tmin, tmax) as used by Scene::visible (include/lm/scene.h:343-364). */
+int lmgpu_accel_occluded(const lmgpu_accel* accel, const lmgpu_ray* rays, uint8_t* occluded, int64_t n,
+                         void* stream);
+
+/* Synchronizes `stream`; returns LMGPU_ERR_STACK_OVERFLOW if a traversal on
+ * this accel exceeded its stack since the last call. Cannot happen when
+ * depth * (width - 1) <= stack_capacity (see lmgpu_accel_info). */
+int lmgpu_accel_status(const lmgpu_accel* accel, void* stream);
+
+/* Closest hit plus traversal counters summed over the rays (node visits,
+ * triangle tests). Synchronous. */
+int lmgpu_accel_intersect_counted(const lmgpu_accel* accel, const lmgpu_ray* rays, lmgpu_hit* hits,
+                                  int64_t n, int64_t* node_visits, int64_t* tri_tests, void* stream);
+
+/* ---- scene + renderer::pt ------------------------------------------------------ */
+
+/* Material type codes: material::diffuse, ::glossy, ::mirror, ::glass. */
+#define LMGPU_MAT_DIFFUSE 0
+#define LMGPU_MAT_GLOSSY 1
+#define LMGPU_MAT_MIRROR 2
+#define LMGPU_MAT_GLASS 3
+
+/* Flattened scene with identity transforms (world-space vertices). */
+typedef struct lmgpu_scene_desc {
+    int64_t num_tris;
+    const float* verts;        /* [N][9] positions p1,p2,p3 (host) */
+    const float* normals;      /* [N][9] vertex normals n1,n2,n3 (host) */
+    const int32_t* tri_prim;   /* [N] primitive (scene node) index */
+    int32_t num_prims;
+    const int32_t* mat_type;   /* [P] LMGPU_MAT_* */
+    const float* mat_param;    /* [P][8]: color rgb (Kd or Ks), ax, ay, Ni, -, - */
+    const float* prim_Ke;      /* [P][3] light::area radiance (unused if not a light) */
+    int32_t num_area_lights;
+    const int32_t* light_prims;/* [L] primitives carrying light::area, in light order */
+    int32_t has_env;           /* light::envconst present (light index L) */
+    float env_Le[3];
+    float cam_pos[3], cam_center[3], cam_up[3];  /* camera::pinhole */
+    float vfov;                /* degrees */
+    float aspect;              /* width / height */
+    int32_t w, h;              /* film size */
+} lmgpu_scene_desc;
+
+typedef struct lmgpu_scene lmgpu_scene;
+
+/* Uploads the scene and builds its accel (opts may be NULL). */
+int lmgpu_scene_create(lmgpu_scene** out, const lmgpu_scene_desc* desc, const lmgpu_build_opts* opts);
+int lmgpu_scene_destroy(lmgpu_scene* scene);
+/* Borrowed pointer to the scene's accel (owned by the scene). */
+int lmgpu_scene_accel(lmgpu_scene* scene, lmgpu_accel** accel);
+
+/* renderer::pt parameters (src/renderer/renderer_pt.cpp:51-77, scheduler.cpp:32-35,72-75). */
+#define LMGPU_SAMPLING_NAIVE 0
+#define LMGPU_SAMPLING_NEE 1
+#define LMGPU_SAMPLING_MIS 2
+
+typedef struct lmgpu_pt_params {
+    int32_t spp;              /* scheduler::spp::sample: samples per pixel */
+    int32_t max_verts;        /* max path vertices */
+    int32_t sampling_mode;    /* LMGPU_SAMPLING_* */
+    int32_t rr_depth;         /* Russian roulette when num_verts > rr_depth (reference 5) */
+    uint64_t seed;            /* Philox key */
+    int32_t sample_offset;    /* index of the first sample (progressive rendering) */
+    int32_t guiding;          /* 0 = plain PT; 1 = MLP path guiding (lmgpu_guide) */
+    double render_time;       /* > 0: scheduler::spp::time, render 1-spp passes for this many seconds */
+    int32_t reserved[8];
+} lmgpu_pt_params;
+
+typedef struct lmgpu_pt_stats {
+    int64_t samples;          /* path samples traced */
+    int64_t spp;              /* samples per pixel accumulated */
+    int64_t extension_rays;
+    int64_t shadow_rays;
+    double elapsed_s;         /* wall time of the render call */
+    int64_t reserved[4];
+} lmgpu_pt_stats;
+
+/* Accumulates, per pixel, the sum and the sum of squares of the per-sample
+ * estimate (FP64, [h][w][3], y = 0 the bottom row, as Film_Bitmap) into
+ * `film_sum` / `film_sumsq` (either may be NULL... sum is required).
+ * `film_on_device` != 0: device pointers, else host pointers (copied back on
+ * return). The film value of renderer::pt is film_sum / spp
+ * (renderer_pt.cpp:307-308). Synchronous. */
+int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* film_sum, double* film_sumsq,
+                    int film_on_device, lmgpu_pt_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* LMGPU_H */

@@ -1,0 +1,537 @@
+// GPU BVH build and ray-cast entry points of the lmgpu C ABI (gfx950).
+//
+// Replaces Accel_SAHBVH::build (src/accel/accel_sahbvh.cpp:127-267), a
+// top-down full-sort SAH build on host threads, with a fully device-side
+// pipeline:
+//   1. per-triangle bounds + centroid bounds (ordered-int atomics),
+//   2. 63-bit Morton codes of the centroids,
+//   3. radix sort of (code, index) pairs (rocPRIM onesweep),
+//   4. binary LBVH topology (Karras, HPG 2012), ties broken by index,
+//   5. bottom-up bounds refit (second-arriving thread merges),
+//   6. collapse into a W-wide tree (W = 2, 4, 8): each wide node repeatedly
+//      opens its largest-area child until it has W children; subtrees of at
+//      most `leaf_size` triangles become leaves over a contiguous range of the
+//      Morton-sorted triangle array,
+//   7. triangles copied into traversal order (48 B each).
+// and Accel_SAHBVH::intersect (:269-299) with the ordered wide traversal of
+// bvh.h.
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "bvh.h"
+#include "lmgpu_internal.h"
+
+namespace lmgpu {
+
+// ---------------------------------------------------------------------------
+// build kernels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float ord2f(uint32_t u) {
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+__global__ void k_init_bounds(uint32_t* cb) {
+    if (threadIdx.x < 3) cb[threadIdx.x] = 0xffffffffu;        // min
+    else if (threadIdx.x < 6) cb[threadIdx.x] = 0u;            // max
+}
+
+// Per-triangle AABB (float4 lo, hi) and the centroid bounds.
+__global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, int64_t n, float4* __restrict__ tlo,
+                                              float4* __restrict__ thi, uint32_t* __restrict__ cb) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float cmin[3] = {3e38f, 3e38f, 3e38f}, cmax[3] = {-3e38f, -3e38f, -3e38f};
+    if (i < n) {
+        const float* v = verts + 9 * i;
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(v[a], fminf(v[3 + a], v[6 + a]));
+            hi[a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a]));
+            const float c = (lo[a] + hi[a]) * 0.5f;
+            cmin[a] = c;
+            cmax[a] = c;
+        }
+        tlo[i] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        thi[i] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+    // wave reduction then one atomic per wave
+    for (int a = 0; a < 3; a++) {
+        float mn = cmin[a], mx = cmax[a];
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&cb[a], f2ord(mn));
+            atomicMax(&cb[3 + a], f2ord(mx));
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tlo, const float4* __restrict__ thi,
+                                                int64_t n, const uint32_t* __restrict__ cb,
+                                                uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 lo = tlo[i], hi = thi[i];
+    const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+    uint64_t code = 0;
+    for (int a = 0; a < 3; a++) {
+        const float mn = ord2f(cb[a]), mx = ord2f(cb[3 + a]);
+        const float ext = mx - mn;
+        float f = ext > 0.f ? (c[a] - mn) / ext : 0.5f;
+        f = fminf(fmaxf(f, 0.f), 1.f);
+        const uint64_t q = uint64_t(fminf(f * 2097152.0f, 2097151.0f));
+        code |= spread21(q) << (2 - a);
+    }
+    keys[i] = code;
+    vals[i] = int32_t(i);
+}
+
+// Binary node refs: >= 0 internal node index, < 0 leaf ~j (sorted position j).
+struct BNode {
+    int32_t left, right;
+    int32_t first, last;
+};
+
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clzll(uint64_t(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_lbvh(const uint64_t* __restrict__ k, int64_t n, BNode* __restrict__ bn,
+                                              int32_t* __restrict__ parent_int, int32_t* __restrict__ parent_leaf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(k, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int64_t l = 0;
+    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(k, n, i, j);
+    int64_t s = 0;
+    int64_t len = l;
+    int64_t t = (len + 1) >> 1;
+    while (true) {
+        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+        t = (t + 1) >> 1;
+    }
+    const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    BNode node;
+    node.first = int32_t(lo);
+    node.last = int32_t(hi);
+    if (lo == g) {
+        node.left = ~int32_t(g);
+        parent_leaf[g] = int32_t(i);
+    } else {
+        node.left = int32_t(g);
+        parent_int[g] = int32_t(i);
+    }
+    if (hi == g + 1) {
+        node.right = ~int32_t(g + 1);
+        parent_leaf[g + 1] = int32_t(i);
+    } else {
+        node.right = int32_t(g + 1);
+        parent_int[g + 1] = int32_t(i);
+    }
+    bn[i] = node;
+}
+
+// Bottom-up refit: one thread per leaf walks to the root; at each internal
+// node the first arrival leaves and the second merges both children.
+__global__ __launch_bounds__(256) void k_refit(int64_t n, const int32_t* __restrict__ order,
+                                               const float4* __restrict__ tlo, const float4* __restrict__ thi,
+                                               const BNode* __restrict__ bn, const int32_t* __restrict__ parent_int,
+                                               const int32_t* __restrict__ parent_leaf, float4* leaf_lo,
+                                               float4* leaf_hi, float4* node_lo, float4* node_hi,
+                                               int32_t* flags) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int32_t ti = order[j];
+    leaf_lo[j] = tlo[ti];
+    leaf_hi[j] = thi[ti];
+    if (n == 1) return;
+    __threadfence();
+    int32_t p = parent_leaf[j];
+    while (p >= 0) {
+        if (atomicAdd(&flags[p], 1) == 0) return;
+        __threadfence();
+        const BNode nd = bn[p];
+        const float4 l0 = nd.left < 0 ? leaf_lo[~nd.left] : node_lo[nd.left];
+        const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
+        const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
+        const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
+        node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+        node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+        __threadfence();
+        p = p == 0 ? -1 : parent_int[p];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reorder(int64_t n, const int32_t* __restrict__ order,
+                                                 const float* __restrict__ verts, TriV* __restrict__ tris) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int32_t ti = order[j];
+    const float* v = verts + 9 * int64_t(ti);
+    TriV t;
+    t.v0 = make_float4(v[0], v[1], v[2], __int_as_float(ti));
+    t.v1 = make_float4(v[3], v[4], v[5], 0.f);
+    t.v2 = make_float4(v[6], v[7], v[8], 0.f);
+    tris[j] = t;
+}
+
+__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+struct CollapseItem {
+    int32_t bref;   // binary ref (internal index or ~leaf)
+    int32_t wnode;  // wide node slot
+};
+
+// One work item per wide node: open children until W, then write the node
+// and emit the internal children as next-level items.
+template <int W>
+__global__ __launch_bounds__(256) void k_collapse(const CollapseItem* __restrict__ in, const int32_t* __restrict__ n_in,
+                                                  CollapseItem* __restrict__ out, int32_t* __restrict__ n_out,
+                                                  int32_t* __restrict__ node_counter, const BNode* __restrict__ bn,
+                                                  const float4* __restrict__ node_lo, const float4* __restrict__ node_hi,
+                                                  const float4* __restrict__ leaf_lo, const float4* __restrict__ leaf_hi,
+                                                  WNode<W>* __restrict__ wnodes, int leaf_size, double* sah_acc,
+                                                  int32_t* leaf_counter) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *n_in) return;
+    const CollapseItem it = in[idx];
+    int32_t ref[W];
+    int nc = 0;
+    auto cnt_of = [&](int32_t r) -> int { return r < 0 ? 1 : bn[r].last - bn[r].first + 1; };
+    auto lo_of = [&](int32_t r) { return r < 0 ? leaf_lo[~r] : node_lo[r]; };
+    auto hi_of = [&](int32_t r) { return r < 0 ? leaf_hi[~r] : node_hi[r]; };
+    if (it.bref < 0 || cnt_of(it.bref) <= leaf_size) {
+        ref[nc++] = it.bref;   // whole range is one leaf (tiny scenes)
+    } else {
+        ref[nc++] = bn[it.bref].left;
+        ref[nc++] = bn[it.bref].right;
+        while (nc < W) {
+            int best = -1;
+            float best_a = -1.f;
+            for (int c = 0; c < nc; c++) {
+                if (ref[c] >= 0 && cnt_of(ref[c]) > leaf_size) {
+                    const float a = half_area(lo_of(ref[c]), hi_of(ref[c]));
+                    if (a > best_a) { best_a = a; best = c; }
+                }
+            }
+            if (best < 0) break;
+            const BNode b = bn[ref[best]];
+            ref[best] = b.left;
+            ref[nc++] = b.right;
+        }
+    }
+    WNode<W> node;
+    double sah = 0.0;
+    int nleaf = 0;
+    for (int c = 0; c < W; c++) {
+        if (c < nc) {
+            const int32_t r = ref[c];
+            const float4 lo = lo_of(r), hi = hi_of(r);
+            node.lox[c] = lo.x; node.loy[c] = lo.y; node.loz[c] = lo.z;
+            node.hix[c] = hi.x; node.hiy[c] = hi.y; node.hiz[c] = hi.z;
+            const int cnt = cnt_of(r);
+            const double a = double(half_area(lo, hi));
+            if (r < 0 || cnt <= leaf_size) {
+                node.child[c] = r < 0 ? ~r : bn[r].first;
+                node.count[c] = cnt;
+                sah += a * cnt;
+                nleaf++;
+            } else {
+                const int32_t slot = atomicAdd(node_counter, 1);
+                node.child[c] = slot;
+                node.count[c] = 0;
+                sah += a;
+                const int32_t q = atomicAdd(n_out, 1);
+                out[q] = CollapseItem{r, slot};
+            }
+        } else {
+            node.lox[c] = node.loy[c] = node.loz[c] = 1e30f;
+            node.hix[c] = node.hiy[c] = node.hiz[c] = -1e30f;
+            node.child[c] = 0;
+            node.count[c] = -1;
+        }
+    }
+    wnodes[it.wnode] = node;
+    atomicAdd(sah_acc, sah);
+    atomicAdd(leaf_counter, nleaf);
+}
+
+// ---------------------------------------------------------------------------
+// ray-cast kernels
+// ---------------------------------------------------------------------------
+
+constexpr int kTraceBlock = 256;
+constexpr int kLdsDepth = 24;
+constexpr int kPrivDepth = 72;
+
+template <int W, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const WNode<W>* __restrict__ nodes,
+                                                        const TriV* __restrict__ tris,
+                                                        const lmgpu_ray* __restrict__ rays,
+                                                        lmgpu_hit* __restrict__ hits, uint8_t* __restrict__ occl,
+                                                        int64_t n, int32_t* __restrict__ status,
+                                                        unsigned long long* __restrict__ counters) {
+    __shared__ int lds[kLdsDepth][kTraceBlock];
+    // XCD-aware block order: hardware block b runs on XCD b % 8 (round-robin
+    // dispatch); give each XCD a contiguous range of logical blocks so
+    // neighbouring rays share that XCD's L2. XCD x owns q + (x < r) blocks
+    // starting at x*q + min(x, r), a bijection for any grid size.
+    const int b = blockIdx.x;
+    const int bq = gridDim.x >> 3, br = gridDim.x & 7;
+    const int x = b & 7, k = b >> 3;
+    const int lb = x * bq + min(x, br) + k;
+    const int64_t i = int64_t(lb) * kTraceBlock + threadIdx.x;
+    const bool valid = i < n;
+    TraceResult r;
+    r.node_visits = 0;
+    r.tri_tests = 0;
+    if (valid) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0], c = rp[1];
+        r = trace<W, ANY, COUNT, kLdsDepth, kPrivDepth, kTraceBlock>(nodes, tris, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z),
+                                                                      a.w, c.w, lds, threadIdx.x);
+        if (r.overflow) atomicOr(status, 1);
+        if (ANY) {
+            occl[i] = r.prim >= 0 ? 1 : 0;
+        } else {
+            float4 h;
+            h.x = __int_as_float(r.prim);
+            h.y = r.prim >= 0 ? r.t : 0.f;
+            h.z = r.u;
+            h.w = r.v;
+            reinterpret_cast<float4*>(hits)[i] = h;
+        }
+    }
+    if (COUNT) {
+        unsigned long long nv = valid ? r.node_visits : 0, nt = valid ? r.tri_tests : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            nv += __shfl_xor(nv, off);
+            nt += __shfl_xor(nt, off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&counters[0], nv);
+            atomicAdd(&counters[1], nt);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+template <int W>
+static void launch_collapse(const CollapseItem* in, const int32_t* n_in, CollapseItem* out, int32_t* n_out,
+                            int32_t* node_counter, const BNode* bn, const float4* node_lo, const float4* node_hi,
+                            const float4* leaf_lo, const float4* leaf_hi, void* wnodes, int leaf_size,
+                            double* sah, int32_t* leaves, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_collapse<W>, dim3(nblocks(count, 256)), dim3(256), 0, s, in, n_in, out, n_out, node_counter,
+                       bn, node_lo, node_hi, leaf_lo, leaf_hi, static_cast<WNode<W>*>(wnodes), leaf_size, sah,
+                       leaves);
+}
+
+int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_build_opts* opts, hipStream_t s) {
+    int W = opts && opts->width ? opts->width : 4;
+    int leaf = opts && opts->leaf_size ? opts->leaf_size : 4;
+    if (W != 2 && W != 4 && W != 8) return fail(LMGPU_ERR_INVALID_ARGUMENT, "BVH width must be 2, 4 or 8");
+    if (leaf < 1 || leaf > 15) return fail(LMGPU_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 15]");
+    if (n <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel build needs at least one triangle");
+    if (n >= (int64_t(1) << 27)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "too many triangles (max 2^27)");
+    if (!verts_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "verts is NULL");
+    release();
+    width = W;
+    leaf_size = leaf;
+    num_tris = n;
+    DevBuf<float> dverts;
+    const float* verts = verts_in;
+    if (!on_device) {
+        LMGPU_TRY(dverts.alloc(size_t(n) * 9));
+        LMGPU_HIP(hipMemcpyAsync(dverts.p, verts_in, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+        verts = dverts.p;
+    }
+    hipEvent_t e0, e1;
+    LMGPU_HIP(hipEventCreate(&e0));
+    LMGPU_HIP(hipEventCreate(&e1));
+    LMGPU_HIP(hipEventRecord(e0, s));
+
+    DevBuf<float4> tlo, thi, leaf_lo, leaf_hi, node_lo, node_hi;
+    DevBuf<uint32_t> cb;
+    DevBuf<uint64_t> keys, keys2;
+    DevBuf<int32_t> vals, vals2, parent_int, parent_leaf, flags, counters;
+    DevBuf<BNode> bn;
+    DevBuf<CollapseItem> q0, q1;
+    DevBuf<double> sah;
+    LMGPU_TRY(tlo.alloc(n)); LMGPU_TRY(thi.alloc(n));
+    LMGPU_TRY(leaf_lo.alloc(n)); LMGPU_TRY(leaf_hi.alloc(n));
+    LMGPU_TRY(node_lo.alloc(std::max<int64_t>(n - 1, 1))); LMGPU_TRY(node_hi.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(cb.alloc(6));
+    LMGPU_TRY(keys.alloc(n)); LMGPU_TRY(keys2.alloc(n));
+    LMGPU_TRY(vals.alloc(n)); LMGPU_TRY(vals2.alloc(n));
+    LMGPU_TRY(parent_int.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(parent_leaf.alloc(n));
+    LMGPU_TRY(flags.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(counters.alloc(8));
+    LMGPU_TRY(bn.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(q0.alloc(n)); LMGPU_TRY(q1.alloc(n));
+    LMGPU_TRY(sah.alloc(1));
+
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, cb.p);
+    hipLaunchKernelGGL(k_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, verts, n, tlo.p, thi.p, cb.p);
+    hipLaunchKernelGGL(k_morton, dim3(nblocks(n, 256)), dim3(256), 0, s, tlo.p, thi.p, n, cb.p, keys.p, vals.p);
+    size_t temp_bytes = 0;
+    LMGPU_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
+    DevBuf<char> temp;
+    LMGPU_TRY(temp.alloc(std::max<size_t>(temp_bytes, 16)));
+    LMGPU_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
+    if (n > 1) {
+        LMGPU_HIP(hipMemsetAsync(flags.p, 0, sizeof(int32_t) * (n - 1), s));
+        LMGPU_HIP(hipMemsetAsync(parent_int.p, 0xff, sizeof(int32_t) * (n - 1), s));
+        hipLaunchKernelGGL(k_lbvh, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, keys2.p, n, bn.p, parent_int.p,
+                           parent_leaf.p);
+    }
+    hipLaunchKernelGGL(k_refit, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, tlo.p, thi.p, bn.p,
+                       parent_int.p, parent_leaf.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p);
+    LMGPU_TRY(tris.alloc(n));
+    hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, verts, tris.p);
+
+    // collapse, level by level
+    const size_t node_bytes = size_t(32) * W;
+    const int64_t max_nodes = std::max<int64_t>(n, 1);
+    LMGPU_TRY(nodes.alloc(size_t(max_nodes) * node_bytes / sizeof(float4)));
+    const CollapseItem root{n > 1 ? 0 : ~0, 0};
+    LMGPU_HIP(hipMemcpyAsync(q0.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    int32_t init[4] = {1, 0, 1, 0};   // n_in, n_out, node_counter, leaves
+    LMGPU_HIP(hipMemcpyAsync(counters.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    LMGPU_HIP(hipMemsetAsync(sah.p, 0, sizeof(double), s));
+    int64_t count = 1;
+    int levels = 0;
+    CollapseItem* qin = q0.p;
+    CollapseItem* qout = q1.p;
+    while (count > 0) {
+        levels++;
+        LMGPU_HIP(hipMemsetAsync(counters.p + 1, 0, sizeof(int32_t), s));
+        if (W == 2)
+            launch_collapse<2>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
+                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+        else if (W == 4)
+            launch_collapse<4>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
+                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+        else
+            launch_collapse<8>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
+                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+        int32_t nout = 0;
+        LMGPU_HIP(hipMemcpyAsync(&nout, counters.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LMGPU_HIP(hipStreamSynchronize(s));
+        LMGPU_HIP(hipMemcpyAsync(counters.p, counters.p + 1, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        count = nout;
+        std::swap(qin, qout);
+        if (levels > 4096) return fail(LMGPU_ERR_DEVICE, "BVH collapse did not terminate");
+    }
+    LMGPU_HIP(hipEventRecord(e1, s));
+    int32_t fin[4];
+    double sah_sum = 0;
+    LMGPU_HIP(hipMemcpyAsync(fin, counters.p, sizeof(fin), hipMemcpyDeviceToHost, s));
+    LMGPU_HIP(hipMemcpyAsync(&sah_sum, sah.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    LMGPU_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    LMGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // root half-area for the SAH normalisation
+    float4 rlo, rhi;
+    if (n > 1) {
+        LMGPU_HIP(hipMemcpy(&rlo, node_lo.p, sizeof(float4), hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&rhi, node_hi.p, sizeof(float4), hipMemcpyDeviceToHost));
+    } else {
+        LMGPU_HIP(hipMemcpy(&rlo, leaf_lo.p, sizeof(float4), hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&rhi, leaf_hi.p, sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    const double ra = double(rhi.x - rlo.x) * (rhi.y - rlo.y) + double(rhi.y - rlo.y) * (rhi.z - rlo.z) +
+                      double(rhi.z - rlo.z) * (rhi.x - rlo.x);
+    num_nodes = fin[2];
+    num_leaves = fin[3];
+    depth = levels;
+    sah_cost = ra > 0 ? 1.0 + sah_sum / ra : 1.0;
+    build_ms = ms;
+    root_lo[0] = rlo.x; root_lo[1] = rlo.y; root_lo[2] = rlo.z;
+    root_hi[0] = rhi.x; root_hi[1] = rhi.y; root_hi[2] = rhi.z;
+    LMGPU_HIP(hipGetLastError());
+    built = true;
+    return LMGPU_OK;
+}
+
+template <int W, bool ANY, bool COUNT>
+static void launch_trace(const Accel& a, const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n,
+                         int32_t* status, unsigned long long* counters, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace<W, ANY, COUNT>), dim3(nblocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s,
+                       reinterpret_cast<const WNode<W>*>(a.nodes.p), a.tris.p, rays, hits, occl, n, status, counters);
+}
+
+int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
+                 unsigned long long* counters, hipStream_t s) const {
+    if (!built) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel is not built");
+    if (n < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return LMGPU_OK;
+    if (!rays || (!any && !hits) || (any && !occl))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL ray or result buffer");
+    if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15)))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "rays and hits must be 16-byte aligned");
+    if (n > int64_t(0x7fffffff) * kTraceBlock / 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "too many rays");
+    int32_t* st = status_dev();
+    if (!st) return fail(LMGPU_ERR_OUT_OF_MEMORY, "status buffer");
+#define LMGPU_DISPATCH(WW)                                                                    \
+    if (width == WW) {                                                                        \
+        if (any) launch_trace<WW, true, false>(*this, rays, hits, occl, n, st, counters, s);  \
+        else if (counters) launch_trace<WW, false, true>(*this, rays, hits, occl, n, st, counters, s); \
+        else launch_trace<WW, false, false>(*this, rays, hits, occl, n, st, counters, s);     \
+    }
+    LMGPU_DISPATCH(2)
+    LMGPU_DISPATCH(4)
+    LMGPU_DISPATCH(8)
+#undef LMGPU_DISPATCH
+    LMGPU_HIP(hipGetLastError());
+    return LMGPU_OK;
+}
+
+}  // namespace lmgpu

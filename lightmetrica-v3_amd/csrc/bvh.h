@@ -1,0 +1,292 @@
+// Wide BVH layout and closest-hit / any-hit traversal (device functions).
+//
+// Replaces the traversal of accel::sahbvh (src/accel/accel_sahbvh.cpp:269-299:
+// FP64 slab test, unordered int[99] stack, Moller-Trumbore) with:
+//   * a W-wide BVH (W = 2, 4, 8) whose node is 32*W bytes laid out SoA
+//     (lo.x[W], lo.y[W], lo.z[W], hi.x[W], hi.y[W], hi.z[W], child[W], count[W]);
+//     W = 4 is exactly one 128-B line,
+//   * nearest-first ordered traversal with a per-lane stack in LDS
+//     ([depth][lane], bank-conflict free) spilling to private memory,
+//   * the watertight FP32 ray/triangle test of Woop, Benthin and Wald
+//     (JCGT 2(1), 2013), with the reference's closed edges, closed
+//     [tmin, tmax] range and |det| < 1e-8 rejection (accel_sahbvh.cpp:67-71).
+#pragma once
+
+#include "common.h"
+#include "../../include/lmgpu.h"
+
+namespace lmgpu {
+
+template <int W>
+struct alignas(16) WNode {
+    float lox[W], loy[W], loz[W];
+    float hix[W], hiy[W], hiz[W];
+    int32_t child[W];   // internal: wide node index; leaf: first triangle (sorted order)
+    int32_t count[W];   // 0 internal, > 0 leaf triangle count, < 0 empty slot
+};
+static_assert(sizeof(WNode<4>) == 128, "BVH4 node must be one 128-B line");
+
+// Triangle in traversal order: 3 x float4 = 48 B; v0.w holds the original
+// (flattened) triangle index as raw bits.
+struct alignas(16) TriV {
+    float4 v0, v1, v2;
+};
+
+// Stack entries: internal node index (>= 0) or an encoded leaf (< 0).
+LMGPU_HD int encode_leaf(int first, int count) { return -((first << 4) | count) - 1; }
+LMGPU_HD int leaf_first(int e) { return (-(e + 1)) >> 4; }
+LMGPU_HD int leaf_count(int e) { return (-(e + 1)) & 15; }
+
+struct RayPre {
+    f3 o, d, inv;
+    float Sx, Sy, Sz;
+    int kx, ky, kz;
+    float dkz_abs;   // |d[kz]|, relates the watertight det to the MT det
+};
+
+__device__ __forceinline__ RayPre ray_prepare(f3 o, f3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    // Reciprocal direction; a zero component becomes a tiny signed value so
+    // the slab distances are +-huge rather than NaN.
+    const float tiny = 1e-30f;
+    const float dx = fabsf(d.x) < tiny ? copysignf(tiny, d.x) : d.x;
+    const float dy = fabsf(d.y) < tiny ? copysignf(tiny, d.y) : d.y;
+    const float dz = fabsf(d.z) < tiny ? copysignf(tiny, d.z) : d.z;
+    r.inv = mk(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float dkz = comp(d, kz);
+    if (dkz < 0.f) {   // keep the winding of the projected triangle
+        const int t = kx; kx = ky; ky = t;
+    }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(d, kx) / dkz;
+    r.Sy = comp(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    r.dkz_abs = fabsf(dkz);
+    return r;
+}
+
+// Watertight ray/triangle test. On acceptance (t in [tmin, tmax], closed
+// edges, |det_MT| >= 1e-8 as the reference), writes t and the barycentric
+// weights (u, v) of v1 and v2.
+__device__ __forceinline__ bool tri_intersect(const RayPre& r, float4 p0, float4 p1, float4 p2, float tmin,
+                                              float tmax, float& t, float& u, float& v) {
+    const f3 A = mk(p0.x - r.o.x, p0.y - r.o.y, p0.z - r.o.z);
+    const f3 B = mk(p1.x - r.o.x, p1.y - r.o.y, p1.z - r.o.z);
+    const f3 C = mk(p2.x - r.o.x, p2.y - r.o.y, p2.z - r.o.z);
+    const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    const float Ax = fmaf(-r.Sx, Akz, comp(A, r.kx));
+    const float Ay = fmaf(-r.Sy, Akz, comp(A, r.ky));
+    const float Bx = fmaf(-r.Sx, Bkz, comp(B, r.kx));
+    const float By = fmaf(-r.Sy, Bkz, comp(B, r.ky));
+    const float Cx = fmaf(-r.Sx, Ckz, comp(C, r.kx));
+    const float Cy = fmaf(-r.Sy, Ckz, comp(C, r.ky));
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float Wt = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || Wt == 0.0f) {
+        // Exact edge: recompute the edge functions in FP64 (Woop et al., sec. 3).
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        Wt = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.f || V < 0.f || Wt < 0.f) && (U > 0.f || V > 0.f || Wt > 0.f)) return false;
+    const float det = U + V + Wt;
+    if (det == 0.f) return false;
+    // |det_MT| = |d . (e1 x e2)| = |det| |d[kz]| for a unit-length direction
+    if (fabsf(det) * r.dkz_abs < 1e-8f) return false;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + Wt * Cz;
+    const float sdet = fabsf(det);
+    const float sT = det < 0.f ? -T : T;
+    if (sT < tmin * sdet || sT > tmax * sdet) return false;
+    const float rcp = 1.0f / det;
+    t = T * rcp;
+    u = V * rcp;
+    v = Wt * rcp;
+    return true;
+}
+
+// Slab test of one child box; conservative against FP32 rounding by scaling
+// the far distance with 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal",
+// JCGT 2(2), 2013).
+__device__ __forceinline__ float box_near(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                                          float hz, float tmin, float tmax, bool& hit) {
+    const float t1x = (lx - r.o.x) * r.inv.x, t2x = (hx - r.o.x) * r.inv.x;
+    const float t1y = (ly - r.o.y) * r.inv.y, t2y = (hy - r.o.y) * r.inv.y;
+    const float t1z = (lz - r.o.z) * r.inv.z, t2z = (hz - r.o.z) * r.inv.z;
+    const float tn = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fmaxf(fminf(t1z, t2z), tmin));
+    const float tf = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fminf(fmaxf(t1z, t2z), tmax));
+    hit = tn <= tf * 1.00000036f;
+    return tn;
+}
+
+// Per-lane traversal stack: the first LDS_DEPTH entries live in LDS,
+// [entry][lane] so a wave's accesses hit distinct banks, the rest in
+// private memory (only reached by deep, rare traversals).
+template <int LDS_DEPTH, int PRIV_DEPTH, int BLOCK>
+struct Stack {
+    int (*lds)[BLOCK];
+    int priv[PRIV_DEPTH];
+    int sp;
+    int lane;
+    static constexpr int capacity = LDS_DEPTH + PRIV_DEPTH;
+    __device__ __forceinline__ bool push(int e) {
+        if (sp < LDS_DEPTH) lds[sp][lane] = e;
+        else if (sp < capacity) priv[sp - LDS_DEPTH] = e;
+        else return false;
+        sp++;
+        return true;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        return sp < LDS_DEPTH ? lds[sp][lane] : priv[sp - LDS_DEPTH];
+    }
+};
+
+struct TraceResult {
+    int prim;
+    float t, u, v;
+    int best;   // index of the hit triangle in traversal order (-1 if none)
+    bool overflow;
+    int node_visits, tri_tests;
+};
+
+// Hit record of the accepted triangle re-evaluated once in FP64 with the
+// reference's Moller-Trumbore formula (accel_sahbvh.cpp:58-75), whose Hit
+// carries FP64 t and uv. The hit decision itself stays the FP32 watertight
+// test; this only removes the FP32 conditioning error of t at grazing
+// incidence (error ~ eps * edge / (t * cos)).
+__device__ __forceinline__ void refine_hit(const TriV& tv, f3 o, f3 d, float& t, float& u, float& v) {
+    const double p1x = tv.v0.x, p1y = tv.v0.y, p1z = tv.v0.z;
+    const double e1x = double(tv.v1.x) - p1x, e1y = double(tv.v1.y) - p1y, e1z = double(tv.v1.z) - p1z;
+    const double e2x = double(tv.v2.x) - p1x, e2y = double(tv.v2.y) - p1y, e2z = double(tv.v2.z) - p1z;
+    const double dx = d.x, dy = d.y, dz = d.z;
+    const double px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
+    const double tx = double(o.x) - p1x, ty = double(o.y) - p1y, tz = double(o.z) - p1z;
+    const double qx = ty * e1z - tz * e1y, qy = tz * e1x - tx * e1z, qz = tx * e1y - ty * e1x;
+    const double det = e1x * px + e1y * py + e1z * pz;
+    if (det == 0.0) return;
+    const double inv = 1.0 / det;
+    t = float((e2x * qx + e2y * qy + e2z * qz) * inv);
+    u = float((tx * px + ty * py + tz * pz) * inv);
+    v = float((dx * qx + dy * qy + dz * qz) * inv);
+}
+
+// Sorting network helper: orders (k, e) pairs by k ascending.
+__device__ __forceinline__ void cswap(float& ka, int& ea, float& kb, int& eb) {
+    const bool s = kb < ka;
+    const float tk = s ? kb : ka;
+    const int te = s ? eb : ea;
+    kb = s ? ka : kb;
+    eb = s ? ea : eb;
+    ka = tk;
+    ea = te;
+}
+
+template <int W>
+__device__ __forceinline__ void sort_children(float (&k)[W], int (&e)[W]) {
+    if constexpr (W == 2) {
+        cswap(k[0], e[0], k[1], e[1]);
+    } else if constexpr (W == 4) {
+        cswap(k[0], e[0], k[1], e[1]);
+        cswap(k[2], e[2], k[3], e[3]);
+        cswap(k[0], e[0], k[2], e[2]);
+        cswap(k[1], e[1], k[3], e[3]);
+        cswap(k[1], e[1], k[2], e[2]);
+    } else {
+        // Batcher odd-even merge sort network for 8 inputs (19 comparators).
+        cswap(k[0], e[0], k[1], e[1]); cswap(k[2], e[2], k[3], e[3]);
+        cswap(k[4], e[4], k[5], e[5]); cswap(k[6], e[6], k[7], e[7]);
+        cswap(k[0], e[0], k[2], e[2]); cswap(k[1], e[1], k[3], e[3]);
+        cswap(k[4], e[4], k[6], e[6]); cswap(k[5], e[5], k[7], e[7]);
+        cswap(k[1], e[1], k[2], e[2]); cswap(k[5], e[5], k[6], e[6]);
+        cswap(k[0], e[0], k[4], e[4]); cswap(k[1], e[1], k[5], e[5]);
+        cswap(k[2], e[2], k[6], e[6]); cswap(k[3], e[3], k[7], e[7]);
+        cswap(k[2], e[2], k[4], e[4]); cswap(k[3], e[3], k[5], e[5]);
+        cswap(k[1], e[1], k[2], e[2]); cswap(k[3], e[3], k[4], e[4]);
+        cswap(k[5], e[5], k[6], e[6]);
+    }
+}
+
+// Closest-hit (ANY = false) or any-hit (ANY = true) traversal from the root
+// (wide node 0). Equal t: the first accepted triangle is replaced by a later
+// one with t == tmax, as the reference's closed [tmin, tmax] test does.
+template <int W, bool ANY, bool COUNT, int LDS_DEPTH, int PRIV_DEPTH, int BLOCK>
+__device__ __forceinline__ TraceResult trace(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+                                             f3 o, f3 d, float tmin, float tmax,
+                                             int (*lds)[BLOCK], int lane) {
+    const RayPre r = ray_prepare(o, d);
+    Stack<LDS_DEPTH, PRIV_DEPTH, BLOCK> st;
+    st.lds = lds;
+    st.sp = 0;
+    st.lane = lane;
+    TraceResult res;
+    res.prim = -1;
+    res.t = tmax;
+    res.u = res.v = 0.f;
+    res.best = -1;
+    res.overflow = false;
+    res.node_visits = 0;
+    res.tri_tests = 0;
+    int cur = 0;
+    while (true) {
+        if (cur >= 0) {
+            const WNode<W>& nd = nodes[cur];
+            if (COUNT) res.node_visits++;
+            float key[W];
+            int ent[W];
+#pragma unroll
+            for (int c = 0; c < W; c++) {
+                bool hit;
+                const float tn = box_near(r, nd.lox[c], nd.loy[c], nd.loz[c], nd.hix[c], nd.hiy[c], nd.hiz[c],
+                                          tmin, res.t, hit);
+                const int cnt = nd.count[c];
+                const int ch = nd.child[c];
+                hit = hit && cnt >= 0;
+                key[c] = hit ? tn : __builtin_inff();
+                ent[c] = cnt == 0 ? ch : encode_leaf(ch, cnt);
+            }
+            sort_children<W>(key, ent);
+            // push hit children farthest first, continue with the nearest
+            bool ok = true;
+#pragma unroll
+            for (int c = W - 1; c >= 1; c--) {
+                if (key[c] != __builtin_inff()) ok = ok && st.push(ent[c]);
+            }
+            if (!ok) { res.overflow = true; break; }
+            if (key[0] != __builtin_inff()) {
+                cur = ent[0];
+                continue;
+            }
+        } else {
+            const int first = leaf_first(cur), cnt = leaf_count(cur);
+            bool done = false;
+            for (int k = 0; k < cnt; k++) {
+                const TriV tv = tris[first + k];
+                if (COUNT) res.tri_tests++;
+                float t, u, v;
+                if (tri_intersect(r, tv.v0, tv.v1, tv.v2, tmin, res.t, t, u, v)) {
+                    res.t = t;
+                    res.u = u;
+                    res.v = v;
+                    res.prim = __float_as_int(tv.v0.w);
+                    res.best = first + k;
+                    if (ANY) { done = true; break; }
+                }
+            }
+            if (done) break;
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    if (!ANY && res.best >= 0) refine_hit(tris[res.best], o, d, res.t, res.u, res.v);
+    return res;
+}
+
+}  // namespace lmgpu

@@ -1,0 +1,817 @@
+// Wavefront renderer::pt for gfx950 (C ABI: lmgpu_scene_*, lmgpu_pt_render).
+//
+// Restates Renderer_PT::render (src/renderer/renderer_pt.cpp:80-315) in
+// pixel-sampling mode with scheduler::spp::{sample,time}
+// (src/scheduler.cpp:37-106) as a wavefront path tracer. One pass = one
+// sample for every pixel; path p is pixel p. A pass is
+//   generate  camera rays (camera_pinhole.cpp:128-132), path state reset
+//   repeat max_verts-1 times:
+//     extend   closest hit of the active extension rays (bvh.h)
+//     shade    emitter hit + MIS (renderer_pt.cpp:236-271), env termination,
+//              Russian roulette (:281-287), component sampling (:292), NEE
+//              light sampling (:142-192) -> shadow queue, BSDF direction
+//              sampling (:197-229) -> next extension queue
+//     connect  any-hit visibility of the shadow queue; unoccluded
+//              contributions added to the path radiance
+//   accumulate per-pixel FP64 sum and sum of squares (Film_Bitmap is FP64,
+//   src/film/film_bitmap.cpp:19-55)
+// Queues are SoA arrays appended with one atomic per wave (ballot + popcount);
+// counts live in device memory so the whole pass is captured once as a
+// hipGraph and replayed per sample.
+//
+// Random numbers: Philox4x32-10 keyed by the seed with counter
+// (pixel, sample, vertex, block); see common.h. All transport math is FP32.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "lmgpu_internal.h"
+#include "shade.h"
+
+namespace lmgpu {
+
+// Triangle shading record, original (flattened) order: positions, vertex
+// normals; p0.w = primitive index bits.
+struct alignas(16) ShadeTri {
+    float4 p0, p1, p2, n0, n1, n2;
+};
+
+struct alignas(16) DevMat {
+    float4 a;   // rgb, type (as float)
+    float4 b;   // ax, ay, ni, light index (as float, -1 = none)
+};
+
+struct alignas(16) DevLight {
+    float4 Ke_invA;   // Ke rgb, 1/area
+    int32_t cdf_off, count, pad0, pad1;
+};
+
+struct SceneDev {
+    const WNode<4>* nodes4;
+    const WNode<2>* nodes2;
+    const WNode<8>* nodes8;
+    const TriV* tris;
+    const ShadeTri* stri;
+    const DevMat* mats;
+    const DevLight* lights;
+    const float* light_cdf;    // per light: count+1 entries, c[0] = 0, normalised
+    const int32_t* light_tri;  // per light: original triangle indices (face order)
+    int num_area_lights;
+    int num_lights;            // area + env
+    int has_env;
+    f3 env_Le;
+    Camera cam;
+    int w, h;
+};
+
+// Path state, SoA over pixels.
+struct PathState {
+    float4* thr;    // throughput xyz
+    float4* L;      // radiance xyz
+    float4* prev;   // shading normal of the previous vertex xyz, pdf_bsdf (projected SA) w
+    int4* info;     // nv, comp, flags (bit0 prev samplable by NEE, bit1 prev degenerate), -
+};
+
+struct Queues {
+    lmgpu_ray* ext[2];
+    lmgpu_hit* hit;
+    int32_t* ext_path[2];
+    lmgpu_ray* sh;
+    float4* sh_contrib;   // xyz contribution, w = path index bits
+    int32_t* counts;      // [0],[1] extension queues, [2] shadow queue, [3] sample index, [4..] stats
+};
+
+struct PTConsts {
+    int max_verts;
+    int mode;
+    int rr_depth;
+    uint32_t seed_lo, seed_hi;
+    int npix;
+};
+
+constexpr int kBlock = 256;
+constexpr int kLds = 24;
+constexpr int kPriv = 72;
+
+__device__ __forceinline__ Mat load_mat(const DevMat* mats, int prim, int& light) {
+    const DevMat d = mats[prim];
+    Mat m;
+    m.type = int(d.a.w);
+    m.c = mk(d.a.x, d.a.y, d.a.z);
+    m.ax = d.b.x;
+    m.ay = d.b.y;
+    m.ni = d.b.z;
+    light = int(d.b.w);
+    return m;
+}
+
+// Per-wave append: returns this lane's slot (lanes with want = false get -1).
+__device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll(mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long lower = mask & ((1ull << lane) - 1ull);
+    return want ? base + __popcll(lower) : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathState ps, Queues q, PTConsts c) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= c.npix) return;
+    const uint32_t sample = uint32_t(q.counts[3]);
+    const int x = p % sc.w, y = p / sc.w;
+    const rand4 r = rng(uint32_t(p), sample, 0u, 0u, (uint64_t(c.seed_hi) << 32) | c.seed_lo);
+    const float dx = 1.f / sc.w, dy = 1.f / sc.h;
+    const f3 d = primary_dir(sc.cam, dx * x + dx * r.u[2], dy * y + dy * r.u[3]);
+    ps.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.prev[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.info[p] = make_int4(1, 0, 2, 0);   // nv = 1, comp 0, prev degenerate (camera), not NEE-samplable
+    lmgpu_ray ray;
+    ray.org[0] = sc.cam.pos.x; ray.org[1] = sc.cam.pos.y; ray.org[2] = sc.cam.pos.z;
+    ray.tmin = kEps;
+    ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+    ray.tmax = kInf;
+    q.ext[0][p] = ray;
+    q.ext_path[0][p] = p;
+    if (p == 0) {
+        q.counts[0] = c.npix;
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npix);
+    }
+}
+
+__global__ void k_reset_counts(int32_t* counts, int out_q) {
+    if (threadIdx.x == 0) {
+        counts[out_q] = 0;
+        counts[2] = 0;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ const WNode<W>* nodes_of(const SceneDev& sc);
+template <>
+__device__ __forceinline__ const WNode<2>* nodes_of<2>(const SceneDev& sc) { return sc.nodes2; }
+template <>
+__device__ __forceinline__ const WNode<4>* nodes_of<4>(const SceneDev& sc) { return sc.nodes4; }
+template <>
+__device__ __forceinline__ const WNode<8>* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = q.counts[in_q];
+    const lmgpu_ray* rays = q.ext[in_q];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0], b = rp[1];
+        const TraceResult r = trace<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, mk(a.x, a.y, a.z),
+                                                                          mk(b.x, b.y, b.z), a.w, b.w, lds,
+                                                                          threadIdx.x);
+        reinterpret_cast<float4*>(q.hit)[i] =
+            make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = q.counts[2];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4* rp = reinterpret_cast<const float4*>(q.sh + i);
+        const float4 a = rp[0], b = rp[1];
+        const TraceResult r = trace<W, true, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, mk(a.x, a.y, a.z),
+                                                                         mk(b.x, b.y, b.z), a.w, b.w, lds,
+                                                                         threadIdx.x);
+        if (r.overflow) atomicOr(q.counts + 5, 1);
+        if (r.prim < 0 && !r.overflow) {
+            const float4 c = q.sh_contrib[i];
+            const int p = __float_as_int(c.w);
+            float4 L = ps.L[p];
+            L.x += c.x; L.y += c.y; L.z += c.z;
+            ps.L[p] = L;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 12), (unsigned long long)n);
+}
+
+// pdf of light endpoint `light` seen from the previous vertex (prev normal
+// np, degenerate flag) along d (unit, from the previous vertex to the light),
+// including the uniform selection probability (path::pdf_direct, path.h:563-583).
+__device__ __forceinline__ float pdf_light_area(const SceneDev& sc, int light, f3 np, bool prev_degen, f3 d,
+                                                float dist2, f3 nL) {
+    const float psel = 1.f / sc.num_lights;
+    const float cL = dot(-d, nL);
+    if (cL <= 0.f) return 0.f;
+    const float c1 = prev_degen ? 1.f : fabsf(dot(np, d));
+    const float G = c1 * fabsf(cL) / dist2;
+    if (G == 0.f) return 0.f;
+    return psel * sc.lights[light].Ke_invA.w / G;
+}
+
+__device__ __forceinline__ float pdf_light_env(const SceneDev& sc, f3 np, bool prev_degen, f3 d) {
+    const float psel = 1.f / sc.num_lights;
+    const float pSA = 1.f / (4.f * kPi);
+    if (prev_degen) return psel * pSA;
+    const float J = fabsf(dot(np, d));
+    return J == 0.f ? 0.f : psel * pSA / J;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Queues q, PTConsts c, int in_q) {
+    const int n = q.counts[in_q];
+    const int out_q = in_q ^ 1;
+    const uint32_t sample = uint32_t(q.counts[3]);
+    const uint64_t seed = (uint64_t(c.seed_hi) << 32) | c.seed_lo;
+    const int stride = gridDim.x * kBlock;
+    // every lane of a wave runs the same number of iterations so the
+    // wave-level appends see the whole wave
+    const int n_iter = (n + stride - 1) / stride;
+    for (int it = 0; it < n_iter; it++) {
+        const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
+        bool active = i < n;
+        bool emit_ext = false, emit_sh = false;
+        lmgpu_ray ext_ray, sh_ray;
+        float4 sh_c = make_float4(0, 0, 0, 0);
+        int p = 0;
+        if (active) {
+            p = q.ext_path[in_q][i];
+            const float4* rp = reinterpret_cast<const float4*>(q.ext[in_q] + i);
+            const float4 ra = rp[0], rb = rp[1];
+            const f3 o = mk(ra.x, ra.y, ra.z), wo = mk(rb.x, rb.y, rb.z);
+            const float4 h = reinterpret_cast<const float4*>(q.hit)[i];
+            const int tri = __float_as_int(h.x);
+            float4 thr4 = ps.thr[p];
+            float4 L4 = ps.L[p];
+            const float4 pv = ps.prev[p];
+            int4 info = ps.info[p];
+            f3 thr = mk(thr4.x, thr4.y, thr4.z);
+            f3 L = mk(L4.x, L4.y, L4.z);
+            const f3 np = mk(pv.x, pv.y, pv.z);
+            const bool prev_nee = info.z & 1;
+            const bool prev_degen = info.z & 2;
+            const bool direct_ok = c.mode == LMGPU_SAMPLING_NEE ? !prev_nee : true;
+            const bool use_mis = c.mode != LMGPU_SAMPLING_NAIVE && prev_nee;
+            if (tri == -2) atomicOr(q.counts + 5, 1);
+            if (tri < 0) {
+                // miss: environment light endpoint (Scene_::intersect, scene.cpp:321-332)
+                if (sc.has_env && direct_ok) {
+                    float mis = 1.f;
+                    if (use_mis) mis = balance_heuristic(pv.w, pdf_light_env(sc, np, prev_degen, wo));
+                    L = L + thr * sc.env_Le * mis;
+                }
+            } else {
+                const ShadeTri st = sc.stri[tri];
+                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
+                const f3 P0 = mk(st.p0.x, st.p0.y, st.p0.z), P1 = mk(st.p1.x, st.p1.y, st.p1.z),
+                         P2 = mk(st.p2.x, st.p2.y, st.p2.z);
+                const f3 pos = P0 * b0 + P1 * b1 + P2 * b2;
+                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+                const f3 gn = normalize(cross(P1 - P0, P2 - P0));
+                const Geom g = make_on_surface(pos, nsh, gn);
+                const int prim = __float_as_int(st.p0.w);
+                int light;
+                const Mat m = load_mat(sc.mats, prim, light);
+                // emitter hit (renderer_pt.cpp:246-271)
+                if (light >= 0 && direct_ok) {
+                    const f3 woL = -wo;
+                    if (dot(woL, g.n) > 0.f) {
+                        const DevLight dl = sc.lights[light];
+                        float mis = 1.f;
+                        if (use_mis) {
+                            const f3 dd = pos - o;
+                            const float dist2 = dot(dd, dd);
+                            mis = balance_heuristic(pv.w, pdf_light_area(sc, light, np, prev_degen, wo, dist2, g.n));
+                        }
+                        L = L + thr * mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) * mis;
+                    }
+                }
+                // Russian roulette and component sampling at this vertex
+                const int nv = info.x;
+                const rand4 r0 = rng(uint32_t(p), sample, uint32_t(nv), 0u, seed);
+                bool alive = true;
+                if (nv > c.rr_depth) {
+                    const float qq = fmaxf(.2f, 1.f - maxc(thr));
+                    if (r0.u[0] < qq) alive = false;
+                    else thr = thr / (1.f - qq);
+                }
+                const f3 wi = -wo;
+                int comp = 0;
+                if (alive) {
+                    float cw;
+                    comp = mat_sample_component(m, g, wi, r0.u[1], cw);
+                    thr = thr * cw;
+                }
+                const int nv2 = nv + 1;
+                if (alive && nv2 < c.max_verts) {
+                    const bool spec = is_specular(m);
+                    const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
+                    if (nee) {
+                        // path::sample_direct (path.h:509-528)
+                        const rand4 r1 = rng(uint32_t(p), sample, uint32_t(nv), 1u, seed);
+                        const int li = min(max(int(r1.u[0] * sc.num_lights), 0), sc.num_lights - 1);
+                        const float psel = 1.f / sc.num_lights;
+                        f3 dirL;   // from the shading point toward the light
+                        float tmaxL = 0.f;
+                        f3 weight = mk(0, 0, 0);
+                        float p_light = 0.f;
+                        bool ok = false;
+                        if (li < sc.num_area_lights) {
+                            const DevLight dl = sc.lights[li];
+                            // Dist::sample: upper_bound on the CDF
+                            const float* cdf = sc.light_cdf + dl.cdf_off;
+                            int lo = 0, hi = dl.count + 1;
+                            const float u = r1.u[1];
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if (cdf[mid] <= u) lo = mid + 1;
+                                else hi = mid;
+                            }
+                            const int k = min(max(lo - 1, 0), dl.count - 1);
+                            const int lt = sc.light_tri[dl.cdf_off - li + k];
+                            const ShadeTri ls = sc.stri[lt];
+                            const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z),
+                                     C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
+                            const float s = sqrtf(fmaxf(0.f, r1.u[2]));
+                            const float bu = 1.f - s, bv = r1.u[3] * s;
+                            const f3 pL = A * (1.f - bu - bv) + B * bu + C * bv;
+                            const f3 nL = normalize(cross(B - A, C - A));
+                            const f3 wL = normalize(pos - pL);   // from the light to the point
+                            if (dot(wL, nL) > 0.f) {
+                                const f3 dd = pL - pos;
+                                const float d2 = dot(dd, dd);
+                                const float G = fabsf(dot(g.n, wL)) * fabsf(dot(nL, -wL)) / d2;
+                                if (G != 0.f) {
+                                    const float pA = dl.Ke_invA.w / G;
+                                    weight = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) / pA / psel;
+                                    p_light = psel * pA;
+                                    dirL = normalize(pL - pos);
+                                    tmaxL = sqrtf(d2) * (1.f - kEps);
+                                    ok = true;
+                                }
+                            }
+                        } else {
+                            const f3 wL = sample_uniform_sphere(r1.u[2], r1.u[3]);
+                            const float J = fabsf(dot(g.n, -wL));
+                            if (J != 0.f) {
+                                const float pL = (1.f / (4.f * kPi)) / J;
+                                weight = sc.env_Le / pL / psel;
+                                p_light = psel * pL;
+                                dirL = -wL;
+                                tmaxL = kInf - 1.f;
+                                ok = true;
+                            }
+                        }
+                        if (ok) {
+                            const f3 fs = mat_eval(m, g, wi, dirL) * sn_correction(g, wi, dirL);
+                            if (!is_zero(fs)) {
+                                float mis = 1.f;
+                                if (c.mode == LMGPU_SAMPLING_MIS) mis = balance_heuristic(p_light, mat_pdf(m, g, wi, dirL));
+                                const f3 C = thr * fs * weight * mis;
+                                if (!is_zero(C)) {
+                                    emit_sh = true;
+                                    sh_ray.org[0] = pos.x; sh_ray.org[1] = pos.y; sh_ray.org[2] = pos.z;
+                                    sh_ray.tmin = kEps;
+                                    sh_ray.dir[0] = dirL.x; sh_ray.dir[1] = dirL.y; sh_ray.dir[2] = dirL.z;
+                                    sh_ray.tmax = tmaxL;
+                                    sh_c = make_float4(C.x, C.y, C.z, __int_as_float(p));
+                                }
+                            }
+                        }
+                    }
+                    // path::sample_direction (path.h:410-419)
+                    f3 wo2, sw;
+                    if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw)) {
+                        sw = sw * sn_correction(g, wi, wo2);
+                        thr = thr * sw;
+                        if (!is_zero(thr)) {
+                            emit_ext = true;
+                            ext_ray.org[0] = pos.x; ext_ray.org[1] = pos.y; ext_ray.org[2] = pos.z;
+                            ext_ray.tmin = kEps;
+                            ext_ray.dir[0] = wo2.x; ext_ray.dir[1] = wo2.y; ext_ray.dir[2] = wo2.z;
+                            ext_ray.tmax = kInf;
+                            ps.prev[p] = make_float4(g.n.x, g.n.y, g.n.z, mat_pdf(m, g, wi, wo2));
+                            info = make_int4(nv2, comp, nee ? 1 : 0, 0);
+                            ps.info[p] = info;
+                            ps.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.f);
+                        }
+                    }
+                }
+            }
+            ps.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+        }
+        const int se = wave_append(q.counts + out_q, emit_ext);
+        if (emit_ext) {
+            q.ext[out_q][se] = ext_ray;
+            q.ext_path[out_q][se] = p;
+        }
+        const int ss = wave_append(q.counts + 2, emit_sh);
+        if (emit_sh) {
+            q.sh[ss] = sh_ray;
+            q.sh_contrib[ss] = sh_c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathState ps, double* __restrict__ sum,
+                                                       double* __restrict__ sumsq, int npix) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    const float4 L = ps.L[p];
+    sum[3 * p + 0] += double(L.x);
+    sum[3 * p + 1] += double(L.y);
+    sum[3 * p + 2] += double(L.z);
+    if (sumsq) {
+        sumsq[3 * p + 0] += double(L.x) * double(L.x);
+        sumsq[3 * p + 1] += double(L.y) * double(L.y);
+        sumsq[3 * p + 2] += double(L.z) * double(L.z);
+    }
+}
+
+__global__ void k_next_sample(int32_t* counts) {
+    if (threadIdx.x == 0) counts[3] += 1;
+}
+
+__global__ void k_set_sample(int32_t* counts, int s) {
+    if (threadIdx.x == 0) {
+        counts[3] = s;
+        counts[5] = 0;
+    }
+}
+
+__global__ void k_zero_f64(double* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+
+struct Scene {
+    Accel accel;
+    DevBuf<ShadeTri> stri;
+    DevBuf<DevMat> mats;
+    DevBuf<DevLight> lights;
+    DevBuf<float> light_cdf;
+    DevBuf<int32_t> light_tri;
+    SceneDev dev{};
+    int w = 0, h = 0;
+    // render workspace
+    int ws_npix = 0;
+    DevBuf<float4> thr, L, prev, shc;
+    DevBuf<int4> info;
+    DevBuf<lmgpu_ray> ext0, ext1, sh;
+    DevBuf<lmgpu_hit> hit;
+    DevBuf<int32_t> path0, path1, counts;
+    DevBuf<double> fsum, fsq;   // device film when the caller passes host pointers
+    hipGraphExec_t graph = nullptr;
+    int graph_key[4] = {0, 0, 0, 0};
+    uint64_t graph_seed = 0;
+    int graph_sumsq = -1;
+    double* graph_sum_ptr = nullptr;
+    double* graph_sq_ptr = nullptr;
+
+    ~Scene() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+
+    int create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts);
+    int ensure_workspace(int npix);
+    int enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream_t s);
+    int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, lmgpu_pt_stats* st,
+               hipStream_t s);
+};
+
+static inline float f_or(const float* p, int i) { return p ? p[i] : 0.f; }
+
+int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
+    if (!d) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene desc is NULL");
+    if (d->num_tris <= 0 || !d->verts || !d->normals || !d->tri_prim)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene needs triangles, normals and primitive ids");
+    if (d->num_prims <= 0 || !d->mat_type || !d->mat_param)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene needs materials");
+    if (d->w <= 0 || d->h <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film size must be positive");
+    if (int64_t(d->w) * d->h > (int64_t(1) << 28)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film too large");
+    const int64_t N = d->num_tris;
+    const int P = d->num_prims;
+    for (int64_t t = 0; t < N; t++)
+        if (d->tri_prim[t] < 0 || d->tri_prim[t] >= P)
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "tri_prim out of range");
+    for (int i = 0; i < P; i++)
+        if (d->mat_type[i] < 0 || d->mat_type[i] > 3) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad material type");
+    LMGPU_TRY(accel.build(d->verts, N, false, opts, nullptr));
+    w = d->w;
+    h = d->h;
+    // shading triangles
+    std::vector<ShadeTri> st(N);
+    for (int64_t t = 0; t < N; t++) {
+        const float* v = d->verts + 9 * t;
+        const float* n = d->normals + 9 * t;
+        int32_t prim = d->tri_prim[t];
+        float pb;
+        std::memcpy(&pb, &prim, 4);
+        st[t].p0 = make_float4(v[0], v[1], v[2], pb);
+        st[t].p1 = make_float4(v[3], v[4], v[5], 0.f);
+        st[t].p2 = make_float4(v[6], v[7], v[8], 0.f);
+        st[t].n0 = make_float4(n[0], n[1], n[2], 0.f);
+        st[t].n1 = make_float4(n[3], n[4], n[5], 0.f);
+        st[t].n2 = make_float4(n[6], n[7], n[8], 0.f);
+    }
+    LMGPU_TRY(stri.alloc(N));
+    LMGPU_HIP(hipMemcpy(stri.p, st.data(), sizeof(ShadeTri) * N, hipMemcpyHostToDevice));
+    // lights (light_area.cpp:62-74: area CDF over the light mesh's faces)
+    const int L = d->num_area_lights;
+    if (L < 0 || (L > 0 && !d->light_prims)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad light list");
+    std::vector<int> prim_light(P, -1);
+    std::vector<DevLight> hl(std::max(L, 1));
+    std::vector<float> cdf;
+    std::vector<int32_t> ltri;
+    std::vector<std::vector<int32_t>> per(L);
+    for (int l = 0; l < L; l++) {
+        const int pr = d->light_prims[l];
+        if (pr < 0 || pr >= P) return fail(LMGPU_ERR_INVALID_ARGUMENT, "light primitive out of range");
+        if (prim_light[pr] >= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "primitive listed twice as a light");
+        prim_light[pr] = l;
+    }
+    for (int64_t t = 0; t < N; t++) {
+        const int l = prim_light[d->tri_prim[t]];
+        if (l >= 0) per[l].push_back(int32_t(t));
+    }
+    // cdf_off indexes light_cdf (count+1 entries per light); light_tri uses
+    // cdf_off - l (count entries per light).
+    for (int l = 0; l < L; l++) {
+        if (per[l].empty()) return fail(LMGPU_ERR_INVALID_ARGUMENT, "area light without triangles");
+        const int off = int(cdf.size());
+        std::vector<double> c{0.0};
+        for (int32_t t : per[l]) {
+            const float* v = d->verts + 9 * int64_t(t);
+            const double e1[3] = {double(v[3]) - v[0], double(v[4]) - v[1], double(v[5]) - v[2]};
+            const double e2[3] = {double(v[6]) - v[0], double(v[7]) - v[1], double(v[8]) - v[2]};
+            const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                         cz = e1[0] * e2[1] - e1[1] * e2[0];
+            c.push_back(c.back() + std::sqrt(std::max(0.0, cx * cx + cy * cy + cz * cz)) * .5);
+        }
+        const double sum = c.back();
+        if (!(sum > 0)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "area light with zero area");
+        for (double x : c) cdf.push_back(float(x / sum));
+        for (int32_t t : per[l]) ltri.push_back(t);
+        const float* ke = d->prim_Ke ? d->prim_Ke + 3 * d->light_prims[l] : nullptr;
+        hl[l].Ke_invA = make_float4(f_or(ke, 0), f_or(ke, 1), f_or(ke, 2), float(1.0 / sum));
+        hl[l].cdf_off = off;
+        hl[l].count = int(per[l].size());
+    }
+    if (L + (d->has_env ? 1 : 0) == 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene has no light");
+    std::vector<DevMat> hm(P);
+    for (int i = 0; i < P; i++) {
+        const float* mp = d->mat_param + 8 * i;
+        hm[i].a = make_float4(mp[0], mp[1], mp[2], float(d->mat_type[i]));
+        hm[i].b = make_float4(mp[3], mp[4], mp[5], float(prim_light[i]));
+        if (d->mat_type[i] == LMGPU_MAT_GLOSSY && (mp[3] <= 0.f || mp[4] <= 0.f))
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "glossy roughness must be positive");
+        if (d->mat_type[i] == LMGPU_MAT_GLASS && !(mp[5] > 0.f))
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "glass Ni must be positive");
+    }
+    LMGPU_TRY(mats.alloc(P));
+    LMGPU_HIP(hipMemcpy(mats.p, hm.data(), sizeof(DevMat) * P, hipMemcpyHostToDevice));
+    LMGPU_TRY(lights.alloc(hl.size()));
+    LMGPU_HIP(hipMemcpy(lights.p, hl.data(), sizeof(DevLight) * hl.size(), hipMemcpyHostToDevice));
+    if (cdf.empty()) cdf.push_back(0.f);
+    if (ltri.empty()) ltri.push_back(0);
+    LMGPU_TRY(light_cdf.alloc(cdf.size()));
+    LMGPU_HIP(hipMemcpy(light_cdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
+    LMGPU_TRY(light_tri.alloc(ltri.size()));
+    LMGPU_HIP(hipMemcpy(light_tri.p, ltri.data(), sizeof(int32_t) * ltri.size(), hipMemcpyHostToDevice));
+    // camera::pinhole construct (camera_pinhole.cpp:86-96)
+    Camera cam;
+    {
+        const double px = d->cam_pos[0], py = d->cam_pos[1], pz = d->cam_pos[2];
+        double wx = px - d->cam_center[0], wy = py - d->cam_center[1], wz = pz - d->cam_center[2];
+        double l = std::sqrt(wx * wx + wy * wy + wz * wz);
+        wx /= l; wy /= l; wz /= l;
+        const double ux0 = d->cam_up[1] * wz - d->cam_up[2] * wy, uy0 = d->cam_up[2] * wx - d->cam_up[0] * wz,
+                     uz0 = d->cam_up[0] * wy - d->cam_up[1] * wx;
+        l = std::sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
+        if (!(l > 0)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "camera up is parallel to the view direction");
+        const double ux = ux0 / l, uy = uy0 / l, uz = uz0 / l;
+        const double vx = wy * uz - wz * uy, vy = wz * ux - wx * uz, vz = wx * uy - wy * ux;
+        cam.pos = mk(float(px), float(py), float(pz));
+        cam.u = mk(float(ux), float(uy), float(uz));
+        cam.v = mk(float(vx), float(vy), float(vz));
+        cam.w = mk(float(wx), float(wy), float(wz));
+        cam.tf = float(std::tan(double(d->vfov) * 3.14159265358979323846 / 180.0 * .5));
+        cam.aspect = d->aspect;
+    }
+    dev = SceneDev{};
+    if (accel.width == 2) dev.nodes2 = reinterpret_cast<const WNode<2>*>(accel.nodes.p);
+    if (accel.width == 4) dev.nodes4 = reinterpret_cast<const WNode<4>*>(accel.nodes.p);
+    if (accel.width == 8) dev.nodes8 = reinterpret_cast<const WNode<8>*>(accel.nodes.p);
+    dev.tris = accel.tris.p;
+    dev.stri = stri.p;
+    dev.mats = mats.p;
+    dev.lights = lights.p;
+    dev.light_cdf = light_cdf.p;
+    dev.light_tri = light_tri.p;
+    dev.num_area_lights = L;
+    dev.num_lights = L + (d->has_env ? 1 : 0);
+    dev.has_env = d->has_env ? 1 : 0;
+    dev.env_Le = mk(d->env_Le[0], d->env_Le[1], d->env_Le[2]);
+    dev.cam = cam;
+    dev.w = w;
+    dev.h = h;
+    return LMGPU_OK;
+}
+
+int Scene::ensure_workspace(int npix) {
+    if (ws_npix == npix) return LMGPU_OK;
+    LMGPU_TRY(thr.alloc(npix)); LMGPU_TRY(L.alloc(npix)); LMGPU_TRY(prev.alloc(npix)); LMGPU_TRY(info.alloc(npix));
+    LMGPU_TRY(ext0.alloc(npix)); LMGPU_TRY(ext1.alloc(npix)); LMGPU_TRY(sh.alloc(npix)); LMGPU_TRY(shc.alloc(npix));
+    LMGPU_TRY(hit.alloc(npix));
+    LMGPU_TRY(path0.alloc(npix)); LMGPU_TRY(path1.alloc(npix));
+    LMGPU_TRY(counts.alloc(16));
+    LMGPU_HIP(hipMemset(counts.p, 0, 16 * sizeof(int32_t)));
+    ws_npix = npix;
+    if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+    return LMGPU_OK;
+}
+
+static int trace_grid(int n) {
+    // enough waves to fill 256 CUs several times over, bounded by the work
+    return std::max(1, std::min((n + kBlock - 1) / kBlock, 256 * 8));
+}
+
+int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream_t s) {
+    PathState ps{thr.p, L.p, prev.p, info.p};
+    Queues q{{ext0.p, ext1.p}, hit.p, {path0.p, path1.p}, sh.p, shc.p, counts.p};
+    const int npix = c.npix;
+    const unsigned gb = unsigned((npix + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_generate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
+    const int tg = trace_grid(npix);
+    for (int b = 0; b < c.max_verts - 1; b++) {
+        const int in_q = b & 1;
+        if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
+        else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
+        else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, counts.p, in_q ^ 1);
+        hipLaunchKernelGGL(k_shade, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, c, in_q);
+        if (c.mode != LMGPU_SAMPLING_NAIVE) {
+            if (accel.width == 4) hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
+            else if (accel.width == 8) hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
+            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
+        }
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(gb), dim3(kBlock), 0, s, ps, sum, sumsq, npix);
+    hipLaunchKernelGGL(k_next_sample, dim3(1), dim3(64), 0, s, counts.p);
+    LMGPU_HIP(hipGetLastError());
+    return LMGPU_OK;
+}
+
+int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bool on_device, lmgpu_pt_stats* st,
+                  hipStream_t s) {
+    if (!p) return fail(LMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!sum_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film_sum is NULL");
+    if (p->max_verts < 2 || p->max_verts > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "max_verts must be in [2, 64]");
+    if (p->sampling_mode < 0 || p->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
+    if (p->render_time <= 0 && p->spp <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "spp must be positive");
+    if (p->guiding) return fail(LMGPU_ERR_UNSUPPORTED, "guiding is enabled through lmgpu_guide_* (not in this entry point)");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int npix = w * h;
+    LMGPU_TRY(ensure_workspace(npix));
+    double* sum = sum_in;
+    double* sumsq = sumsq_in;
+    if (!on_device) {
+        if (fsum.n != size_t(npix) * 3) LMGPU_TRY(fsum.alloc(size_t(npix) * 3));
+        sum = fsum.p;
+        if (sumsq_in) {
+            if (fsq.n != size_t(npix) * 3) LMGPU_TRY(fsq.alloc(size_t(npix) * 3));
+            sumsq = fsq.p;
+        }
+        const unsigned zb = unsigned((int64_t(npix) * 3 + 255) / 256);
+        hipLaunchKernelGGL(k_zero_f64, dim3(zb), dim3(256), 0, s, sum, int64_t(npix) * 3);
+        if (sumsq) hipLaunchKernelGGL(k_zero_f64, dim3(zb), dim3(256), 0, s, sumsq, int64_t(npix) * 3);
+    }
+    PTConsts c;
+    c.max_verts = p->max_verts;
+    c.mode = p->sampling_mode;
+    c.rr_depth = p->rr_depth;
+    c.seed_lo = uint32_t(p->seed);
+    c.seed_hi = uint32_t(p->seed >> 32);
+    c.npix = npix;
+    // stats counters [8..15] as 64-bit: [8] samples, [10] extension rays, [12] shadow rays
+    LMGPU_HIP(hipMemsetAsync(counts.p + 8, 0, 8 * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(64), 0, s, counts.p, p->sample_offset);
+    // one pass captured as a graph, replayed per sample
+    const bool key_same = graph && graph_key[0] == c.max_verts && graph_key[1] == c.mode && graph_key[2] == c.rr_depth &&
+                          graph_seed == p->seed && graph_sum_ptr == sum && graph_sq_ptr == sumsq;
+    if (!key_same) {
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        hipStream_t cs;
+        LMGPU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t g;
+        LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_pass(c, sum, sumsq, cs);
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc != LMGPU_OK) { (void)hipStreamDestroy(cs); return rc; }
+        if (e != hipSuccess) { (void)hipStreamDestroy(cs); return fail(LMGPU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e)); }
+        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) { graph = nullptr; return fail(LMGPU_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+        graph_key[0] = c.max_verts; graph_key[1] = c.mode; graph_key[2] = c.rr_depth;
+        graph_seed = p->seed;
+        graph_sum_ptr = sum;
+        graph_sq_ptr = sumsq;
+    }
+    int64_t passes = 0;
+    if (p->render_time > 0) {
+        // scheduler::spp::time: 1-spp passes until the budget is spent
+        while (true) {
+            LMGPU_HIP(hipGraphLaunch(graph, s));
+            passes++;
+            LMGPU_HIP(hipStreamSynchronize(s));
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > p->render_time) break;
+        }
+    } else {
+        for (int i = 0; i < p->spp; i++) {
+            LMGPU_HIP(hipGraphLaunch(graph, s));
+            passes++;
+        }
+    }
+    int32_t cnt[16];
+    LMGPU_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    if (!on_device) {
+        LMGPU_HIP(hipMemcpyAsync(sum_in, sum, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+        if (sumsq_in) LMGPU_HIP(hipMemcpyAsync(sumsq_in, sumsq, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+    }
+    LMGPU_HIP(hipStreamSynchronize(s));
+    if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during rendering");
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        int64_t v[3];
+        std::memcpy(v, cnt + 8, sizeof(v));
+        st->samples = v[0];
+        st->spp = passes;
+        st->extension_rays = v[1];
+        st->shadow_rays = v[2];
+        st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return LMGPU_OK;
+}
+
+}  // namespace lmgpu
+
+using lmgpu::fail;
+
+extern "C" {
+
+struct lmgpu_scene {
+    lmgpu::Scene s;
+};
+
+int lmgpu_scene_create(lmgpu_scene** out, const lmgpu_scene_desc* desc, const lmgpu_build_opts* opts) {
+    if (!out) return fail(LMGPU_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    auto* h = new (std::nothrow) lmgpu_scene();
+    if (!h) return fail(LMGPU_ERR_OUT_OF_MEMORY, "scene handle");
+    const int rc = h->s.create(desc, opts);
+    if (rc != LMGPU_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return LMGPU_OK;
+}
+
+int lmgpu_scene_destroy(lmgpu_scene* scene) {
+    delete scene;
+    return LMGPU_OK;
+}
+
+struct lmgpu_accel;
+int lmgpu_scene_accel(lmgpu_scene* scene, lmgpu_accel** accel) {
+    if (!scene || !accel) return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL argument");
+    // lmgpu_accel wraps exactly one lmgpu::Accel (capi.hip)
+    *accel = reinterpret_cast<lmgpu_accel*>(&scene->s.accel);
+    return LMGPU_OK;
+}
+
+int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* film_sum, double* film_sumsq,
+                    int film_on_device, lmgpu_pt_stats* stats, void* stream) {
+    if (!scene) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene is NULL");
+    lmgpu::clear_error();
+    return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, stats, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+"""Loader of liblmgpu.so, the HIP (gfx950) library behind the lmgpu C ABI.
+
+The library is built in-tree (``make -C lightmetrica-v3_amd``). There is no
+fallback: if it is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, 'liblmgpu.so')
+
+# Status codes (include/lmgpu.h)
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_UNSUPPORTED = -2
+ERR_FAILED_TO_RENDER = -3
+ERR_DEVICE = -4
+ERR_OUT_OF_MEMORY = -5
+ERR_STACK_OVERFLOW = -6
+
+_ERR_NAMES = {
+    ERR_INVALID_ARGUMENT: 'InvalidArgument',
+    ERR_UNSUPPORTED: 'Unsupported',
+    ERR_FAILED_TO_RENDER: 'FailedToRender',
+    ERR_DEVICE: 'Device',
+    ERR_OUT_OF_MEMORY: 'OutOfMemory',
+    ERR_STACK_OVERFLOW: 'StackOverflow',
+}
+
+# Every symbol include/lmgpu.h declares (checked by the CPU tests).
+EXPORTED = [
+    'lmgpu_last_error', 'lmgpu_version', 'lmgpu_device_count',
+    'lmgpu_accel_create', 'lmgpu_accel_destroy', 'lmgpu_accel_build', 'lmgpu_accel_info_get',
+    'lmgpu_accel_intersect', 'lmgpu_accel_intersect_host', 'lmgpu_accel_occluded', 'lmgpu_accel_status',
+    'lmgpu_accel_intersect_counted',
+    'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
+]
+
+
+class LmError(RuntimeError):
+    """Error raised for a failing lmgpu call (mirrors LM_THROW_EXCEPTION)."""
+
+    def __init__(self, code: int, msg: str, func: str):
+        self.code = code
+        super().__init__(f'{func}: {_ERR_NAMES.get(code, code)}: {msg}')
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [('width', ctypes.c_int32), ('leaf_size', ctypes.c_int32), ('reserved', ctypes.c_int32 * 6)]
+
+
+class AccelInfo(ctypes.Structure):
+    _fields_ = [
+        ('num_tris', ctypes.c_int64), ('num_nodes', ctypes.c_int64), ('num_leaves', ctypes.c_int64),
+        ('width', ctypes.c_int32), ('leaf_size', ctypes.c_int32), ('depth', ctypes.c_int32),
+        ('stack_capacity', ctypes.c_int32), ('node_bytes', ctypes.c_int64), ('tri_bytes', ctypes.c_int64),
+        ('sah_cost', ctypes.c_double), ('build_ms', ctypes.c_double),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ('num_tris', ctypes.c_int64),
+        ('verts', ctypes.c_void_p),
+        ('normals', ctypes.c_void_p),
+        ('tri_prim', ctypes.c_void_p),
+        ('num_prims', ctypes.c_int32),
+        ('mat_type', ctypes.c_void_p),
+        ('mat_param', ctypes.c_void_p),
+        ('prim_Ke', ctypes.c_void_p),
+        ('num_area_lights', ctypes.c_int32),
+        ('light_prims', ctypes.c_void_p),
+        ('has_env', ctypes.c_int32),
+        ('env_Le', ctypes.c_float * 3),
+        ('cam_pos', ctypes.c_float * 3),
+        ('cam_center', ctypes.c_float * 3),
+        ('cam_up', ctypes.c_float * 3),
+        ('vfov', ctypes.c_float),
+        ('aspect', ctypes.c_float),
+        ('w', ctypes.c_int32),
+        ('h', ctypes.c_int32),
+    ]
+
+
+class PTParams(ctypes.Structure):
+    _fields_ = [
+        ('spp', ctypes.c_int32), ('max_verts', ctypes.c_int32), ('sampling_mode', ctypes.c_int32),
+        ('rr_depth', ctypes.c_int32), ('seed', ctypes.c_uint64), ('sample_offset', ctypes.c_int32),
+        ('guiding', ctypes.c_int32), ('render_time', ctypes.c_double), ('reserved', ctypes.c_int32 * 8),
+    ]
+
+
+class PTStats(ctypes.Structure):
+    _fields_ = [
+        ('samples', ctypes.c_int64), ('spp', ctypes.c_int64), ('extension_rays', ctypes.c_int64),
+        ('shadow_rays', ctypes.c_int64), ('elapsed_s', ctypes.c_double), ('reserved', ctypes.c_int64 * 4),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liblmgpu.so in-tree with hipcc (gfx950)."""
+    args = ['make', '-s', '-C', PKG_DIR, f'-j{min(8, os.cpu_count() or 1)}']
+    if force:
+        args.insert(2, '-B')
+    subprocess.check_call(args)
+    return LIB_PATH
+
+
+def lib():
+    """The loaded library; raises if it is not built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f'{LIB_PATH} is not built; run lm_amd._lib.build() '
+                               '(make -C lightmetrica-v3_amd) first')
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        L.lmgpu_last_error.restype = ctypes.c_char_p
+        L.lmgpu_version.restype = ctypes.c_char_p
+        L.lmgpu_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.lmgpu_accel_create.argtypes = [ctypes.POINTER(vp)]
+        L.lmgpu_accel_destroy.argtypes = [vp]
+        L.lmgpu_accel_build.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.POINTER(BuildOpts), vp]
+        L.lmgpu_accel_info_get.argtypes = [vp, ctypes.POINTER(AccelInfo)]
+        L.lmgpu_accel_intersect.argtypes = [vp, vp, vp, i64, vp]
+        L.lmgpu_accel_intersect_host.argtypes = [vp, vp, vp, i64]
+        L.lmgpu_accel_occluded.argtypes = [vp, vp, vp, i64, vp]
+        L.lmgpu_accel_status.argtypes = [vp, vp]
+        L.lmgpu_accel_intersect_counted.argtypes = [vp, vp, vp, i64, ctypes.POINTER(i64), ctypes.POINTER(i64), vp]
+        L.lmgpu_scene_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ctypes.POINTER(BuildOpts)]
+        L.lmgpu_scene_destroy.argtypes = [vp]
+        L.lmgpu_scene_accel.argtypes = [vp, ctypes.POINTER(vp)]
+        L.lmgpu_pt_render.argtypes = [vp, ctypes.POINTER(PTParams), vp, vp, ctypes.c_int, ctypes.POINTER(PTStats), vp]
+        for name in EXPORTED:
+            if name not in ('lmgpu_last_error', 'lmgpu_version'):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int, func: str):
+    if rc != OK:
+        msg = lib().lmgpu_last_error()
+        raise LmError(rc, msg.decode() if msg else '', func)
+
+
+def call(func: str, *args):
+    check(getattr(lib(), func)(*args), func)

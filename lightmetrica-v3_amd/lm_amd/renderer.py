@@ -1,0 +1,202 @@
+"""``renderer::pt_mi355x`` and ``film::bitmap`` host mirrors over the C ABI.
+
+``RendererPT`` takes the construction properties of the reference
+``renderer::pt`` (src/renderer/renderer_pt.cpp:51-77 and the spp schedulers,
+src/scheduler.cpp:32-35,72-75): ``scene``, ``output``, ``max_verts``,
+``seed``, ``sampling_mode`` (naive|nee|mis), ``scheduler`` (sample|time),
+``spp`` / ``render_time``; plus ``rr_depth`` (Russian roulette when
+num_verts > rr_depth; the reference hard-codes 5, renderer_pt.cpp:281).
+``render()`` returns ``{"processed", "elapsed"}`` as the reference does
+(:314) and leaves the mean image (sum / spp) in the film.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .accel import _stream_ptr
+
+_MODES = {'naive': 0, 'nee': 1, 'mis': 2}
+
+
+class Film:
+    """film::bitmap: FP64 RGB, [h][w][3], row 0 = bottom (film_bitmap.cpp:123-275).
+
+    Besides the mean image it keeps the per-pixel sum and sum of squares of
+    the per-sample estimates of the last render (for Welch tests)."""
+
+    def __init__(self, w: int, h: int):
+        self.w = int(w)
+        self.h = int(h)
+        self.data = np.zeros((self.h, self.w, 3), dtype=np.float64)
+        self.sum = None
+        self.sumsq = None
+        self.spp = 0
+
+    def size(self):
+        return self.w, self.h
+
+    def num_pixels(self) -> int:
+        return self.w * self.h
+
+    def aspect(self) -> float:
+        return self.w / self.h
+
+    def clear(self):
+        self.data[:] = 0
+
+    def rescale(self, s: float):
+        self.data *= s
+
+    def buffer(self) -> np.ndarray:
+        return self.data
+
+    def set_pixel(self, x: int, y: int, v):
+        self.data[y, x] = v
+
+    def splat_pixel(self, x: int, y: int, v):
+        self.data[y, x] += v
+
+    def raster_to_pixel(self, rp):
+        """include/lm/film.h:166-171 (clamped)."""
+        x = min(max(int(rp[0] * self.w), 0), self.w - 1)
+        y = min(max(int(rp[1] * self.h), 0), self.h - 1)
+        return x, y
+
+    def splat(self, rp, v):
+        self.splat_pixel(*self.raster_to_pixel(rp), v)
+
+
+class GpuScene:
+    """Device scene (geometry, accel, materials, lights, camera) built from a
+    lm_amd.scenes.FlatScene through lmgpu_scene_create."""
+
+    def __init__(self, flat, width: int = 4, leaf_size: int = 4):
+        self.flat = flat
+        self._keep = {
+            'verts': np.ascontiguousarray(flat.verts, dtype=np.float32),
+            'normals': np.ascontiguousarray(flat.normals, dtype=np.float32),
+            'tri_prim': np.ascontiguousarray(flat.tri_prim, dtype=np.int32),
+            'mat_type': np.ascontiguousarray(flat.mat_type, dtype=np.int32),
+            'mat_param': np.ascontiguousarray(flat.mat_param, dtype=np.float32),
+            'prim_Ke': np.ascontiguousarray(flat.prim_Ke, dtype=np.float32),
+            'light_prims': np.ascontiguousarray(flat.light_prims, dtype=np.int32),
+        }
+        k = self._keep
+        d = _lib.SceneDesc()
+        d.num_tris = flat.num_triangles
+        d.verts = k['verts'].ctypes.data
+        d.normals = k['normals'].ctypes.data
+        d.tri_prim = k['tri_prim'].ctypes.data
+        d.num_prims = int(k['mat_type'].shape[0])
+        d.mat_type = k['mat_type'].ctypes.data
+        d.mat_param = k['mat_param'].ctypes.data
+        d.prim_Ke = k['prim_Ke'].ctypes.data
+        d.num_area_lights = int(k['light_prims'].shape[0])
+        d.light_prims = k['light_prims'].ctypes.data if k['light_prims'].size else None
+        d.has_env = 1 if flat.has_env else 0
+        for i in range(3):
+            d.env_Le[i] = float(flat.env_Le[i])
+            d.cam_pos[i] = float(flat.camera.position[i])
+            d.cam_center[i] = float(flat.camera.center[i])
+            d.cam_up[i] = float(flat.camera.up[i])
+        d.vfov = float(flat.camera.vfov)
+        d.aspect = float(flat.camera.aspect)
+        d.w = int(flat.w)
+        d.h = int(flat.h)
+        opts = _lib.BuildOpts(width, leaf_size)
+        h = ctypes.c_void_p()
+        _lib.call('lmgpu_scene_create', ctypes.byref(h), ctypes.byref(d), ctypes.byref(opts))
+        self._h = h
+        self.w, self.h = int(flat.w), int(flat.h)
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.lmgpu_scene_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def accel_info(self) -> dict:
+        a = ctypes.c_void_p()
+        _lib.call('lmgpu_scene_accel', self._h, ctypes.byref(a))
+        i = _lib.AccelInfo()
+        _lib.call('lmgpu_accel_info_get', a, ctypes.byref(i))
+        return {f: getattr(i, f) for f, _ in _lib.AccelInfo._fields_}
+
+    def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: int = 5,
+                   seed: int = 0, sample_offset: int = 0, render_time: float = 0.0,
+                   film_sum=None, film_sumsq=None, stream=None):
+        """lmgpu_pt_render. With torch CUDA tensors for film_sum / film_sumsq
+        ([h,w,3] float64) the film stays on the device and is accumulated
+        into; otherwise host arrays are returned (fresh sums)."""
+        p = _lib.PTParams()
+        p.spp = int(spp)
+        p.max_verts = int(max_verts)
+        p.sampling_mode = int(sampling_mode)
+        p.rr_depth = int(rr_depth)
+        p.seed = int(seed)
+        p.sample_offset = int(sample_offset)
+        p.guiding = 0
+        p.render_time = float(render_time)
+        st = _lib.PTStats()
+        on_device = film_sum is not None and hasattr(film_sum, 'data_ptr')
+        if on_device:
+            sq = ctypes.c_void_p(film_sumsq.data_ptr()) if film_sumsq is not None else None
+            _lib.call('lmgpu_pt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film_sum.data_ptr()), sq, 1,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+            s, q = film_sum, film_sumsq
+        else:
+            s = np.zeros((self.h, self.w, 3), dtype=np.float64)
+            q = np.zeros((self.h, self.w, 3), dtype=np.float64)
+            _lib.call('lmgpu_pt_render', self._h, ctypes.byref(p), s.ctypes.data_as(ctypes.c_void_p),
+                      q.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+        stats = {'samples': st.samples, 'spp': st.spp, 'extension_rays': st.extension_rays,
+                 'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s}
+        return s, q, stats
+
+
+class RendererPT:
+    """renderer::pt_mi355x: drop-in for renderer::pt (pixel sampling mode)."""
+
+    def __init__(self, scene: GpuScene, output: Film, max_verts: int, seed: Optional[int] = None,
+                 sampling_mode: str = 'mis', scheduler: str = 'sample', spp: Optional[int] = None,
+                 render_time: Optional[float] = None, rr_depth: int = 5,
+                 primary_ray_sampling_mode: str = 'pixel'):
+        if sampling_mode not in _MODES:
+            raise ValueError(f'invalid sampling_mode {sampling_mode!r}')
+        if primary_ray_sampling_mode != 'pixel':
+            raise NotImplementedError('only primary_ray_sampling_mode = "pixel" is implemented')
+        if scheduler == 'sample' and not spp:
+            raise ValueError('scheduler "sample" needs spp')
+        if scheduler == 'time' and not render_time:
+            raise ValueError('scheduler "time" needs render_time')
+        if (output.w, output.h) != (scene.w, scene.h):
+            raise ValueError('film size differs from the scene film size')
+        self.scene = scene
+        self.film = output
+        self.max_verts = int(max_verts)
+        self.seed = int(seed) if seed is not None else int(np.random.SeedSequence().generate_state(1)[0])
+        self.mode = _MODES[sampling_mode]
+        self.scheduler = scheduler
+        self.spp = int(spp or 0)
+        self.render_time = float(render_time or 0.0)
+        self.rr_depth = int(rr_depth)
+
+    def render(self) -> dict:
+        film = self.film
+        film.clear()
+        s, q, st = self.scene.render_raw(
+            spp=self.spp if self.scheduler == 'sample' else 0, max_verts=self.max_verts,
+            sampling_mode=self.mode, rr_depth=self.rr_depth, seed=self.seed,
+            render_time=self.render_time if self.scheduler == 'time' else 0.0)
+        processed = st['spp']
+        film.sum, film.sumsq, film.spp = s, q, processed
+        film.data[:] = s / processed
+        return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}

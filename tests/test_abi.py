@@ -1,0 +1,50 @@
+"""CPU checks of the C ABI: liblmgpu.so loads and exports every function
+include/lmgpu.h declares; host-side argument errors surface as LmError
+without a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from lm_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, 'include', 'lmgpu.h')).read()
+    src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
+    return sorted(set(re.findall(r'\b(lmgpu_[a-z_0-9]+)\s*\(', src)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('liblmgpu.so not built')
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_host_side_errors_without_gpu():
+    L = _lib.lib()
+    assert L.lmgpu_version().startswith(b'lmgpu')
+    # NULL out-pointers are rejected before any device call
+    assert L.lmgpu_accel_create(None) == _lib.ERR_INVALID_ARGUMENT
+    assert b'NULL' in L.lmgpu_last_error()
+    assert L.lmgpu_accel_build(None, None, 0, 0, None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.lmgpu_scene_create(None, None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.lmgpu_pt_render(None, None, None, None, 0, None, None) == _lib.ERR_INVALID_ARGUMENT
+    with pytest.raises(_lib.LmError):
+        _lib.call('lmgpu_accel_info_get', None, None)
+
+
+def test_structure_sizes_match_header():
+    # lmgpu_ray 32 B, lmgpu_hit 16 B as the kernels assume
+    assert ctypes.sizeof(_lib.BuildOpts) == 32
+    assert ctypes.sizeof(_lib.PTParams) == 4 * 4 + 8 + 4 + 4 + 8 + 32
