@@ -74,7 +74,7 @@ def main():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--width', type=int, default=4)
-    ap.add_argument('--leaf', type=int, default=4)
+    ap.add_argument('--leaf', type=int, default=2)
     ap.add_argument('--pt-spp', type=int, default=4)
     ap.add_argument('--no-pt', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')

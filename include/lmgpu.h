@@ -75,9 +75,16 @@ typedef struct lmgpu_accel lmgpu_accel;
 
 typedef struct lmgpu_build_opts {
     int32_t width;       /* BVH branching factor: 2, 4 or 8 (0 = 4) */
-    int32_t leaf_size;   /* max triangles per leaf, 1..15 (0 = 4) */
-    int32_t reserved[6];
+    int32_t leaf_size;   /* max triangles per leaf, 1..15 (0 = 2) */
+    int32_t refill;      /* traversal: a wave takes new rays once this many of its
+                            64 lanes are idle, 1..64 (0 = 32; 64 suits coherent rays) */
+    int32_t builder;     /* LMGPU_BUILDER_LBVH (1) or LMGPU_BUILDER_PLOC (0); with opts == NULL: LBVH */
+    int32_t ploc_radius; /* PLOC nearest-neighbour window, 1..64 (0 = 16) */
+    int32_t reserved[3];
 } lmgpu_build_opts;
+
+#define LMGPU_BUILDER_PLOC 0  /* locally-ordered clustering over Morton order */
+#define LMGPU_BUILDER_LBVH 1  /* Karras radix tree over Morton order */
 
 typedef struct lmgpu_accel_info {
     int64_t num_tris;

@@ -25,6 +25,7 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include "build.h"
 #include "bvh.h"
 #include "lmgpu_internal.h"
 
@@ -34,25 +35,11 @@ namespace lmgpu {
 // build kernels
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t f2ord(float f) {
-    const uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__host__ __device__ __forceinline__ float ord2f(uint32_t u) {
-    const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
-    float f;
-    memcpy(&f, &b, 4);
-    return f;
-}
-
-__global__ void k_init_bounds(uint32_t* cb) {
-    if (threadIdx.x < 3) cb[threadIdx.x] = 0xffffffffu;        // min
-    else if (threadIdx.x < 6) cb[threadIdx.x] = 0u;            // max
-}
-
-// Per-triangle AABB (float4 lo, hi) and the centroid bounds.
+// Per-triangle AABB (float4 lo, hi); per-block centroid bounds into
+// `partial` (6 floats per block), reduced by k_reduce_bounds.
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, int64_t n, float4* __restrict__ tlo,
-                                              float4* __restrict__ thi, uint32_t* __restrict__ cb) {
+                                              float4* __restrict__ thi, float* __restrict__ partial) {
+    __shared__ float red[6][4];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     float cmin[3] = {3e38f, 3e38f, 3e38f}, cmax[3] = {-3e38f, -3e38f, -3e38f};
     if (i < n) {
@@ -68,7 +55,7 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, i
         tlo[i] = make_float4(lo[0], lo[1], lo[2], 0.f);
         thi[i] = make_float4(hi[0], hi[1], hi[2], 0.f);
     }
-    // wave reduction then one atomic per wave
+    const int w = threadIdx.x >> 6;
     for (int a = 0; a < 3; a++) {
         float mn = cmin[a], mx = cmax[a];
         for (int off = 32; off > 0; off >>= 1) {
@@ -76,9 +63,39 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, i
             mx = fmaxf(mx, __shfl_xor(mx, off));
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicMin(&cb[a], f2ord(mn));
-            atomicMax(&cb[3 + a], f2ord(mx));
+            red[a][w] = mn;
+            red[3 + a][w] = mx;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float x = red[a][0];
+        for (int k = 1; k < 4; k++) x = a < 3 ? fminf(x, red[a][k]) : fmaxf(x, red[a][k]);
+        partial[6 * int64_t(blockIdx.x) + a] = x;
+    }
+}
+
+// One block reduces the per-block partial bounds into cb[6] (floats).
+__global__ __launch_bounds__(256) void k_reduce_bounds(const float* __restrict__ partial, int nparts,
+                                                       float* __restrict__ cb) {
+    __shared__ float red[6][4];
+    float v[6] = {3e38f, 3e38f, 3e38f, -3e38f, -3e38f, -3e38f};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int a = 0; a < 6; a++)
+            v[a] = a < 3 ? fminf(v[a], partial[6 * b + a]) : fmaxf(v[a], partial[6 * b + a]);
+    const int w = threadIdx.x >> 6;
+    for (int a = 0; a < 6; a++) {
+        float x = v[a];
+        for (int off = 32; off > 0; off >>= 1) x = a < 3 ? fminf(x, __shfl_xor(x, off)) : fmaxf(x, __shfl_xor(x, off));
+        if ((threadIdx.x & 63) == 0) red[a][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float x = red[a][0];
+        for (int k = 1; k < 4; k++) x = a < 3 ? fminf(x, red[a][k]) : fmaxf(x, red[a][k]);
+        cb[a] = x;
     }
 }
 
@@ -93,7 +110,7 @@ __device__ __forceinline__ uint64_t spread21(uint64_t x) {
 }
 
 __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tlo, const float4* __restrict__ thi,
-                                                int64_t n, const uint32_t* __restrict__ cb,
+                                                int64_t n, const float* __restrict__ cb,
                                                 uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -101,7 +118,7 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tlo, 
     const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
     uint64_t code = 0;
     for (int a = 0; a < 3; a++) {
-        const float mn = ord2f(cb[a]), mx = ord2f(cb[3 + a]);
+        const float mn = cb[a], mx = cb[3 + a];
         const float ext = mx - mn;
         float f = ext > 0.f ? (c[a] - mn) / ext : 0.5f;
         f = fminf(fmaxf(f, 0.f), 1.f);
@@ -112,11 +129,6 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tlo, 
     vals[i] = int32_t(i);
 }
 
-// Binary node refs: >= 0 internal node index, < 0 leaf ~j (sorted position j).
-struct BNode {
-    int32_t left, right;
-    int32_t first, last;
-};
 
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
@@ -168,35 +180,47 @@ __global__ __launch_bounds__(256) void k_lbvh(const uint64_t* __restrict__ k, in
     bn[i] = node;
 }
 
-// Bottom-up refit: one thread per leaf walks to the root; at each internal
-// node the first arrival leaves and the second merges both children.
-__global__ __launch_bounds__(256) void k_refit(int64_t n, const int32_t* __restrict__ order,
-                                               const float4* __restrict__ tlo, const float4* __restrict__ thi,
-                                               const BNode* __restrict__ bn, const int32_t* __restrict__ parent_int,
-                                               const int32_t* __restrict__ parent_leaf, float4* leaf_lo,
-                                               float4* leaf_hi, float4* node_lo, float4* node_hi,
-                                               int32_t* flags) {
+// Bottom-up refit without device-scope fences (which write back the
+// per-XCD L2s): level-synchronous over a frontier. A node enters the
+// frontier of launch k+1 when its second child finished in launch k (atomic
+// arrival counter), so both children's bounds were written by earlier
+// launches and are visible at kernel boundaries.
+__global__ __launch_bounds__(256) void k_refit_leaves(int64_t n, const int32_t* __restrict__ order,
+                                                      const float4* __restrict__ tlo, const float4* __restrict__ thi,
+                                                      const int32_t* __restrict__ parent_leaf,
+                                                      float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi,
+                                                      int32_t* __restrict__ flags, int32_t* __restrict__ frontier,
+                                                      int32_t* __restrict__ fcount) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const int32_t ti = order[j];
     leaf_lo[j] = tlo[ti];
     leaf_hi[j] = thi[ti];
     if (n == 1) return;
-    __threadfence();
-    int32_t p = parent_leaf[j];
-    while (p >= 0) {
-        if (atomicAdd(&flags[p], 1) == 0) return;
-        __threadfence();
-        const BNode nd = bn[p];
-        const float4 l0 = nd.left < 0 ? leaf_lo[~nd.left] : node_lo[nd.left];
-        const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
-        const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
-        const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
-        node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-        node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
-        __threadfence();
-        p = p == 0 ? -1 : parent_int[p];
-    }
+    const int32_t p = parent_leaf[j];
+    if (atomicAdd(&flags[p], 1) == 1) frontier[atomicAdd(fcount, 1)] = p;
+}
+
+__global__ __launch_bounds__(256) void k_refit_nodes(const int32_t* __restrict__ fin, const int32_t* __restrict__ fin_count,
+                                                     const BNode* __restrict__ bn,
+                                                     const int32_t* __restrict__ parent_int,
+                                                     const float4* __restrict__ leaf_lo,
+                                                     const float4* __restrict__ leaf_hi, float4* __restrict__ node_lo,
+                                                     float4* __restrict__ node_hi, int32_t* __restrict__ flags,
+                                                     int32_t* __restrict__ fout, int32_t* __restrict__ fout_count) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *fin_count) return;
+    const int32_t p = fin[idx];
+    const BNode nd = bn[p];
+    const float4 l0 = nd.left < 0 ? leaf_lo[~nd.left] : node_lo[nd.left];
+    const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
+    const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
+    const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
+    node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    if (p == 0) return;
+    const int32_t q = parent_int[p];
+    if (atomicAdd(&flags[q], 1) == 1) fout[atomicAdd(fout_count, 1)] = q;
 }
 
 __global__ __launch_bounds__(256) void k_reorder(int64_t n, const int32_t* __restrict__ order,
@@ -304,55 +328,40 @@ constexpr int kTraceBlock = 256;
 constexpr int kLdsDepth = 24;
 constexpr int kPrivDepth = 72;
 
+// Persistent closest-hit / any-hit kernel over a ray array (bvh.h,
+// trace_persistent). `next` is this launch's ray counter, zeroed by
+// k_zero_counter on the same stream.
 template <int W, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const WNode<W>* __restrict__ nodes,
                                                         const TriV* __restrict__ tris,
                                                         const lmgpu_ray* __restrict__ rays,
                                                         lmgpu_hit* __restrict__ hits, uint8_t* __restrict__ occl,
-                                                        int64_t n, int32_t* __restrict__ status,
-                                                        unsigned long long* __restrict__ counters) {
+                                                        int64_t n, unsigned long long* __restrict__ next,
+                                                        int32_t* __restrict__ status,
+                                                        unsigned long long* __restrict__ counters, int refill) {
     __shared__ int lds[kLdsDepth][kTraceBlock];
-    // XCD-aware block order: hardware block b runs on XCD b % 8 (round-robin
-    // dispatch); give each XCD a contiguous range of logical blocks so
-    // neighbouring rays share that XCD's L2. XCD x owns q + (x < r) blocks
-    // starting at x*q + min(x, r), a bijection for any grid size.
-    const int b = blockIdx.x;
-    const int bq = gridDim.x >> 3, br = gridDim.x & 7;
-    const int x = b & 7, k = b >> 3;
-    const int lb = x * bq + min(x, br) + k;
-    const int64_t i = int64_t(lb) * kTraceBlock + threadIdx.x;
-    const bool valid = i < n;
-    TraceResult r;
-    r.node_visits = 0;
-    r.tri_tests = 0;
-    if (valid) {
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + id);
         const float4 a = rp[0], c = rp[1];
-        r = trace<W, ANY, COUNT, kLdsDepth, kPrivDepth, kTraceBlock>(nodes, tris, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z),
-                                                                      a.w, c.w, lds, threadIdx.x);
-        if (r.overflow) atomicOr(status, 1);
+        o = mk(a.x, a.y, a.z);
+        d = mk(c.x, c.y, c.z);
+        tmin = a.w;
+        tmax = c.w;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
         if (ANY) {
-            occl[i] = r.prim >= 0 ? 1 : 0;
+            occl[id] = r.prim >= 0 ? 1 : 0;
         } else {
-            float4 h;
-            h.x = __int_as_float(r.prim);
-            h.y = r.prim >= 0 ? r.t : 0.f;
-            h.z = r.u;
-            h.w = r.v;
-            reinterpret_cast<float4*>(hits)[i] = h;
+            reinterpret_cast<float4*>(hits)[id] =
+                make_float4(__int_as_float(r.prim), r.prim >= 0 ? r.t : 0.f, r.u, r.v);
         }
-    }
-    if (COUNT) {
-        unsigned long long nv = valid ? r.node_visits : 0, nt = valid ? r.tri_tests : 0;
-        for (int off = 32; off > 0; off >>= 1) {
-            nv += __shfl_xor(nv, off);
-            nt += __shfl_xor(nt, off);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&counters[0], nv);
-            atomicAdd(&counters[1], nt);
-        }
-    }
+    };
+    trace_persistent<W, ANY, COUNT, kLdsDepth, kPrivDepth, kTraceBlock>(nodes, tris, n, next, lds, fetch, emit,
+                                                                         status, counters, refill);
+}
+
+__global__ void k_zero_counter(unsigned long long* c) {
+    if (threadIdx.x == 0) *c = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -373,7 +382,13 @@ static void launch_collapse(const CollapseItem* in, const int32_t* n_in, Collaps
 
 int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_build_opts* opts, hipStream_t s) {
     int W = opts && opts->width ? opts->width : 4;
-    int leaf = opts && opts->leaf_size ? opts->leaf_size : 4;
+    int leaf = opts && opts->leaf_size ? opts->leaf_size : 2;
+    const int rf = opts && opts->refill ? opts->refill : 32;
+    if (rf < 1 || rf > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "refill must be in [1, 64]");
+    const int builder = opts ? opts->builder : LMGPU_BUILDER_LBVH;
+    if (builder != LMGPU_BUILDER_PLOC && builder != LMGPU_BUILDER_LBVH)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "unknown builder");
+    const int radius = opts && opts->ploc_radius ? opts->ploc_radius : 16;
     if (W != 2 && W != 4 && W != 8) return fail(LMGPU_ERR_INVALID_ARGUMENT, "BVH width must be 2, 4 or 8");
     if (leaf < 1 || leaf > 15) return fail(LMGPU_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 15]");
     if (n <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel build needs at least one triangle");
@@ -382,6 +397,7 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     release();
     width = W;
     leaf_size = leaf;
+    refill = rf;
     num_tris = n;
     DevBuf<float> dverts;
     const float* verts = verts_in;
@@ -396,7 +412,7 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     LMGPU_HIP(hipEventRecord(e0, s));
 
     DevBuf<float4> tlo, thi, leaf_lo, leaf_hi, node_lo, node_hi;
-    DevBuf<uint32_t> cb;
+    DevBuf<float> cb, partial;
     DevBuf<uint64_t> keys, keys2;
     DevBuf<int32_t> vals, vals2, parent_int, parent_leaf, flags, counters;
     DevBuf<BNode> bn;
@@ -406,6 +422,8 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     LMGPU_TRY(leaf_lo.alloc(n)); LMGPU_TRY(leaf_hi.alloc(n));
     LMGPU_TRY(node_lo.alloc(std::max<int64_t>(n - 1, 1))); LMGPU_TRY(node_hi.alloc(std::max<int64_t>(n - 1, 1)));
     LMGPU_TRY(cb.alloc(6));
+    const unsigned nb_prep = nblocks(n, 256);
+    LMGPU_TRY(partial.alloc(size_t(nb_prep) * 6));
     LMGPU_TRY(keys.alloc(n)); LMGPU_TRY(keys2.alloc(n));
     LMGPU_TRY(vals.alloc(n)); LMGPU_TRY(vals2.alloc(n));
     LMGPU_TRY(parent_int.alloc(std::max<int64_t>(n - 1, 1)));
@@ -416,30 +434,60 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     LMGPU_TRY(q0.alloc(n)); LMGPU_TRY(q1.alloc(n));
     LMGPU_TRY(sah.alloc(1));
 
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, cb.p);
-    hipLaunchKernelGGL(k_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, verts, n, tlo.p, thi.p, cb.p);
+    hipLaunchKernelGGL(k_prep, dim3(nb_prep), dim3(256), 0, s, verts, n, tlo.p, thi.p, partial.p);
+    hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(256), 0, s, partial.p, int(nb_prep), cb.p);
     hipLaunchKernelGGL(k_morton, dim3(nblocks(n, 256)), dim3(256), 0, s, tlo.p, thi.p, n, cb.p, keys.p, vals.p);
     size_t temp_bytes = 0;
     LMGPU_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
     DevBuf<char> temp;
     LMGPU_TRY(temp.alloc(std::max<size_t>(temp_bytes, 16)));
     LMGPU_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
-    if (n > 1) {
-        LMGPU_HIP(hipMemsetAsync(flags.p, 0, sizeof(int32_t) * (n - 1), s));
-        LMGPU_HIP(hipMemsetAsync(parent_int.p, 0xff, sizeof(int32_t) * (n - 1), s));
-        hipLaunchKernelGGL(k_lbvh, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, keys2.p, n, bn.p, parent_int.p,
-                           parent_leaf.p);
+    int32_t root_ref = n > 1 ? 0 : ~0;
+    if (builder == LMGPU_BUILDER_PLOC) {
+        DevBuf<int32_t> perm;
+        LMGPU_TRY(perm.alloc(n));
+        LMGPU_TRY(build_ploc(n, vals2.p, tlo.p, thi.p, radius, bn.p, node_lo.p, node_hi.p, leaf_lo.p, leaf_hi.p,
+                             perm.p, &root_ref, s));
+        LMGPU_TRY(tris.alloc(n));
+        hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, verts, tris.p);
+        LMGPU_HIP(hipStreamSynchronize(s));
+    } else {
+        if (n > 1) {
+            LMGPU_HIP(hipMemsetAsync(flags.p, 0, sizeof(int32_t) * (n - 1), s));
+            LMGPU_HIP(hipMemsetAsync(parent_int.p, 0xff, sizeof(int32_t) * (n - 1), s));
+            hipLaunchKernelGGL(k_lbvh, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, keys2.p, n, bn.p, parent_int.p,
+                               parent_leaf.p);
+        }
+        {
+            // frontier refit: q0/q1 (reinterpreted as int32 lists) hold the frontiers
+            int32_t* fr[2] = {reinterpret_cast<int32_t*>(q0.p), reinterpret_cast<int32_t*>(q1.p)};
+            LMGPU_HIP(hipMemsetAsync(counters.p, 0, 2 * sizeof(int32_t), s));
+            hipLaunchKernelGGL(k_refit_leaves, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, tlo.p, thi.p,
+                               parent_leaf.p, leaf_lo.p, leaf_hi.p, flags.p, fr[0], counters.p);
+            int cur = 0;
+            int levels_refit = 0;
+            while (n > 1) {
+                int32_t cnt = 0;
+                LMGPU_HIP(hipMemcpyAsync(&cnt, counters.p + cur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                LMGPU_HIP(hipStreamSynchronize(s));
+                if (cnt == 0) break;
+                LMGPU_HIP(hipMemsetAsync(counters.p + (cur ^ 1), 0, sizeof(int32_t), s));
+                hipLaunchKernelGGL(k_refit_nodes, dim3(nblocks(cnt, 256)), dim3(256), 0, s, fr[cur], counters.p + cur,
+                                   bn.p, parent_int.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p, fr[cur ^ 1],
+                                   counters.p + (cur ^ 1));
+                cur ^= 1;
+                if (++levels_refit > 4096) return fail(LMGPU_ERR_DEVICE, "BVH refit did not terminate");
+            }
+        }
+        LMGPU_TRY(tris.alloc(n));
+        hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, verts, tris.p);
     }
-    hipLaunchKernelGGL(k_refit, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, tlo.p, thi.p, bn.p,
-                       parent_int.p, parent_leaf.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p);
-    LMGPU_TRY(tris.alloc(n));
-    hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, verts, tris.p);
 
     // collapse, level by level
     const size_t node_bytes = size_t(32) * W;
     const int64_t max_nodes = std::max<int64_t>(n, 1);
     LMGPU_TRY(nodes.alloc(size_t(max_nodes) * node_bytes / sizeof(float4)));
-    const CollapseItem root{n > 1 ? 0 : ~0, 0};
+    const CollapseItem root{root_ref, 0};
     LMGPU_HIP(hipMemcpyAsync(q0.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
     int32_t init[4] = {1, 0, 1, 0};   // n_in, n_out, node_counter, leaves
     LMGPU_HIP(hipMemcpyAsync(counters.p, init, sizeof(init), hipMemcpyHostToDevice, s));
@@ -480,9 +528,9 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     (void)hipEventDestroy(e1);
     // root half-area for the SAH normalisation
     float4 rlo, rhi;
-    if (n > 1) {
-        LMGPU_HIP(hipMemcpy(&rlo, node_lo.p, sizeof(float4), hipMemcpyDeviceToHost));
-        LMGPU_HIP(hipMemcpy(&rhi, node_hi.p, sizeof(float4), hipMemcpyDeviceToHost));
+    if (root_ref >= 0) {
+        LMGPU_HIP(hipMemcpy(&rlo, node_lo.p + root_ref, sizeof(float4), hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&rhi, node_hi.p + root_ref, sizeof(float4), hipMemcpyDeviceToHost));
     } else {
         LMGPU_HIP(hipMemcpy(&rlo, leaf_lo.p, sizeof(float4), hipMemcpyDeviceToHost));
         LMGPU_HIP(hipMemcpy(&rhi, leaf_hi.p, sizeof(float4), hipMemcpyDeviceToHost));
@@ -501,11 +549,28 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     return LMGPU_OK;
 }
 
+// Resident grid of the persistent kernel: CUs x active blocks per CU.
+template <int W, bool ANY, bool COUNT>
+static int persistent_grid() {
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<W, ANY, COUNT>, kTraceBlock, 0);
+        grid = std::max(1, cus) * std::max(1, per);
+    }
+    return grid;
+}
+
 template <int W, bool ANY, bool COUNT>
 static void launch_trace(const Accel& a, const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n,
-                         int32_t* status, unsigned long long* counters, hipStream_t s) {
-    hipLaunchKernelGGL((k_trace<W, ANY, COUNT>), dim3(nblocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s,
-                       reinterpret_cast<const WNode<W>*>(a.nodes.p), a.tris.p, rays, hits, occl, n, status, counters);
+                         unsigned long long* next, int32_t* status, unsigned long long* counters, hipStream_t s) {
+    const int grid = int(std::min<int64_t>(persistent_grid<W, ANY, COUNT>(), (n + kTraceBlock - 1) / kTraceBlock));
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, next);
+    hipLaunchKernelGGL((k_trace<W, ANY, COUNT>), dim3(grid), dim3(kTraceBlock), 0, s,
+                       reinterpret_cast<const WNode<W>*>(a.nodes.p), a.tris.p, rays, hits, occl, n, next, status,
+                       counters, a.refill);
 }
 
 int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
@@ -517,14 +582,15 @@ int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t 
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL ray or result buffer");
     if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15)))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "rays and hits must be 16-byte aligned");
-    if (n > int64_t(0x7fffffff) * kTraceBlock / 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "too many rays");
     int32_t* st = status_dev();
     if (!st) return fail(LMGPU_ERR_OUT_OF_MEMORY, "status buffer");
-#define LMGPU_DISPATCH(WW)                                                                    \
-    if (width == WW) {                                                                        \
-        if (any) launch_trace<WW, true, false>(*this, rays, hits, occl, n, st, counters, s);  \
-        else if (counters) launch_trace<WW, false, true>(*this, rays, hits, occl, n, st, counters, s); \
-        else launch_trace<WW, false, false>(*this, rays, hits, occl, n, st, counters, s);     \
+    unsigned long long* next = next_counter();
+    if (!next) return fail(LMGPU_ERR_OUT_OF_MEMORY, "ray counter");
+#define LMGPU_DISPATCH(WW)                                                                          \
+    if (width == WW) {                                                                              \
+        if (any) launch_trace<WW, true, false>(*this, rays, hits, occl, n, next, st, counters, s);  \
+        else if (counters) launch_trace<WW, false, true>(*this, rays, hits, occl, n, next, st, counters, s); \
+        else launch_trace<WW, false, false>(*this, rays, hits, occl, n, next, st, counters, s);     \
     }
     LMGPU_DISPATCH(2)
     LMGPU_DISPATCH(4)

@@ -289,4 +289,132 @@ __device__ __forceinline__ TraceResult trace(const WNode<W>* __restrict__ nodes,
     return res;
 }
 
+// Persistent traversal: every lane owns one ray at a time; when at least
+// `refill` lanes of a wave are idle (or the whole wave is), the wave takes
+// that many new rays from one global counter with one atomic (Aila & Laine,
+// HPG 2009, "persistent threads"), so lanes do not sit idle behind a wave's
+// slowest ray. refill = 64 keeps 64 consecutive rays per wave (best for
+// coherent rays), small values keep lanes busy (best for incoherent rays).
+// Each loop iteration advances every active lane by one step: one wide node
+// or one leaf. `fetch(id, o, d, tmin, tmax)` loads ray `id`; `emit(id, res)`
+// consumes its result.
+template <int W, bool ANY, bool COUNT, int LDS_DEPTH, int PRIV_DEPTH, int BLOCK, class Fetch, class Emit>
+__device__ __forceinline__ void trace_persistent(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+                                                 int64_t n, unsigned long long* next, int (*lds)[BLOCK],
+                                                 Fetch fetch, Emit emit, int32_t* status,
+                                                 unsigned long long* counters, int refill) {
+    const int lane = threadIdx.x & 63;
+    Stack<LDS_DEPTH, PRIV_DEPTH, BLOCK> st;
+    st.lds = lds;
+    st.sp = 0;
+    st.lane = threadIdx.x;
+    RayPre r{};
+    TraceResult res{};
+    int64_t rid = -1;       // ray of this lane, -1 idle
+    bool exhausted = false;
+    float tmin = 0.f;
+    int cur = 0;
+    unsigned long long nv = 0, nt = 0;
+    while (true) {
+        const bool idle = rid < 0 && !exhausted;
+        const unsigned long long idle_mask = __ballot(idle);
+        const unsigned long long busy_mask = __ballot(rid >= 0);
+        const int nidle = __popcll(idle_mask);
+        if (nidle >= refill || (busy_mask == 0 && nidle > 0)) {
+            const int leader = __ffsll(idle_mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(next, (unsigned long long)nidle);
+            base = (unsigned long long)__shfl((long long)base, leader);
+            if (idle) {
+                const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull));
+                const int64_t id = int64_t(base) + rank;
+                if (id < n) {
+                    rid = id;
+                    f3 o, d;
+                    float tmax;
+                    fetch(id, o, d, tmin, tmax);
+                    r = ray_prepare(o, d);
+                    res.prim = -1;
+                    res.t = tmax;
+                    res.u = res.v = 0.f;
+                    res.best = -1;
+                    res.overflow = false;
+                    cur = 0;
+                    st.sp = 0;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(rid >= 0) == 0) break;
+        if (rid < 0) continue;
+        bool finished = false;
+        if (cur >= 0) {
+            const WNode<W>& nd = nodes[cur];
+            if (COUNT) nv++;
+            float key[W];
+            int ent[W];
+#pragma unroll
+            for (int c = 0; c < W; c++) {
+                bool hit;
+                const float tn = box_near(r, nd.lox[c], nd.loy[c], nd.loz[c], nd.hix[c], nd.hiy[c], nd.hiz[c],
+                                          tmin, res.t, hit);
+                const int cnt = nd.count[c];
+                const int ch = nd.child[c];
+                hit = hit && cnt >= 0;
+                key[c] = hit ? tn : __builtin_inff();
+                ent[c] = cnt == 0 ? ch : encode_leaf(ch, cnt);
+            }
+            sort_children<W>(key, ent);
+            bool ok = true;
+#pragma unroll
+            for (int c = W - 1; c >= 1; c--)
+                if (key[c] != __builtin_inff()) ok = ok && st.push(ent[c]);
+            if (!ok) {
+                res.overflow = true;
+                finished = true;
+            } else if (key[0] != __builtin_inff()) {
+                cur = ent[0];
+            } else if (st.sp == 0) {
+                finished = true;
+            } else {
+                cur = st.pop();
+            }
+        } else {
+            const int first = leaf_first(cur), cnt = leaf_count(cur);
+            for (int k = 0; k < cnt; k++) {
+                const TriV tv = tris[first + k];
+                if (COUNT) nt++;
+                float t, u, v;
+                if (tri_intersect(r, tv.v0, tv.v1, tv.v2, tmin, res.t, t, u, v)) {
+                    res.t = t;
+                    res.u = u;
+                    res.v = v;
+                    res.prim = __float_as_int(tv.v0.w);
+                    res.best = first + k;
+                    if (ANY) break;
+                }
+            }
+            if ((ANY && res.prim >= 0) || st.sp == 0) finished = true;
+            else cur = st.pop();
+        }
+        if (finished) {
+            if (res.overflow) atomicOr(status, 1);
+            if (!ANY && res.best >= 0) refine_hit(tris[res.best], r.o, r.d, res.t, res.u, res.v);
+            emit(rid, res);
+            rid = -1;
+        }
+    }
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) {
+            nv += __shfl_xor(nv, off);
+            nt += __shfl_xor(nt, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&counters[0], nv);
+            atomicAdd(&counters[1], nt);
+        }
+    }
+}
+
 }  // namespace lmgpu

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -13,6 +14,7 @@
 #include "lmgpu_internal.h"
 
 namespace lmgpu {
+
 
 static thread_local std::string g_last_error;
 

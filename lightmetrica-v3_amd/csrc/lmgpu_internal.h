@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -70,8 +71,11 @@ struct Accel {
     DevBuf<float4> nodes;   // W-wide nodes, 32*W bytes each
     DevBuf<TriV> tris;      // triangles in traversal order
     DevBuf<int32_t> status; // bit 0: traversal stack overflow
+    DevBuf<unsigned long long> ring;  // per-launch ray counters of the persistent kernel
+    mutable std::atomic<uint32_t> ring_pos{0};
     int width = 4;
     int leaf_size = 4;
+    int refill = 16;        // persistent-kernel refill threshold (1..64)
     int64_t num_tris = 0;
     int64_t num_nodes = 0;
     int64_t num_leaves = 0;
@@ -93,6 +97,17 @@ struct Accel {
             if (hipMemset(self->status.p, 0, sizeof(int32_t)) != hipSuccess) return nullptr;
         }
         return status.p;
+    }
+    // Ray counter for one persistent launch: a ring of slots so launches that
+    // overlap on different streams use different counters (up to kRing in
+    // flight at once).
+    static constexpr int kRing = 256;
+    unsigned long long* next_counter() const {
+        if (!ring.p) {
+            auto* self = const_cast<Accel*>(this);
+            if (self->ring.alloc(kRing) != LMGPU_OK) return nullptr;
+        }
+        return ring.p + (ring_pos.fetch_add(1) % kRing);
     }
     // Stack entries needed in the worst case: (W-1) pushes per level.
     int stack_bound() const { return (width - 1) * depth; }

@@ -168,42 +168,58 @@ template <>
 __device__ __forceinline__ const WNode<8>* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q) {
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[in_q];
     const lmgpu_ray* rays = q.ext[in_q];
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + id);
         const float4 a = rp[0], b = rp[1];
-        const TraceResult r = trace<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, mk(a.x, a.y, a.z),
-                                                                          mk(b.x, b.y, b.z), a.w, b.w, lds,
-                                                                          threadIdx.x);
-        reinterpret_cast<float4*>(q.hit)[i] =
-            make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
+        reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
+    };
+    trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
+                                                           q.counts + 5, nullptr, refill);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q) {
+__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q, unsigned long long* next, int refill) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float4* rp = reinterpret_cast<const float4*>(q.sh + i);
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
         const float4 a = rp[0], b = rp[1];
-        const TraceResult r = trace<W, true, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, mk(a.x, a.y, a.z),
-                                                                         mk(b.x, b.y, b.z), a.w, b.w, lds,
-                                                                         threadIdx.x);
-        if (r.overflow) atomicOr(q.counts + 5, 1);
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
         if (r.prim < 0 && !r.overflow) {
-            const float4 c = q.sh_contrib[i];
+            const float4 c = q.sh_contrib[id];
             const int p = __float_as_int(c.w);
             float4 L = ps.L[p];
-            L.x += c.x; L.y += c.y; L.z += c.z;
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
             ps.L[p] = L;
         }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 12), (unsigned long long)n);
+    };
+    trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
+                                                          q.counts + 5, nullptr, refill);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 12), (unsigned long long)n);
+}
+
+__global__ void k_bounce_begin(unsigned long long* next) {
+    if (threadIdx.x < 2) next[threadIdx.x] = 0ull;
 }
 
 // pdf of light endpoint `light` seen from the previous vertex (prev normal
@@ -475,8 +491,10 @@ struct Scene {
     DevBuf<lmgpu_ray> ext0, ext1, sh;
     DevBuf<lmgpu_hit> hit;
     DevBuf<int32_t> path0, path1, counts;
+    DevBuf<unsigned long long> next;   // per-bounce ray counters of the persistent trace kernels
     DevBuf<double> fsum, fsq;   // device film when the caller passes host pointers
     hipGraphExec_t graph = nullptr;
+    int refill_primary = 64, refill_secondary = 16;
     int graph_key[4] = {0, 0, 0, 0};
     uint64_t graph_seed = 0;
     int graph_sumsq = -1;
@@ -512,6 +530,7 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
     for (int i = 0; i < P; i++)
         if (d->mat_type[i] < 0 || d->mat_type[i] > 3) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad material type");
     LMGPU_TRY(accel.build(d->verts, N, false, opts, nullptr));
+    refill_secondary = accel.refill;
     w = d->w;
     h = d->h;
     // shading triangles
@@ -640,13 +659,28 @@ int Scene::ensure_workspace(int npix) {
     LMGPU_TRY(hit.alloc(npix));
     LMGPU_TRY(path0.alloc(npix)); LMGPU_TRY(path1.alloc(npix));
     LMGPU_TRY(counts.alloc(16));
+    LMGPU_TRY(next.alloc(2));
     LMGPU_HIP(hipMemset(counts.p, 0, 16 * sizeof(int32_t)));
     ws_npix = npix;
     if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
     return LMGPU_OK;
 }
 
-static int trace_grid(int n) {
+template <int W>
+static int pt_trace_grid() {
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, a = 0, b = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<W>, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<W>, kBlock, 0);
+        grid = std::max(1, cus) * std::max(1, std::min(a, b));
+    }
+    return grid;
+}
+
+static int shade_grid(int n) {
     // enough waves to fill 256 CUs several times over, bounded by the work
     return std::max(1, std::min((n + kBlock - 1) / kBlock, 256 * 8));
 }
@@ -657,18 +691,23 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream
     const int npix = c.npix;
     const unsigned gb = unsigned((npix + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_generate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
-    const int tg = trace_grid(npix);
+    const int sg = shade_grid(npix);
+    const int tg = accel.width == 4 ? pt_trace_grid<4>() : (accel.width == 8 ? pt_trace_grid<8>() : pt_trace_grid<2>());
     for (int b = 0; b < c.max_verts - 1; b++) {
         const int in_q = b & 1;
-        if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
-        else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
-        else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q);
+        // camera rays are coherent: keep 64 consecutive rays per wave
+        const int pm = b == 0 ? refill_primary : refill_secondary;
+        hipLaunchKernelGGL(k_bounce_begin, dim3(1), dim3(64), 0, s, next.p);
+        if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
+        else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
+        else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_shade, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, c, in_q);
+        hipLaunchKernelGGL(k_shade, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q);
         if (c.mode != LMGPU_SAMPLING_NAIVE) {
-            if (accel.width == 4) hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
-            else if (accel.width == 8) hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
-            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q);
+            unsigned long long* nx = next.p + 1;
+            if (accel.width == 4) hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
+            else if (accel.width == 8) hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
+            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
         }
     }
     hipLaunchKernelGGL(k_accumulate, dim3(gb), dim3(kBlock), 0, s, ps, sum, sumsq, npix);

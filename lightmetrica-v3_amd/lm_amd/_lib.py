@@ -49,7 +49,12 @@ class LmError(RuntimeError):
 
 
 class BuildOpts(ctypes.Structure):
-    _fields_ = [('width', ctypes.c_int32), ('leaf_size', ctypes.c_int32), ('reserved', ctypes.c_int32 * 6)]
+    _fields_ = [('width', ctypes.c_int32), ('leaf_size', ctypes.c_int32), ('refill', ctypes.c_int32),
+                ('builder', ctypes.c_int32), ('ploc_radius', ctypes.c_int32), ('reserved', ctypes.c_int32 * 3)]
+
+
+BUILDER_PLOC = 0
+BUILDER_LBVH = 1
 
 
 class AccelInfo(ctypes.Structure):

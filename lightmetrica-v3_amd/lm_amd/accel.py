@@ -67,9 +67,12 @@ def _stream_ptr(stream) -> Optional[int]:
 class Accel:
     """accel::mi355x: GPU LBVH collapsed to a W-wide BVH + ordered traversal."""
 
-    def __init__(self, width: int = 4, leaf_size: int = 4):
+    def __init__(self, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1, radius: int = 16):
         self.width = width
         self.leaf_size = leaf_size
+        self.refill = refill
+        self.builder = builder
+        self.radius = radius
         h = ctypes.c_void_p()
         _lib.call('lmgpu_accel_create', ctypes.byref(h))
         self._h = h
@@ -95,7 +98,7 @@ class Accel:
             verts = scene_or_verts.verts
             self.tri_prim = np.asarray(scene_or_verts.tri_prim)
             self.tri_face = np.asarray(scene_or_verts.tri_face)
-        opts = _lib.BuildOpts(self.width, self.leaf_size)
+        opts = _lib.BuildOpts(self.width, self.leaf_size, self.refill, self.builder, self.radius)
         try:
             import torch
             is_tensor = isinstance(verts, torch.Tensor)

@@ -1,0 +1,59 @@
+"""Profile driver: a fixed number of closest-hit launches over the config-1
+incoherent rays (and, optionally, one wavefront path-tracing pass), preceded
+by a plain device copy of known size that calibrates the HBM byte counters.
+
+Run under rocprofv3, e.g.
+  rocprofv3 --kernel-trace --stats -d OUT -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py
+  rocprofv3 --pmc FETCH_SIZE -d OUT -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py
+"""
+import argparse
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+COPY_BYTES = 512 * 1024 * 1024
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--launches', type=int, default=5)
+    ap.add_argument('--width', type=int, default=4)
+    ap.add_argument('--leaf', type=int, default=2)
+    ap.add_argument('--rays', default='incoherent')
+    ap.add_argument('--pt', action='store_true', help='also run one 1080p config-2 PT pass')
+    args = ap.parse_args()
+    import torch
+    from lm_amd import scenes
+    from lm_amd.accel import Accel, rays_from_dict
+
+    dev = torch.device('cuda:0')
+    # calibration: one copy of COPY_BYTES (reads and writes COPY_BYTES)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f'copy calibration: {COPY_BYTES} bytes read + written', flush=True)
+    del src, dst
+    scene, rays = scenes.config1()
+    flat = scene.flatten()
+    acc = Accel(args.width, args.leaf)
+    acc.build(flat)
+    r = torch.from_numpy(rays_from_dict(rays[args.rays])).to(dev)
+    h = torch.zeros((r.shape[0], 4), dtype=torch.float32, device=dev)
+    for i in range(args.launches):
+        acc.intersect_device(r, h)
+    torch.cuda.synchronize()
+    print(f'{args.launches} x k_trace over {r.shape[0]} {args.rays} rays', flush=True)
+    if args.pt:
+        from lm_amd.renderer import GpuScene
+        s2 = scenes.config2().flatten()
+        sc = GpuScene(s2, width=args.width, leaf_size=args.leaf)
+        sc.render_raw(spp=1, max_verts=16, seed=1)
+        torch.cuda.synchronize()
+        print('1 x 1080p config-2 PT pass', flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -74,7 +74,8 @@ class GpuScene:
     """Device scene (geometry, accel, materials, lights, camera) built from a
     lm_amd.scenes.FlatScene through lmgpu_scene_create."""
 
-    def __init__(self, flat, width: int = 4, leaf_size: int = 4):
+    def __init__(self, flat, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1,
+                 radius: int = 16):
         self.flat = flat
         self._keep = {
             'verts': np.ascontiguousarray(flat.verts, dtype=np.float32),
@@ -107,7 +108,7 @@ class GpuScene:
         d.aspect = float(flat.camera.aspect)
         d.w = int(flat.w)
         d.h = int(flat.h)
-        opts = _lib.BuildOpts(width, leaf_size)
+        opts = _lib.BuildOpts(width, leaf_size, refill, builder, radius)
         h = ctypes.c_void_p()
         _lib.call('lmgpu_scene_create', ctypes.byref(h), ctypes.byref(d), ctypes.byref(opts))
         self._h = h
