@@ -210,6 +210,39 @@ typedef struct lmgpu_pt_stats {
 int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* film_sum, double* film_sumsq,
                     int film_on_device, lmgpu_pt_stats* stats, void* stream);
 
+/* ---- learned path guiding --------------------------------------------------
+ * No counterpart in the reference: an MLP maps (position, shading normal,
+ * outgoing direction) to a mixture of 8 von Mises-Fisher lobes over incident
+ * directions; it is trained online on radiance recorded by the path tracer
+ * and mixed 50/50 with BSDF sampling by one-sample MIS (lmgpu_pt_render_guided),
+ * so the rendered estimate stays unbiased for any network state. */
+#define LMGPU_GUIDE_NPARAMS 17664   /* packed FP32 parameter slots (15976 trained) */
+#define LMGPU_GUIDE_MIX 40          /* decoded mixture: pi[8], kappa[8], mu[8][3] */
+#define LMGPU_GUIDE_RECORD 16       /* training record floats: pos3 nrm3 wo3 wi3 pdf L3 */
+
+typedef struct lmgpu_guide lmgpu_guide;
+
+/* box_lo/box_hi: scene bounds used to normalise positions. */
+int lmgpu_guide_create(lmgpu_guide** out, const float* box_lo, const float* box_hi, uint64_t seed);
+int lmgpu_guide_destroy(lmgpu_guide* guide);
+/* Copies the FP32 parameters out (get, host, LMGPU_GUIDE_NPARAMS floats) and/or
+ * replaces them (set; resets the Adam state). Either may be NULL. */
+int lmgpu_guide_params(lmgpu_guide* guide, float* get, const float* set);
+/* Decoded mixtures for n queries: query[n][9] = pos, nrm, wo; mix[n][40]
+ * (device pointers, asynchronous on stream). */
+int lmgpu_guide_infer(const lmgpu_guide* guide, const float* query, float* mix, int64_t n, void* stream);
+/* Samples one direction per decoded mixture (u[n][3] uniforms: lobe, two for
+ * the vMF) into dirs[n][3] and, if pdf != NULL, its mixture pdf (solid
+ * angle). Device pointers, asynchronous. */
+int lmgpu_guide_sample(const float* mix, const float* u, float* dirs, float* pdf, int64_t n, void* stream);
+/* Mixture pdf (solid angle) of dirs[n][3] under mix[n][40]. */
+int lmgpu_guide_pdf(const float* mix, const float* dirs, float* pdf, int64_t n, void* stream);
+/* One Adam step (learning rate lr) on n training records (device pointer,
+ * [n][16]); the loss (host) and the gradient before the step (host,
+ * LMGPU_GUIDE_NPARAMS floats) are returned when non-NULL. */
+int lmgpu_guide_train(lmgpu_guide* guide, const float* records, int64_t n, float lr, float* loss, float* grads,
+                      void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -1,0 +1,216 @@
+// Learned path-guiding distribution: device-side definitions.
+//
+// The guide maps a surface point (position, shading normal, outgoing
+// direction) to a mixture of kLobes von Mises-Fisher lobes over incident
+// directions. Network: 12-frequency sin/cos encoding of the position in the
+// scene box (72) + normal (3) + outgoing direction (3) = 78 inputs (padded to
+// 80), three 64-wide ReLU layers, a 40-wide linear output: per lobe a weight
+// logit, a concentration (softplus) and a mean direction (normalised).
+// Matrix products run on MFMA (v_mfma_f32_32x32x16_bf16: BF16 operands, FP32
+// accumulation); master weights and Adam state are FP32.
+#pragma once
+
+#include "common.h"
+
+namespace lmgpu {
+
+constexpr int kLobes = 8;
+constexpr int kFreq = 12;
+constexpr int kIn = 80;      // 78 inputs + 2 zero pads
+constexpr int kHid = 64;
+constexpr int kOut = 40;     // logits[8], kappa[8], mu[8][3]
+constexpr int kOutPad = 64;
+// Packed parameter layout (FP32 master copy), weights stored transposed
+// (output neuron major): Wt1[64][80], b1[64], Wt2[64][64], b2[64],
+// Wt3[64][64], b3[64], Wt4[64][64] (rows >= 40 zero), b4[64] (>= 40 zero).
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kHid * kIn;
+constexpr int kOffW2 = kOffB1 + kHid;
+constexpr int kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid;
+constexpr int kOffB3 = kOffW3 + kHid * kHid;
+constexpr int kOffW4 = kOffB3 + kHid;
+constexpr int kOffB4 = kOffW4 + kOutPad * kHid;
+constexpr int kNParams = kOffB4 + kOutPad;   // 17664 slots, 15976 trained
+constexpr float kKappaMin = 0.05f, kKappaMax = 4000.f;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Round-to-nearest-even FP32 -> BF16 bits (finite inputs).
+LMGPU_HD uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+LMGPU_HD float bf2f(uint16_t h) {
+    const uint32_t u = uint32_t(h) << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// Decoded mixture of one query point.
+struct Mixture {
+    float pi[kLobes];
+    float kappa[kLobes];
+    f3 mu[kLobes];
+};
+
+__device__ __forceinline__ float softplus(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+
+// raw network outputs (40 floats) -> mixture parameters
+__device__ __forceinline__ void decode_mixture(const float* y, Mixture& m) {
+    float mx = y[0];
+    for (int k = 1; k < kLobes; k++) mx = fmaxf(mx, y[k]);
+    float s = 0.f;
+    for (int k = 0; k < kLobes; k++) {
+        m.pi[k] = expf(y[k] - mx);
+        s += m.pi[k];
+    }
+    const float inv = 1.f / s;
+    for (int k = 0; k < kLobes; k++) {
+        m.pi[k] *= inv;
+        m.kappa[k] = fminf(fmaxf(softplus(y[kLobes + k]), kKappaMin), kKappaMax);
+        const f3 r = mk(y[2 * kLobes + 3 * k], y[2 * kLobes + 3 * k + 1], y[2 * kLobes + 3 * k + 2]);
+        const float l = length(r);
+        m.mu[k] = l > 1e-8f ? r / l : mk(0.f, 0.f, 1.f);
+    }
+}
+
+// vMF density on the unit sphere (solid angle), stable for large kappa:
+// kappa / (2 pi (1 - exp(-2 kappa))) * exp(kappa (mu.w - 1)).
+__device__ __forceinline__ float vmf_pdf(float kappa, f3 mu, f3 w) {
+    return kappa / (2.f * 3.14159265358979f * (-expm1f(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
+}
+
+__device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
+    float p = 0.f;
+    for (int k = 0; k < kLobes; k++) p += m.pi[k] * vmf_pdf(m.kappa[k], m.mu[k], w);
+    return p;
+}
+
+// Samples the mixture: lobe by u0, direction by (u1, u2) with the stable vMF
+// inversion of W. Jakob, "Numerically stable sampling of the von
+// Mises-Fisher distribution on S^2" (2012).
+__device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+    int k = 0;
+    float c = m.pi[0];
+    while (k < kLobes - 1 && u0 >= c) {
+        k++;
+        c += m.pi[k];
+    }
+    const float kap = m.kappa[k];
+    // W in [-1, 1]; the clamp covers u1 = 0 with exp(-2 kappa) underflowing
+    const float W = fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap);
+    const float sw = sqrtf(fmaxf(0.f, 1.f - W * W));
+    float s, co;
+    sincosf(2.f * 3.14159265358979f * u2, &s, &co);
+    const f3 mu = m.mu[k];
+    const float sg = copysignf(1.0f, mu.z);
+    const float a = -1.0f / (sg + mu.z);
+    const float b = mu.x * mu.y * a;
+    const f3 t1 = mk(1.0f + sg * mu.x * mu.x * a, sg * b, -sg * mu.x);
+    const f3 t2 = mk(b, sg + mu.y * mu.y * a, -mu.y);
+    return normalize(mu * W + (t1 * co + t2 * s) * sw);
+}
+
+// Input encoding of one query point into 80 features (bf16 bits).
+// Position normalised to [0,1]^3 in the scene box; frequencies 2^j * pi.
+__device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box_lo, const float* box_inv,
+                                             uint16_t* out) {
+    const float q[3] = {(p.x - box_lo[0]) * box_inv[0], (p.y - box_lo[1]) * box_inv[1], (p.z - box_lo[2]) * box_inv[2]};
+    int o = 0;
+    for (int a = 0; a < 3; a++) {
+        float f = 3.14159265358979f * q[a];
+        for (int j = 0; j < kFreq; j++) {
+            float s, c;
+            sincosf(f, &s, &c);
+            out[o++] = f2bf(s);
+            out[o++] = f2bf(c);
+            f *= 2.f;
+        }
+    }
+    out[o++] = f2bf(n.x);
+    out[o++] = f2bf(n.y);
+    out[o++] = f2bf(n.z);
+    out[o++] = f2bf(wo.x);
+    out[o++] = f2bf(wo.y);
+    out[o++] = f2bf(wo.z);
+    out[o++] = 0;
+    out[o++] = 0;
+}
+
+}  // namespace lmgpu
+
+namespace lmgpu {
+
+// LDS pitches (in bf16 / f32 elements) of the per-wave tile buffers.
+constexpr int kPitchX = 88;    // [32][88] bf16: inputs / even layers
+constexpr int kPitchH = 72;    // [32][72] bf16: odd layers
+constexpr int kPitchY = 68;    // [32][68] f32: outputs
+constexpr int kTileLdsBytes = 32 * kPitchX * 2 + 32 * kPitchH * 2 + 32 * kPitchY * 4;
+
+__device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// One dense layer for the 32 rows of a wave tile: out = act(in[32][K] W + b)
+// with in from LDS (bf16, pitch pin), weights Wt[N][K] bf16 in global
+// memory (output-neuron major) and N = 64. The MFMA result layout (lane l,
+// element r: row 8(r/4) + 4(l/32) + r%4, column 32t + l%32) is written
+// back to LDS by `store(row, col, value)`.
+template <int K, class Store>
+__device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin, const uint16_t* __restrict__ Wt,
+                                        const float* __restrict__ bias, int lane, Store store) {
+    f32x16 acc0 = {}, acc1 = {};
+    const int r = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r * pin + k0 + 8 * h);
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(Wt + r * K + k0 + 8 * h);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(Wt + (32 + r) * K + k0 + 8 * h);
+        acc0 = mfma_bf16(a, b0, acc0);
+        acc1 = mfma_bf16(a, b1, acc1);
+    }
+    const float bb0 = bias[r], bb1 = bias[32 + r];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int row = 8 * (e >> 2) + 4 * h + (e & 3);
+        store(row, r, acc0[e] + bb0);
+        store(row, 32 + r, acc1[e] + bb1);
+    }
+}
+
+// Forward pass of the guide MLP for the 32 rows of a wave tile whose encoded
+// inputs are in lx ([32][kPitchX] bf16). Raw outputs land in ly
+// ([32][kPitchY] f32, columns 0..39). `save(layer, row, col, bf16)` sees
+// every hidden activation (layer 1..3) for training. Callers put a block
+// barrier between writing lx and calling this, and after it before reading ly.
+template <class Save>
+__device__ __forceinline__ void mlp_tile_forward(const uint16_t* __restrict__ Wbf, const float* __restrict__ P,
+                                                 uint16_t* lx, uint16_t* lh, float* ly, int lane, Save save) {
+    dense64<kIn>(lx, kPitchX, Wbf + kOffW1, P + kOffB1, lane, [&](int row, int col, float v) {
+        const uint16_t b = f2bf(fmaxf(v, 0.f));
+        lh[row * kPitchH + col] = b;
+        save(1, row, col, b);
+    });
+    __syncthreads();
+    dense64<kHid>(lh, kPitchH, Wbf + kOffW2, P + kOffB2, lane, [&](int row, int col, float v) {
+        const uint16_t b = f2bf(fmaxf(v, 0.f));
+        lx[row * kPitchX + col] = b;
+        save(2, row, col, b);
+    });
+    __syncthreads();
+    dense64<kHid>(lx, kPitchX, Wbf + kOffW3, P + kOffB3, lane, [&](int row, int col, float v) {
+        const uint16_t b = f2bf(fmaxf(v, 0.f));
+        lh[row * kPitchH + col] = b;
+        save(3, row, col, b);
+    });
+    __syncthreads();
+    dense64<kHid>(lh, kPitchH, Wbf + kOffW4, P + kOffB4, lane,
+                  [&](int row, int col, float v) { ly[row * kPitchY + col] = v; });
+}
+
+}  // namespace lmgpu

@@ -1,0 +1,38 @@
+// Host-side guide handle behind lmgpu_guide (guide.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lmgpu_internal.h"
+
+namespace lmgpu {
+
+// Training record written by the path tracer and read by Guide::train:
+// pos(3), shading normal(3), outgoing direction(3), sampled incident
+// direction(3), mixture pdf of that direction (solid angle), incident
+// radiance estimate L(3).
+constexpr int kRecordFloats = 16;
+
+struct Guide {
+    DevBuf<float> P, M, V, G;       // parameters, Adam moments, gradients (packed layout)
+    DevBuf<uint16_t> Wbf, WnBf;     // BF16 copies: packed transposed weights; layers 2..4 as W[k][n]
+    DevBuf<float> zero64, scal;     // zero bias; scalars: [0] weight sum, [1] loss
+    DevBuf<uint16_t> xT, hT, dT;    // training activations / deltas, feature major
+    DevBuf<float> y, w;             // raw outputs, target weights
+    int64_t train_cap = 0;
+    float box_lo[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
+    float wclamp = 50.f;            // clamp of normalised target weights
+    int step = 0;
+
+    int create(const float* lo, const float* hi, uint64_t seed);
+    int set_params(const float* host);
+    int get_params(float* host) const;
+    int ensure_train(int64_t B);
+    // query: n rows of `qs` floats starting with pos(3), nrm(3), wo(3)
+    int infer(const float* query, int qs, float* mix, int64_t n, hipStream_t s) const;
+    int train(const float* rec, int64_t n, float lr, float* loss, float* grad, hipStream_t s);
+};
+
+}  // namespace lmgpu

@@ -37,6 +37,8 @@ EXPORTED = [
     'lmgpu_accel_intersect', 'lmgpu_accel_intersect_host', 'lmgpu_accel_occluded', 'lmgpu_accel_status',
     'lmgpu_accel_intersect_counted',
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
+    'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
+    'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train',
 ]
 
 
@@ -143,6 +145,14 @@ def lib():
         L.lmgpu_scene_destroy.argtypes = [vp]
         L.lmgpu_scene_accel.argtypes = [vp, ctypes.POINTER(vp)]
         L.lmgpu_pt_render.argtypes = [vp, ctypes.POINTER(PTParams), vp, vp, ctypes.c_int, ctypes.POINTER(PTStats), vp]
+        f32p = ctypes.POINTER(ctypes.c_float)
+        L.lmgpu_guide_create.argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_uint64]
+        L.lmgpu_guide_destroy.argtypes = [vp]
+        L.lmgpu_guide_params.argtypes = [vp, vp, vp]
+        L.lmgpu_guide_infer.argtypes = [vp, vp, vp, i64, vp]
+        L.lmgpu_guide_sample.argtypes = [vp, vp, vp, vp, i64, vp]
+        L.lmgpu_guide_pdf.argtypes = [vp, vp, vp, i64, vp]
+        L.lmgpu_guide_train.argtypes = [vp, vp, i64, ctypes.c_float, f32p, vp, vp]
         for name in EXPORTED:
             if name not in ('lmgpu_last_error', 'lmgpu_version'):
                 getattr(L, name).restype = ctypes.c_int
