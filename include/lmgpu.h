@@ -187,7 +187,7 @@ typedef struct lmgpu_pt_params {
     int32_t rr_depth;         /* Russian roulette when num_verts > rr_depth (reference 5) */
     uint64_t seed;            /* Philox key */
     int32_t sample_offset;    /* index of the first sample (progressive rendering) */
-    int32_t guiding;          /* 0 = plain PT; 1 = MLP path guiding (lmgpu_guide) */
+    int32_t guiding;          /* must be 0: guided rendering is lmgpu_pt_render_guided */
     double render_time;       /* > 0: scheduler::spp::time, render 1-spp passes for this many seconds */
     int32_t reserved[8];
 } lmgpu_pt_params;
@@ -198,7 +198,8 @@ typedef struct lmgpu_pt_stats {
     int64_t extension_rays;
     int64_t shadow_rays;
     double elapsed_s;         /* wall time of the render call */
-    int64_t reserved[4];
+    int64_t guide_records;    /* guided rendering: training records collected */
+    int64_t reserved[3];
 } lmgpu_pt_stats;
 
 /* Accumulates, per pixel, the sum and the sum of squares of the per-sample
@@ -242,6 +243,32 @@ int lmgpu_guide_pdf(const float* mix, const float* dirs, float* pdf, int64_t n, 
  * LMGPU_GUIDE_NPARAMS floats) are returned when non-NULL. */
 int lmgpu_guide_train(lmgpu_guide* guide, const float* records, int64_t n, float lr, float* loss, float* grads,
                       void* stream);
+/* Adam steps taken so far and the loss of the last one (synchronous). */
+int lmgpu_guide_stats(const lmgpu_guide* guide, int64_t* steps, float* last_loss);
+
+/* Guided renderer::pt. At diffuse and glossy vertices the next direction is
+ * drawn from the guide with probability alpha and from the BSDF otherwise;
+ * the mixture density is used in the throughput and in all MIS weights, so
+ * the estimate is unbiased for any guide. After every 1-spp pass the
+ * radiance that arrived along the sampled directions of the first
+ * record_depth surface vertices is turned into training records (pos, nrm,
+ * wo, wi, p_mix, f|cos|L_i), up to batch * steps_per_pass of them in a
+ * film- and depth-spread order, and the guide takes steps_per_pass Adam
+ * steps; the next pass samples the updated guide. The whole pass, training
+ * included, is one captured hipGraph. */
+typedef struct lmgpu_guide_opts {
+    float alpha;              /* guide sampling probability (0.5) */
+    float learning_rate;      /* Adam learning rate (1e-3) */
+    int32_t batch;            /* records per Adam step, rounded up to 128 (2^18) */
+    int32_t steps_per_pass;   /* Adam steps per pass (1); 0 = frozen guide, nothing recorded */
+    int32_t record_depth;     /* recording surface vertices per path (8, at most 16) */
+    int32_t reserved[7];
+} lmgpu_guide_opts;
+
+/* As lmgpu_pt_render; opts may be NULL (the defaults above). */
+int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lmgpu_guide* guide,
+                           const lmgpu_guide_opts* opts, double* film_sum, double* film_sumsq, int film_on_device,
+                           lmgpu_pt_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }  /* extern "C" */

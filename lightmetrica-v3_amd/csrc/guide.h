@@ -85,29 +85,33 @@ __device__ __forceinline__ float vmf_pdf(float kappa, f3 mu, f3 w) {
     return kappa / (2.f * 3.14159265358979f * (-expm1f(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
 }
 
-__device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
+// Density (solid angle) and sampling of a decoded mixture (pi[8], kappa[8],
+// mu[8][3]) read lobe by lobe from memory. Sampling picks the lobe by u0 and
+// the direction by (u1, u2) with the stable vMF inversion of W. Jakob,
+// "Numerically stable sampling of the von Mises-Fisher distribution on S^2"
+// (2012).
+__device__ __forceinline__ float mixture_pdf(const float* __restrict__ o, f3 w) {
     float p = 0.f;
-    for (int k = 0; k < kLobes; k++) p += m.pi[k] * vmf_pdf(m.kappa[k], m.mu[k], w);
+    for (int k = 0; k < kLobes; k++) {
+        const f3 mu = mk(o[2 * kLobes + 3 * k], o[2 * kLobes + 3 * k + 1], o[2 * kLobes + 3 * k + 2]);
+        p += o[k] * vmf_pdf(o[kLobes + k], mu, w);
+    }
     return p;
 }
 
-// Samples the mixture: lobe by u0, direction by (u1, u2) with the stable vMF
-// inversion of W. Jakob, "Numerically stable sampling of the von
-// Mises-Fisher distribution on S^2" (2012).
-__device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+__device__ __forceinline__ f3 mixture_sample(const float* __restrict__ o, float u0, float u1, float u2) {
     int k = 0;
-    float c = m.pi[0];
+    float c = o[0];
     while (k < kLobes - 1 && u0 >= c) {
         k++;
-        c += m.pi[k];
+        c += o[k];
     }
-    const float kap = m.kappa[k];
-    // W in [-1, 1]; the clamp covers u1 = 0 with exp(-2 kappa) underflowing
+    const float kap = o[kLobes + k];
     const float W = fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap);
     const float sw = sqrtf(fmaxf(0.f, 1.f - W * W));
     float s, co;
     sincosf(2.f * 3.14159265358979f * u2, &s, &co);
-    const f3 mu = m.mu[k];
+    const f3 mu = mk(o[2 * kLobes + 3 * k], o[2 * kLobes + 3 * k + 1], o[2 * kLobes + 3 * k + 2]);
     const float sg = copysignf(1.0f, mu.z);
     const float a = -1.0f / (sg + mu.z);
     const float b = mu.x * mu.y * a;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -41,15 +42,26 @@ struct Box {
 
 __device__ __forceinline__ float lum(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
 
+// Rows of a training batch actually present: n, or, when the count lives in
+// device memory (batches compacted by the path tracer inside a captured
+// graph), clamp(*ndev - off, 0, n).
+__device__ __forceinline__ int64_t batch_rows(const int32_t* ndev, int64_t off, int64_t n) {
+    if (!ndev) return n;
+    return std::min<int64_t>(n, std::max<int64_t>(0, int64_t(*ndev) - off));
+}
+__device__ __forceinline__ int64_t round128(int64_t n) { return (n + 127) / 128 * 128; }
+
 // Forward over query arrays. query[i] = pos(3), nrm(3), wo(3) (stride qs
 // floats). INFER: decoded mixtures mix[i][40] = pi[8], kappa[8], mu[8][3].
 // TRAIN: saves xT[96][B], hT[3][64][B] (bf16, feature major) and raw y[i][40].
 template <bool TRAIN>
-__global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n, int64_t B, const float* __restrict__ query, int qs,
+__global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+                                                       const float* __restrict__ query, int qs,
                                                        const uint16_t* __restrict__ Wbf, const float* __restrict__ P,
                                                        Box box, float* __restrict__ mix, uint16_t* __restrict__ xT,
                                                        uint16_t* __restrict__ hT, float* __restrict__ yout) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
+    const int64_t n = batch_rows(ndev, off, n_max);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
@@ -102,9 +114,11 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n, int64_t B, con
 
 // Target weights w_i = lum(L_i) / p_mix_i of training records
 // rec[i] = pos(3), nrm(3), wo(3), wi(3), p_mix, L(3); sum into *wsum.
-__global__ __launch_bounds__(256) void k_guide_weights(int64_t n, const float* __restrict__ rec,
-                                                       float* __restrict__ w, float* __restrict__ wsum) {
+__global__ __launch_bounds__(256) void k_guide_weights(int64_t n_max, const int32_t* ndev, int64_t off,
+                                                       const float* __restrict__ rec, float* __restrict__ w,
+                                                       float* __restrict__ wsum) {
     __shared__ float red[4];
+    const int64_t n = batch_rows(ndev, off, n_max);
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     float v = 0.f;
     if (i < n) {
@@ -114,21 +128,26 @@ __global__ __launch_bounds__(256) void k_guide_weights(int64_t n, const float* _
         v = (p > 0.f && l > 0.f && isfinite(l / p)) ? l / p : 0.f;
         w[i] = v;
     }
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(wsum, red[0] + red[1] + red[2] + red[3]);
 }
 
 // Per-sample gradient of -wt log p_guide(w_i) w.r.t. the 40 raw outputs,
-// scaled by 1/B, written feature-major as bf16 into d4T[64][B].
-__global__ __launch_bounds__(256) void k_guide_loss(int64_t n, int64_t B, const float* __restrict__ y,
-                                                    const float* __restrict__ rec, const float* __restrict__ w,
-                                                    const float* __restrict__ wsum, float wclamp,
-                                                    uint16_t* __restrict__ d4T, float* __restrict__ loss) {
+// scaled by 1/n, written feature-major as bf16 into d4T[64][B]; rows from n
+// to the end of the last 128-row tile are zeroed.
+__global__ __launch_bounds__(256) void k_guide_loss(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+                                                    const float* __restrict__ y, const float* __restrict__ rec,
+                                                    const float* __restrict__ w, const float* __restrict__ wsum,
+                                                    float wclamp, uint16_t* __restrict__ d4T,
+                                                    float* __restrict__ loss) {
     __shared__ float red[4];
+    const int64_t n = batch_rows(ndev, off, n_max);
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     float li = 0.f;
+    if (i >= n && i < round128(n))
+        for (int k = 0; k < kOut; k++) d4T[int64_t(k) * B + i] = 0;
     if (i < n) {
         const float mean = *wsum / float(n);
         const float wt = mean > 0.f ? fminf(w[i] / mean, wclamp) : 0.f;
@@ -172,22 +191,24 @@ __global__ __launch_bounds__(256) void k_guide_loss(int64_t n, int64_t B, const 
         }
         for (int k = 0; k < kOut; k++) d4T[int64_t(k) * B + i] = f2bf(g[k]);
     }
-    for (int off = 32; off > 0; off >>= 1) li += __shfl_xor(li, off);
+    for (int sh = 32; sh > 0; sh >>= 1) li += __shfl_xor(li, sh);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = li;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / float(n));
+    if (threadIdx.x == 0 && n > 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / float(n));
 }
 
 // Backward through layers 4..2 for 32-row wave tiles: delta_{l-1} =
 // (delta_l W_l^T) * relu'(h_{l-1}); saves d3T, d2T, d1T (bf16, feature major).
-__global__ __launch_bounds__(kGB) void k_guide_backward(int64_t B, const uint16_t* __restrict__ WnBf,
+__global__ __launch_bounds__(kGB) void k_guide_backward(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+                                                        const uint16_t* __restrict__ WnBf,
                                                         const float* __restrict__ zero64,
                                                         const uint16_t* __restrict__ hT, uint16_t* __restrict__ dT) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
-    for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < B; base += int64_t(gridDim.x) * kRowsPerBlock) {
+    const int64_t rows = round128(batch_rows(ndev, off, n_max));
+    for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < rows; base += int64_t(gridDim.x) * kRowsPerBlock) {
         const int64_t i0 = base + wave * 32;
         if (lane < 32) {
             const int64_t i = i0 + lane;
@@ -231,9 +252,9 @@ struct DwJob {
 };
 __constant__ DwJob c_dw_jobs[18];
 
-__global__ __launch_bounds__(64) void k_guide_dw(int64_t B, int slice, const uint16_t* __restrict__ xT,
-                                                 const uint16_t* __restrict__ hT, const uint16_t* __restrict__ dT,
-                                                 float* __restrict__ grad) {
+__global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B, int slice,
+                                                 const uint16_t* __restrict__ xT, const uint16_t* __restrict__ hT,
+                                                 const uint16_t* __restrict__ dT, float* __restrict__ grad) {
     const DwJob job = c_dw_jobs[blockIdx.y];
     const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
     const int K = job.layer == 1 ? kIn : kHid;
@@ -241,35 +262,37 @@ __global__ __launch_bounds__(64) void k_guide_dw(int64_t B, int slice, const uin
     const uint16_t* A = dT + int64_t(job.layer - 1) * kHid * B + int64_t(job.nt * 32 + r) * B;
     const uint16_t* Bm = (job.layer == 1 ? xT : hT + int64_t(job.layer - 2) * kHid * B) + int64_t(job.kt * 32 + r) * B;
     const int64_t i_beg = int64_t(blockIdx.x) * slice;
-    const int64_t i_end = std::min<int64_t>(B, i_beg + slice);
+    const int64_t i_end = std::min<int64_t>(round128(batch_rows(ndev, off, n_max)), i_beg + slice);
     f32x16 acc = {};
     for (int64_t i = i_beg; i < i_end; i += 16) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + i + 8 * h);
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bm + i + 8 * h);
         acc = mfma_bf16(a, b, acc);
     }
-    const int off = job.layer == 1 ? kOffW1 : (job.layer == 2 ? kOffW2 : (job.layer == 3 ? kOffW3 : kOffW4));
+    const int poff = job.layer == 1 ? kOffW1 : (job.layer == 2 ? kOffW2 : (job.layer == 3 ? kOffW3 : kOffW4));
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int n = job.nt * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
         const int k = job.kt * 32 + r;
-        if (n < N && k < K) atomicAdd(&grad[off + n * K + k], acc[e]);
+        if (n < N && k < K) atomicAdd(&grad[poff + n * K + k], acc[e]);
     }
 }
 
 // Bias gradients: db_l[n] = sum_i d_l[n][i]; one block per (layer, n).
-__global__ __launch_bounds__(256) void k_guide_db(int64_t B, const uint16_t* __restrict__ dT, float* __restrict__ grad) {
+__global__ __launch_bounds__(256) void k_guide_db(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+                                                  const uint16_t* __restrict__ dT, float* __restrict__ grad) {
     __shared__ float red[4];
     const int layer = blockIdx.x / kHid + 1, n = blockIdx.x % kHid;
     const uint16_t* d = dT + (int64_t(layer - 1) * kHid + n) * B;
+    const int64_t rows = round128(batch_rows(ndev, off, n_max));
     float s = 0.f;
-    for (int64_t i = threadIdx.x; i < B; i += 256) s += bf2f(d[i]);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    for (int64_t i = threadIdx.x; i < rows; i += 256) s += bf2f(d[i]);
+    for (int sh = 32; sh > 0; sh >>= 1) s += __shfl_xor(s, sh);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int off = layer == 1 ? kOffB1 : (layer == 2 ? kOffB2 : (layer == 3 ? kOffB3 : kOffB4));
-        if (layer != 4 || n < kOut) grad[off + n] = red[0] + red[1] + red[2] + red[3];
+        const int poff = layer == 1 ? kOffB1 : (layer == 2 ? kOffB2 : (layer == 3 ? kOffB3 : kOffB4));
+        if (layer != 4 || n < kOut) grad[poff + n] = red[0] + red[1] + red[2] + red[3];
     }
 }
 
@@ -294,13 +317,18 @@ __global__ __launch_bounds__(256) void k_guide_sync(const float* __restrict__ P,
     if (j < kNParams) write_bf16(j, P[j], Wbf, WnBf);
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes BF16 copies.
-__global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ P, float* __restrict__ M, float* __restrict__ V,
+// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16
+// copies. The step count t lives in device memory (k_guide_step advances it
+// afterwards); an empty batch leaves everything unchanged.
+__global__ __launch_bounds__(256) void k_guide_adam(int64_t n_max, const int32_t* ndev, int64_t off,
+                                                    const int32_t* __restrict__ stepc, float* __restrict__ P,
+                                                    float* __restrict__ M, float* __restrict__ V,
                                                     const float* __restrict__ G, float lr, float b1, float b2, float eps,
-                                                    float c1, float c2, uint16_t* __restrict__ Wbf,
-                                                    uint16_t* __restrict__ WnBf) {
+                                                    uint16_t* __restrict__ Wbf, uint16_t* __restrict__ WnBf) {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= kNParams) return;
+    if (j >= kNParams || batch_rows(ndev, off, n_max) == 0) return;
+    const float t = float(*stepc + 1);
+    const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
     const float g = G[j];
     const float m = b1 * M[j] + (1.f - b1) * g;
     const float v = b2 * V[j] + (1.f - b2) * g * g;
@@ -311,17 +339,13 @@ __global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ P, float
     write_bf16(j, p, Wbf, WnBf);
 }
 
+__global__ void k_guide_step(int64_t n_max, const int32_t* ndev, int64_t off, int32_t* stepc) {
+    if (threadIdx.x == 0 && batch_rows(ndev, off, n_max) > 0) *stepc += 1;
+}
+
 __global__ void k_zero_f32(float* p, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.f;
-}
-
-__device__ __forceinline__ void load_mixture(const float* o, Mixture& m) {
-    for (int k = 0; k < kLobes; k++) {
-        m.pi[k] = o[k];
-        m.kappa[k] = o[kLobes + k];
-        m.mu[k] = mk(o[2 * kLobes + 3 * k], o[2 * kLobes + 3 * k + 1], o[2 * kLobes + 3 * k + 2]);
-    }
 }
 
 __global__ __launch_bounds__(256) void k_guide_sample(int64_t n, const float* __restrict__ mix,
@@ -329,22 +353,19 @@ __global__ __launch_bounds__(256) void k_guide_sample(int64_t n, const float* __
                                                       float* __restrict__ pdf) {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
-    Mixture m;
-    load_mixture(mix + i * kOut, m);
-    const f3 d = mixture_sample(m, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    const float* o = mix + i * kOut;
+    const f3 d = mixture_sample(o, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
     dirs[3 * i] = d.x;
     dirs[3 * i + 1] = d.y;
     dirs[3 * i + 2] = d.z;
-    if (pdf) pdf[i] = mixture_pdf(m, d);
+    if (pdf) pdf[i] = mixture_pdf(o, d);
 }
 
 __global__ __launch_bounds__(256) void k_guide_pdf(int64_t n, const float* __restrict__ mix,
                                                    const float* __restrict__ dirs, float* __restrict__ pdf) {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
-    Mixture m;
-    load_mixture(mix + i * kOut, m);
-    pdf[i] = mixture_pdf(m, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    pdf[i] = mixture_pdf(mix + i * kOut, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
 
 inline unsigned nb(int64_t n, int b) { return unsigned((n + b - 1) / b); }
@@ -352,6 +373,8 @@ inline unsigned nb(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 }  // namespace
 
 int Guide::create(const float* lo, const float* hi, uint64_t seed) {
+    static std::atomic<uint64_t> next_uid{1};
+    uid = next_uid.fetch_add(1);
     for (int a = 0; a < 3; a++) {
         if (!(hi[a] > lo[a])) return fail(LMGPU_ERR_INVALID_ARGUMENT, "guide box must have positive extent");
         box_lo[a] = lo[a];
@@ -365,6 +388,14 @@ int Guide::create(const float* lo, const float* hi, uint64_t seed) {
     LMGPU_TRY(WnBf.alloc(3 * kHid * kHid));
     LMGPU_TRY(zero64.alloc(64));
     LMGPU_TRY(scal.alloc(4));
+    LMGPU_TRY(stepc.alloc(1));
+    // (layer, output tile, input tile) of every 32x32 weight-gradient tile
+    DwJob jobs[18];
+    int c = 0;
+    for (int l = 1; l <= 4; l++)
+        for (int nt = 0; nt < 2; nt++)
+            for (int kt = 0; kt < (l == 1 ? 3 : 2); kt++) jobs[c++] = DwJob{l, nt, kt};
+    LMGPU_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dw_jobs), jobs, sizeof(jobs)));
     LMGPU_HIP(hipMemset(zero64.p, 0, 64 * sizeof(float)));
     LMGPU_HIP(hipMemset(WnBf.p, 0, 3 * kHid * kHid * sizeof(uint16_t)));
     // torch.nn.Linear-style init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for W and b
@@ -404,10 +435,10 @@ int Guide::set_params(const float* host) {
     LMGPU_HIP(hipMemcpy(P.p, h.data(), sizeof(float) * kNParams, hipMemcpyHostToDevice));
     LMGPU_HIP(hipMemset(M.p, 0, sizeof(float) * kNParams));
     LMGPU_HIP(hipMemset(V.p, 0, sizeof(float) * kNParams));
+    LMGPU_HIP(hipMemset(stepc.p, 0, sizeof(int32_t)));
     hipLaunchKernelGGL(k_guide_sync, dim3(nb(kNParams, 256)), dim3(256), 0, nullptr, P.p, Wbf.p, WnBf.p);
     LMGPU_HIP(hipGetLastError());
     LMGPU_HIP(hipDeviceSynchronize());
-    step = 0;
     return LMGPU_OK;
 }
 
@@ -424,67 +455,62 @@ int Guide::infer(const float* query, int qs, float* mix, int64_t n, hipStream_t 
     if (n <= 0) return LMGPU_OK;
     Box b;
     for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
-    hipLaunchKernelGGL(k_guide_forward<false>, dim3(grid_for(n)), dim3(kGB), 0, s, n, int64_t(0), query, qs, Wbf.p,
-                       P.p, b, mix, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_guide_forward<false>, dim3(grid_for(n)), dim3(kGB), 0, s, n, nullptr, int64_t(0), int64_t(0),
+                       query, qs, Wbf.p, P.p, b, mix, nullptr, nullptr, nullptr);
     LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }
 
 int Guide::ensure_train(int64_t B) {
+    if (B % 128) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training capacity must be a multiple of 128");
     if (B <= train_cap) return LMGPU_OK;
     LMGPU_TRY(xT.alloc(size_t(96) * B));
     LMGPU_TRY(hT.alloc(size_t(3) * kHid * B));
     LMGPU_TRY(dT.alloc(size_t(4) * kHid * B));
     LMGPU_TRY(y.alloc(size_t(kOut) * B));
     LMGPU_TRY(w.alloc(B));
+    // Input rows 80..95 and output-delta rows 40..63 are read by the 32-wide
+    // weight-gradient tiles and never written: they stay zero.
+    LMGPU_HIP(hipMemset(xT.p, 0, sizeof(uint16_t) * 96 * B));
+    LMGPU_HIP(hipMemset(dT.p, 0, sizeof(uint16_t) * 4 * kHid * B));
+    LMGPU_HIP(hipMemset(hT.p, 0, sizeof(uint16_t) * 3 * kHid * B));
     train_cap = B;
+    gen++;
+    return LMGPU_OK;
+}
+
+int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* ndev, int64_t off, float lr,
+                         float* grad_host, hipStream_t s) {
+    if (B > train_cap || n > B) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training batch exceeds the workspace");
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, scal.p, int64_t(4));
+    hipLaunchKernelGGL(k_zero_f32, dim3(nb(kNParams, 256)), dim3(256), 0, s, G.p, int64_t(kNParams));
+    hipLaunchKernelGGL(k_guide_weights, dim3(nb(n, 256)), dim3(256), 0, s, n, ndev, off, rec, w.p, scal.p);
+    Box b;
+    for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
+    hipLaunchKernelGGL(k_guide_forward<true>, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, rec, kRecordFloats,
+                       Wbf.p, P.p, b, nullptr, xT.p, hT.p, y.p);
+    hipLaunchKernelGGL(k_guide_loss, dim3(nb(B, 256)), dim3(256), 0, s, n, ndev, off, B, y.p, rec, w.p, scal.p, wclamp,
+                       dT.p + 3 * kHid * B, scal.p + 1);
+    hipLaunchKernelGGL(k_guide_backward, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, WnBf.p, zero64.p, hT.p,
+                       dT.p);
+    const int slice = 4096;
+    hipLaunchKernelGGL(k_guide_dw, dim3(nb(B, slice), 18), dim3(64), 0, s, n, ndev, off, B, slice, xT.p, hT.p, dT.p,
+                       G.p);
+    hipLaunchKernelGGL(k_guide_db, dim3(4 * kHid), dim3(256), 0, s, n, ndev, off, B, dT.p, G.p);
+    if (grad_host) LMGPU_HIP(hipMemcpyAsync(grad_host, G.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_guide_adam, dim3(nb(kNParams, 256)), dim3(256), 0, s, n, ndev, off, stepc.p, P.p, M.p, V.p,
+                       G.p, lr, 0.9f, 0.999f, 1e-8f, Wbf.p, WnBf.p);
+    hipLaunchKernelGGL(k_guide_step, dim3(1), dim3(64), 0, s, n, ndev, off, stepc.p);
+    LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }
 
 int Guide::train(const float* rec, int64_t n, float lr, float* loss_out, float* grad_out, hipStream_t s) {
     if (n <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "empty training batch");
-    // batch padded to the dW slice granularity (16 samples per MFMA step)
+    // batch padded to whole 128-row tiles
     const int64_t B = (n + 127) / 128 * 128;
     LMGPU_TRY(ensure_train(B));
-    // zero padded features / deltas (xT rows 80..95 and everything past n)
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb(int64_t(96) * B / 2, 256)), dim3(256), 0, s, reinterpret_cast<float*>(xT.p),
-                       int64_t(96) * B / 2);
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb(int64_t(4) * kHid * B / 2, 256)), dim3(256), 0, s,
-                       reinterpret_cast<float*>(dT.p), int64_t(4) * kHid * B / 2);
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb(int64_t(3) * kHid * B / 2, 256)), dim3(256), 0, s,
-                       reinterpret_cast<float*>(hT.p), int64_t(3) * kHid * B / 2);
-    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, scal.p, int64_t(4));
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb(kNParams, 256)), dim3(256), 0, s, G.p, int64_t(kNParams));
-    hipLaunchKernelGGL(k_guide_weights, dim3(nb(n, 256)), dim3(256), 0, s, n, rec, w.p, scal.p);
-    Box b;
-    for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
-    hipLaunchKernelGGL(k_guide_forward<true>, dim3(grid_for(B)), dim3(kGB), 0, s, n, B, rec, kRecordFloats, Wbf.p, P.p,
-                       b, nullptr, xT.p, hT.p, y.p);
-    hipLaunchKernelGGL(k_guide_loss, dim3(nb(n, 256)), dim3(256), 0, s, n, B, y.p, rec, w.p, scal.p, wclamp, dT.p + 3 * kHid * B,
-                       scal.p + 1);
-    hipLaunchKernelGGL(k_guide_backward, dim3(grid_for(B)), dim3(kGB), 0, s, B, WnBf.p, zero64.p, hT.p, dT.p);
-    static bool jobs_set = false;
-    if (!jobs_set) {
-        DwJob jobs[18];
-        int c = 0;
-        for (int l = 1; l <= 4; l++)
-            for (int nt = 0; nt < 2; nt++)
-                for (int kt = 0; kt < (l == 1 ? 3 : 2); kt++) jobs[c++] = DwJob{l, nt, kt};
-        LMGPU_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dw_jobs), jobs, sizeof(jobs)));
-        jobs_set = true;
-    }
-    const int slice = 4096;
-    hipLaunchKernelGGL(k_guide_dw, dim3(nb(B, slice), 18), dim3(64), 0, s, B, slice, xT.p, hT.p, dT.p, G.p);
-    hipLaunchKernelGGL(k_guide_db, dim3(4 * kHid), dim3(256), 0, s, B, dT.p, G.p);
-    step++;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, float(step)), c2 = 1.f - std::pow(b2, float(step));
-    if (grad_out) {
-        LMGPU_HIP(hipMemcpyAsync(grad_out, G.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, s));
-    }
-    hipLaunchKernelGGL(k_guide_adam, dim3(nb(kNParams, 256)), dim3(256), 0, s, P.p, M.p, V.p, G.p, lr, b1, b2, 1e-8f,
-                       c1, c2, Wbf.p, WnBf.p);
-    LMGPU_HIP(hipGetLastError());
+    LMGPU_TRY(enqueue_train(rec, n, B, nullptr, 0, lr, grad_out, s));
     if (loss_out) {
         float l = 0.f;
         LMGPU_HIP(hipMemcpyAsync(&l, scal.p + 1, sizeof(float), hipMemcpyDeviceToHost, s));
@@ -501,10 +527,6 @@ int Guide::train(const float* rec, int64_t n, float lr, float* loss_out, float* 
 using lmgpu::fail;
 
 extern "C" {
-
-struct lmgpu_guide {
-    lmgpu::Guide g;
-};
 
 int lmgpu_guide_create(lmgpu_guide** out, const float* box_lo, const float* box_hi, uint64_t seed) {
     if (!out || !box_lo || !box_hi) return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -553,6 +575,17 @@ int lmgpu_guide_pdf(const float* mix, const float* dirs, float* pdf, int64_t n, 
     hipLaunchKernelGGL(lmgpu::k_guide_pdf, dim3(lmgpu::nb(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n,
                        mix, dirs, pdf);
     LMGPU_HIP(hipGetLastError());
+    return LMGPU_OK;
+}
+
+int lmgpu_guide_stats(const lmgpu_guide* guide, int64_t* steps, float* last_loss) {
+    if (!guide) return fail(LMGPU_ERR_INVALID_ARGUMENT, "guide is NULL");
+    if (steps) {
+        int32_t t = 0;
+        LMGPU_HIP(hipMemcpy(&t, guide->g.stepc.p, sizeof(t), hipMemcpyDeviceToHost));
+        *steps = t;
+    }
+    if (last_loss) LMGPU_HIP(hipMemcpy(last_loss, guide->g.scal.p + 1, sizeof(float), hipMemcpyDeviceToHost));
     return LMGPU_OK;
 }
 

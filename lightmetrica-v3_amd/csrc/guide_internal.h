@@ -18,13 +18,17 @@ constexpr int kRecordFloats = 16;
 struct Guide {
     DevBuf<float> P, M, V, G;       // parameters, Adam moments, gradients (packed layout)
     DevBuf<uint16_t> Wbf, WnBf;     // BF16 copies: packed transposed weights; layers 2..4 as W[k][n]
-    DevBuf<float> zero64, scal;     // zero bias; scalars: [0] weight sum, [1] loss
+    DevBuf<float> zero64, scal;     // zero bias; scalars: [0] weight sum, [1] loss of the last step
+    DevBuf<int32_t> stepc;          // Adam step count
     DevBuf<uint16_t> xT, hT, dT;    // training activations / deltas, feature major
     DevBuf<float> y, w;             // raw outputs, target weights
     int64_t train_cap = 0;
+    // identity of the device buffers, for callers that capture them in graphs:
+    // uid is unique per guide in the process, gen grows with every reallocation
+    uint64_t uid = 0;
+    int gen = 0;
     float box_lo[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
     float wclamp = 50.f;            // clamp of normalised target weights
-    int step = 0;
 
     int create(const float* lo, const float* hi, uint64_t seed);
     int set_params(const float* host);
@@ -33,6 +37,18 @@ struct Guide {
     // query: n rows of `qs` floats starting with pos(3), nrm(3), wo(3)
     int infer(const float* query, int qs, float* mix, int64_t n, hipStream_t s) const;
     int train(const float* rec, int64_t n, float lr, float* loss, float* grad, hipStream_t s);
+    // One Adam step on the first min(n, max(0, *ndev - off)) records of rec
+    // (all n when ndev is NULL); rec points at row `off` of a compacted
+    // buffer whose total count *ndev is in device memory. B >= n is the
+    // padded row capacity (multiple of 128, prepared by ensure_train).
+    // Enqueues kernels only (graph-capturable) unless grad_host is given.
+    int enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* ndev, int64_t off, float lr,
+                      float* grad_host, hipStream_t s);
 };
 
 }  // namespace lmgpu
+
+// Opaque handle of the C ABI.
+struct lmgpu_guide {
+    lmgpu::Guide g;
+};

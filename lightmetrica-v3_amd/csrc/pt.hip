@@ -33,6 +33,8 @@
 #include <vector>
 
 #include "bvh.h"
+#include "guide.h"
+#include "guide_internal.h"
 #include "lmgpu_internal.h"
 #include "shade.h"
 
@@ -95,6 +97,29 @@ struct PTConsts {
     int rr_depth;
     uint32_t seed_lo, seed_hi;
     int npix;
+};
+
+// Training record of one guided vertex of one path (slot v * npix + p for
+// the v-th surface vertex of path p).
+struct alignas(16) GRec {
+    float4 a;   // position, mixture pdf of the sampled direction (solid angle; 0 = empty slot)
+    float4 b;   // shading normal
+    float4 c;   // direction toward the previous vertex
+    float4 d;   // sampled direction
+    float4 e;   // path radiance when the direction was sampled, plus this vertex's NEE
+    float4 f;   // per channel: mixture pdf / throughput before the direction was sampled
+};
+
+struct GuideDev {
+    const float* mix;   // decoded mixture per extension-queue entry of the bounce [npix][40]
+    GRec* rec;          // [rec_depth][npix]
+    int32_t* sh_rec;    // record slot of each shadow-queue entry (-1: none)
+    float alpha;        // probability of sampling the guide at a non-specular vertex
+    int rec_depth;      // surface vertices per path that record (0: no recording)
+};
+
+struct GBox {
+    float lo[3], inv[3];
 };
 
 constexpr int kBlock = 256;
@@ -189,8 +214,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in
         atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
 }
 
+// Unoccluded shadow rays add their contribution to the path radiance and,
+// when the vertex records guide training data, to the record's radiance
+// baseline (the NEE of a vertex is not part of its sampled direction's
+// incident radiance).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q, unsigned long long* next, int refill) {
+__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q, unsigned long long* next,
+                                                   int refill, GRec* __restrict__ rec, const int32_t* __restrict__ sh_rec) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
@@ -210,6 +240,16 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Q
             L.y += c.y;
             L.z += c.z;
             ps.L[p] = L;
+            if (sh_rec) {
+                const int slot = sh_rec[id];
+                if (slot >= 0) {
+                    float4 e = rec[slot].e;
+                    e.x += c.x;
+                    e.y += c.y;
+                    e.z += c.z;
+                    rec[slot].e = e;
+                }
+            }
         }
     };
     trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
@@ -244,7 +284,12 @@ __device__ __forceinline__ float pdf_light_env(const SceneDev& sc, f3 np, bool p
     return J == 0.f ? 0.f : psel * pSA / J;
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Queues q, PTConsts c, int in_q) {
+// Guided vertices (GUIDED, diffuse or glossy) sample the next direction from
+// the one-sample MIS mixture alpha * guide + (1 - alpha) * BSDF, whose
+// density (solid angle) p_mix enters both the throughput f |cos| / p_mix and,
+// as p_mix / |cos|, every MIS weight against light sampling.
+template <bool GUIDED>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Queues q, PTConsts c, int in_q, GuideDev gd) {
     const int n = q.counts[in_q];
     const int out_q = in_q ^ 1;
     const uint32_t sample = uint32_t(q.counts[3]);
@@ -259,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
         bool emit_ext = false, emit_sh = false;
         lmgpu_ray ext_ray, sh_ray;
         float4 sh_c = make_float4(0, 0, 0, 0);
-        int p = 0;
+        int p = 0, sh_slot = -1;
         if (active) {
             p = q.ext_path[in_q][i];
             const float4* rp = reinterpret_cast<const float4*>(q.ext[in_q] + i);
@@ -333,6 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                 if (alive && nv2 < c.max_verts) {
                     const bool spec = is_specular(m);
                     const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
+                    // guide mixture of this queue entry (guided vertices only)
+                    const float* gmix = (GUIDED && !spec) ? gd.mix + int64_t(i) * kOut : nullptr;
+                    const int slot = (gmix && nv - 1 < gd.rec_depth) ? (nv - 1) * c.npix + p : -1;
                     if (nee) {
                         // path::sample_direct (path.h:509-528)
                         const rand4 r1 = rng(uint32_t(p), sample, uint32_t(nv), 1u, seed);
@@ -393,10 +441,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                             const f3 fs = mat_eval(m, g, wi, dirL) * sn_correction(g, wi, dirL);
                             if (!is_zero(fs)) {
                                 float mis = 1.f;
-                                if (c.mode == LMGPU_SAMPLING_MIS) mis = balance_heuristic(p_light, mat_pdf(m, g, wi, dirL));
+                                if (c.mode == LMGPU_SAMPLING_MIS) {
+                                    float pdir = mat_pdf(m, g, wi, dirL);
+                                    if (GUIDED && gmix) {
+                                        float pg = mixture_pdf(gmix, dirL);
+                                        if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
+                                        pdir = gd.alpha * pg / fabsf(dot(dirL, g.n)) + (1.f - gd.alpha) * pdir;
+                                    }
+                                    mis = balance_heuristic(p_light, pdir);
+                                }
                                 const f3 C = thr * fs * weight * mis;
                                 if (!is_zero(C)) {
                                     emit_sh = true;
+                                    sh_slot = slot;
                                     sh_ray.org[0] = pos.x; sh_ray.org[1] = pos.y; sh_ray.org[2] = pos.z;
                                     sh_ray.tmin = kEps;
                                     sh_ray.dir[0] = dirL.x; sh_ray.dir[1] = dirL.y; sh_ray.dir[2] = dirL.z;
@@ -408,8 +465,34 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                     }
                     // path::sample_direction (path.h:410-419)
                     f3 wo2, sw;
-                    if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw)) {
+                    bool sampled;
+                    float pdir = 0.f, pmix = 0.f;
+                    if (GUIDED && gmix) {
+                        const rand4 r2 = rng(uint32_t(p), sample, uint32_t(nv), 2u, seed);
+                        if (r2.u[0] < gd.alpha) {
+                            wo2 = mixture_sample(gmix, r2.u[1], r2.u[2], r2.u[3]);
+                            sampled = true;
+                        } else {
+                            sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
+                        }
+                        sw = mk(0, 0, 0);
+                        if (sampled) {
+                            const float cs = fabsf(dot(wo2, g.n));
+                            float pg = mixture_pdf(gmix, wo2);
+                            if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
+                            pmix = gd.alpha * pg + (1.f - gd.alpha) * mat_pdf(m, g, wi, wo2) * cs;
+                            if (pmix > 0.f && cs > 0.f) {
+                                sw = mat_eval(m, g, wi, wo2) * (cs / pmix);
+                                pdir = pmix / cs;
+                            }
+                        }
+                    } else {
+                        sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
+                        if (sampled) pdir = mat_pdf(m, g, wi, wo2);
+                    }
+                    if (sampled) {
                         sw = sw * sn_correction(g, wi, wo2);
+                        const f3 thr_before = thr;
                         thr = thr * sw;
                         if (!is_zero(thr)) {
                             emit_ext = true;
@@ -417,10 +500,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                             ext_ray.tmin = kEps;
                             ext_ray.dir[0] = wo2.x; ext_ray.dir[1] = wo2.y; ext_ray.dir[2] = wo2.z;
                             ext_ray.tmax = kInf;
-                            ps.prev[p] = make_float4(g.n.x, g.n.y, g.n.z, mat_pdf(m, g, wi, wo2));
+                            ps.prev[p] = make_float4(g.n.x, g.n.y, g.n.z, pdir);
                             info = make_int4(nv2, comp, nee ? 1 : 0, 0);
                             ps.info[p] = info;
                             ps.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.f);
+                            if (GUIDED && slot >= 0) {
+                                // incident radiance along wo2 is (L_final - e) / throughput after this
+                                // vertex; the record keeps p_mix / throughput before it, so that
+                                // (L_final - e) * f is the integrand f |cos| L_i estimate
+                                GRec r;
+                                r.a = make_float4(pos.x, pos.y, pos.z, pmix);
+                                r.b = make_float4(g.n.x, g.n.y, g.n.z, 0.f);
+                                r.c = make_float4(wi.x, wi.y, wi.z, 0.f);
+                                r.d = make_float4(wo2.x, wo2.y, wo2.z, 0.f);
+                                r.e = make_float4(L.x, L.y, L.z, 0.f);
+                                r.f = make_float4(thr_before.x > 0.f ? pmix / thr_before.x : 0.f,
+                                                  thr_before.y > 0.f ? pmix / thr_before.y : 0.f,
+                                                  thr_before.z > 0.f ? pmix / thr_before.z : 0.f, 0.f);
+                                gd.rec[slot] = r;
+                            }
                         }
                     }
                 }
@@ -436,8 +534,118 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
         if (emit_sh) {
             q.sh[ss] = sh_ray;
             q.sh_contrib[ss] = sh_c;
+            if (GUIDED) gd.sh_rec[ss] = sh_slot;
         }
     }
+}
+
+// Guide inference over the extension queue of a bounce: the query of entry
+// i is its hit point, shading normal and the direction back along the ray;
+// the decoded mixture goes to mix[i]. Entries without a hit get a zero query.
+__global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, int in_q, const uint16_t* __restrict__ Wbf,
+                                                        const float* __restrict__ P, GBox box, float* __restrict__ mix) {
+    constexpr int kWaveLds = 32 * kPitchX * 2 + 32 * kPitchH * 2;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
+    uint16_t* lh = lx + 32 * kPitchX;
+    float* ly = reinterpret_cast<float*>(lx);
+    const int n = q.counts[in_q];
+    for (int base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
+        const int i0 = base + wave * 32;
+        if (lane < 32) {
+            const int i = i0 + lane;
+            uint16_t* row = lx + lane * kPitchX;
+            const float4 h = i < n ? reinterpret_cast<const float4*>(q.hit)[i] : make_float4(__int_as_float(-1), 0, 0, 0);
+            const int tri = __float_as_int(h.x);
+            if (tri >= 0) {
+                const ShadeTri st = sc.stri[tri];
+                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
+                const f3 pos = mk(st.p0.x, st.p0.y, st.p0.z) * b0 + mk(st.p1.x, st.p1.y, st.p1.z) * b1 +
+                               mk(st.p2.x, st.p2.y, st.p2.z) * b2;
+                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+                const lmgpu_ray& r = q.ext[in_q][i];
+                encode_input(pos, nsh, mk(-r.dir[0], -r.dir[1], -r.dir[2]), box.lo, box.inv, row);
+            } else {
+                for (int k = 0; k < kIn; k++) row[k] = 0;
+            }
+        }
+        __syncthreads();
+        mlp_tile_forward(Wbf, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
+        __syncthreads();
+        if (lane < 32) {
+            const int i = i0 + lane;
+            if (i < n) {
+                Mixture m;
+                decode_mixture(ly + lane * kPitchY, m);
+                float* o = mix + int64_t(i) * kOut;
+                for (int k = 0; k < kLobes; k++) {
+                    o[k] = m.pi[k];
+                    o[kLobes + k] = m.kappa[k];
+                    o[2 * kLobes + 3 * k] = m.mu[k].x;
+                    o[2 * kLobes + 3 * k + 1] = m.mu[k].y;
+                    o[2 * kLobes + 3 * k + 2] = m.mu[k].z;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// After a pass: turns the filled record slots into training records
+// (pos, nrm, wo, wi, p_mix, f |cos| L_i estimate) appended to train[cap][16]
+// when their target weight is positive, and empties the slots. Slots are
+// visited in the order j * mul mod total (mul coprime to total) so that a
+// capped batch spreads over the film and the path depths.
+__global__ __launch_bounds__(kBlock) void k_guide_collect(PathState ps, GuideDev gd, int npix, int64_t total,
+                                                          int64_t mul, float* __restrict__ train, int cap,
+                                                          int32_t* __restrict__ count,
+                                                          unsigned long long* __restrict__ recorded) {
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    const int64_t n_iter = (total + stride - 1) / stride;
+    for (int64_t it = 0; it < n_iter; it++) {
+        const int64_t j = it * stride + int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        bool keep = false;
+        float out[16];
+        if (j < total) {
+            const int64_t slot = (j * mul) % total;
+            GRec& r = gd.rec[slot];
+            const float4 a = r.a;
+            if (a.w > 0.f) {
+                const int p = int(slot % npix);
+                const float4 Lf = ps.L[p];
+                const float4 e = r.e, f = r.f;
+                const float lx = fmaxf(0.f, Lf.x - e.x) * f.x, ly = fmaxf(0.f, Lf.y - e.y) * f.y,
+                            lz = fmaxf(0.f, Lf.z - e.z) * f.z;
+                const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / a.w;
+                r.a.w = 0.f;
+                if (wgt > 0.f && wgt < 3e38f) {
+                    const float4 b = r.b, c = r.c, d = r.d;
+                    keep = true;
+                    out[0] = a.x; out[1] = a.y; out[2] = a.z;
+                    out[3] = b.x; out[4] = b.y; out[5] = b.z;
+                    out[6] = c.x; out[7] = c.y; out[8] = c.z;
+                    out[9] = d.x; out[10] = d.y; out[11] = d.z;
+                    out[12] = a.w; out[13] = lx; out[14] = ly; out[15] = lz;
+                }
+            }
+        }
+        const unsigned long long mask = __ballot(keep);
+        if (mask && __lane_id() == __ffsll(mask) - 1) atomicAdd(recorded, (unsigned long long)__popcll(mask));
+        const int k = wave_append(count, keep);
+        if (keep && k < cap) {
+            float4* o = reinterpret_cast<float4*>(train + int64_t(k) * 16);
+            o[0] = make_float4(out[0], out[1], out[2], out[3]);
+            o[1] = make_float4(out[4], out[5], out[6], out[7]);
+            o[2] = make_float4(out[8], out[9], out[10], out[11]);
+            o[3] = make_float4(out[12], out[13], out[14], out[15]);
+        }
+    }
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_accumulate(PathState ps, double* __restrict__ sum,
@@ -475,6 +683,16 @@ __global__ void k_zero_f64(double* p, int64_t n) {
 // host
 // ---------------------------------------------------------------------------
 
+// Guided rendering of one render call (opts resolved, workspace sized).
+struct GuidedPass {
+    Guide* g = nullptr;
+    uint64_t uid = 0;   // guide identity and buffer generation (graph key)
+    int gen = 0;
+    float alpha = 0.5f, lr = 1e-3f;
+    int batch = 0, steps = 0, rec_depth = 0;
+    int64_t mul = 1;   // slot visiting order of k_guide_collect
+};
+
 struct Scene {
     Accel accel;
     DevBuf<ShadeTri> stri;
@@ -493,9 +711,15 @@ struct Scene {
     DevBuf<int32_t> path0, path1, counts;
     DevBuf<unsigned long long> next;   // per-bounce ray counters of the persistent trace kernels
     DevBuf<double> fsum, fsq;   // device film when the caller passes host pointers
+    // guided rendering workspace
+    DevBuf<float> gmix, gtrain;
+    DevBuf<GRec> grec;
+    DevBuf<int32_t> gshrec, gcount;
+    int grec_slots = 0;
     hipGraphExec_t graph = nullptr;
     int refill_primary = 64, refill_secondary = 16;
     int graph_key[4] = {0, 0, 0, 0};
+    GuidedPass graph_guide;
     uint64_t graph_seed = 0;
     int graph_sumsq = -1;
     double* graph_sum_ptr = nullptr;
@@ -507,9 +731,10 @@ struct Scene {
 
     int create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts);
     int ensure_workspace(int npix);
-    int enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream_t s);
-    int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, lmgpu_pt_stats* st,
-               hipStream_t s);
+    int ensure_guided(int npix, const GuidedPass& gp);
+    int enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const GuidedPass* gp, hipStream_t s);
+    int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
+               lmgpu_pt_stats* st, hipStream_t s);
 };
 
 static inline float f_or(const float* p, int i) { return p ? p[i] : 0.f; }
@@ -666,6 +891,37 @@ int Scene::ensure_workspace(int npix) {
     return LMGPU_OK;
 }
 
+int Scene::ensure_guided(int npix, const GuidedPass& gp) {
+    // any reallocation here invalidates the captured pass
+    auto drop_graph = [&]() {
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+    };
+    if (gmix.n != size_t(npix) * kOut) {
+        drop_graph();
+        LMGPU_TRY(gmix.alloc(size_t(npix) * kOut));
+        LMGPU_TRY(gshrec.alloc(npix));
+    }
+    const int64_t slots = int64_t(gp.rec_depth) * npix;
+    if (slots > grec_slots) {
+        drop_graph();
+        LMGPU_TRY(grec.alloc(slots));
+        // every slot starts empty (mixture pdf 0)
+        LMGPU_HIP(hipMemset(grec.p, 0, sizeof(GRec) * slots));
+        grec_slots = int(slots);
+    }
+    const size_t tsz = size_t(gp.batch) * std::max(gp.steps, 1) * kRecordFloats;
+    if (gp.steps > 0 && gtrain.n < tsz) {
+        drop_graph();
+        LMGPU_TRY(gtrain.alloc(tsz));
+    }
+    if (!gcount.p) {
+        drop_graph();
+        LMGPU_TRY(gcount.alloc(1));
+    }
+    if (gp.steps > 0) LMGPU_TRY(gp.g->ensure_train(gp.batch));
+    return LMGPU_OK;
+}
+
 template <int W>
 static int pt_trace_grid() {
     static int grid = 0;
@@ -685,9 +941,24 @@ static int shade_grid(int n) {
     return std::max(1, std::min((n + kBlock - 1) / kBlock, 256 * 8));
 }
 
-int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream_t s) {
+int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const GuidedPass* gp, hipStream_t s) {
     PathState ps{thr.p, L.p, prev.p, info.p};
     Queues q{{ext0.p, ext1.p}, hit.p, {path0.p, path1.p}, sh.p, shc.p, counts.p};
+    GuideDev gd{};
+    GBox box{};
+    if (gp) {
+        gd.mix = gmix.p;
+        gd.rec = grec.p;
+        gd.sh_rec = gshrec.p;
+        gd.alpha = gp->alpha;
+        gd.rec_depth = gp->rec_depth;
+        for (int a = 0; a < 3; a++) {
+            box.lo[a] = gp->g->box_lo[a];
+            box.inv[a] = gp->g->box_inv[a];
+        }
+    }
+    GRec* crec = (gp && gp->rec_depth > 0) ? grec.p : nullptr;
+    const int32_t* cshrec = crec ? gshrec.p : nullptr;
     const int npix = c.npix;
     const unsigned gb = unsigned((npix + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_generate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
@@ -701,32 +972,59 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, hipStream
         if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
         else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
         else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
+        if (gp)
+            hipLaunchKernelGGL(k_pt_guide_infer, dim3(std::max(1, std::min((npix + 127) / 128, 2048))), dim3(256), 0, s,
+                               dev, q, in_q, gp->g->Wbf.p, gp->g->P.p, box, gmix.p);
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_shade, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q);
+        if (gp) hipLaunchKernelGGL(k_shade<true>, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q, gd);
+        else hipLaunchKernelGGL(k_shade<false>, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q, gd);
         if (c.mode != LMGPU_SAMPLING_NAIVE) {
             unsigned long long* nx = next.p + 1;
-            if (accel.width == 4) hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
-            else if (accel.width == 8) hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
-            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm);
+            if (accel.width == 4)
+                hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
+            else if (accel.width == 8)
+                hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
+            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
         }
     }
     hipLaunchKernelGGL(k_accumulate, dim3(gb), dim3(kBlock), 0, s, ps, sum, sumsq, npix);
+    if (gp && gp->steps > 0) {
+        // online training on this pass's records
+        const int64_t total = int64_t(gp->rec_depth) * npix;
+        const int cap = gp->batch * gp->steps;
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, gcount.p, 1);
+        hipLaunchKernelGGL(k_guide_collect, dim3(unsigned(std::min<int64_t>((total + kBlock - 1) / kBlock, 4096))),
+                           dim3(kBlock), 0, s, ps, gd, npix, total, gp->mul, gtrain.p, cap, gcount.p,
+                           reinterpret_cast<unsigned long long*>(counts.p + 14));
+        for (int t = 0; t < gp->steps; t++)
+            LMGPU_TRY(gp->g->enqueue_train(gtrain.p + int64_t(t) * gp->batch * kRecordFloats, gp->batch, gp->batch,
+                                           gcount.p, int64_t(t) * gp->batch, gp->lr, nullptr, s));
+    }
     hipLaunchKernelGGL(k_next_sample, dim3(1), dim3(64), 0, s, counts.p);
     LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }
 
-int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bool on_device, lmgpu_pt_stats* st,
-                  hipStream_t s) {
+int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bool on_device, const GuidedPass* gp,
+                  lmgpu_pt_stats* st, hipStream_t s) {
     if (!p) return fail(LMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
     if (!sum_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film_sum is NULL");
     if (p->max_verts < 2 || p->max_verts > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "max_verts must be in [2, 64]");
     if (p->sampling_mode < 0 || p->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
     if (p->render_time <= 0 && p->spp <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "spp must be positive");
-    if (p->guiding) return fail(LMGPU_ERR_UNSUPPORTED, "guiding is enabled through lmgpu_guide_* (not in this entry point)");
+    if (p->guiding) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering goes through lmgpu_pt_render_guided");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
     LMGPU_TRY(ensure_workspace(npix));
+    GuidedPass gpv;
+    if (gp) {
+        LMGPU_TRY(ensure_guided(npix, *gp));
+        // buffers as they are after ensure_guided (it may grow the guide's training workspace)
+        gpv = *gp;
+        gpv.uid = gp->g->uid;
+        gpv.gen = gp->g->gen;
+        gp = &gpv;
+    }
     double* sum = sum_in;
     double* sumsq = sumsq_in;
     if (!on_device) {
@@ -747,19 +1045,26 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     c.seed_lo = uint32_t(p->seed);
     c.seed_hi = uint32_t(p->seed >> 32);
     c.npix = npix;
-    // stats counters [8..15] as 64-bit: [8] samples, [10] extension rays, [12] shadow rays
+    // stats counters [8..15] as 64-bit: [8] samples, [10] extension rays, [12] shadow rays,
+    // [14] guide training records
     LMGPU_HIP(hipMemsetAsync(counts.p + 8, 0, 8 * sizeof(int32_t), s));
     hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(64), 0, s, counts.p, p->sample_offset);
     // one pass captured as a graph, replayed per sample
+    const GuidedPass none;
+    const GuidedPass& gk = gp ? *gp : none;
+    const bool guide_same = graph_guide.g == gk.g && graph_guide.uid == gk.uid && graph_guide.gen == gk.gen &&
+                            graph_guide.alpha == gk.alpha && graph_guide.lr == gk.lr &&
+                            graph_guide.batch == gk.batch && graph_guide.steps == gk.steps &&
+                            graph_guide.rec_depth == gk.rec_depth && graph_guide.mul == gk.mul;
     const bool key_same = graph && graph_key[0] == c.max_verts && graph_key[1] == c.mode && graph_key[2] == c.rr_depth &&
-                          graph_seed == p->seed && graph_sum_ptr == sum && graph_sq_ptr == sumsq;
+                          graph_seed == p->seed && graph_sum_ptr == sum && graph_sq_ptr == sumsq && guide_same;
     if (!key_same) {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
         hipStream_t cs;
         LMGPU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t g;
         LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_pass(c, sum, sumsq, cs);
+        const int rc = enqueue_pass(c, sum, sumsq, gp, cs);
         hipError_t e = hipStreamEndCapture(cs, &g);
         if (rc != LMGPU_OK) { (void)hipStreamDestroy(cs); return rc; }
         if (e != hipSuccess) { (void)hipStreamDestroy(cs); return fail(LMGPU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e)); }
@@ -769,6 +1074,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
         if (e != hipSuccess) { graph = nullptr; return fail(LMGPU_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
         graph_key[0] = c.max_verts; graph_key[1] = c.mode; graph_key[2] = c.rr_depth;
         graph_seed = p->seed;
+        graph_guide = gk;
         graph_sum_ptr = sum;
         graph_sq_ptr = sumsq;
     }
@@ -798,12 +1104,13 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during rendering");
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        int64_t v[3];
+        int64_t v[4];
         std::memcpy(v, cnt + 8, sizeof(v));
         st->samples = v[0];
         st->spp = passes;
         st->extension_rays = v[1];
         st->shadow_rays = v[2];
+        st->guide_records = gp ? v[3] : 0;
         st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return LMGPU_OK;
@@ -850,7 +1157,58 @@ int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* f
                     int film_on_device, lmgpu_pt_stats* stats, void* stream) {
     if (!scene) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene is NULL");
     lmgpu::clear_error();
-    return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, stats, static_cast<hipStream_t>(stream));
+    return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, nullptr, stats,
+                           static_cast<hipStream_t>(stream));
+}
+
+static int64_t gcd64(int64_t a, int64_t b) {
+    while (b) {
+        const int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lmgpu_guide* guide,
+                           const lmgpu_guide_opts* opts, double* film_sum, double* film_sumsq, int film_on_device,
+                           lmgpu_pt_stats* stats, void* stream) {
+    if (!scene || !guide) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene and guide are required");
+    lmgpu::clear_error();
+    lmgpu_guide_opts o{};
+    o.alpha = 0.5f;
+    o.learning_rate = 1e-3f;
+    o.batch = 1 << 18;
+    o.steps_per_pass = 1;
+    o.record_depth = 8;
+    if (opts) o = *opts;
+    if (!(o.alpha >= 0.f && o.alpha <= 1.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "alpha must be in [0, 1]");
+    if (!(o.learning_rate >= 0.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "learning rate must be >= 0");
+    if (o.steps_per_pass < 0 || o.steps_per_pass > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "steps_per_pass must be in [0, 64]");
+    if (o.steps_per_pass > 0 && (o.batch < 1 || o.batch > (1 << 22)))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch must be in [1, 2^22]");
+    if (o.record_depth < 0 || o.record_depth > 16) return fail(LMGPU_ERR_INVALID_ARGUMENT, "record_depth must be in [0, 16]");
+    lmgpu::GuidedPass gp;
+    gp.g = &guide->g;
+    gp.alpha = o.alpha;
+    gp.lr = o.learning_rate;
+    gp.steps = o.steps_per_pass;
+    gp.batch = o.steps_per_pass > 0 ? (o.batch + 127) / 128 * 128 : 0;
+    gp.rec_depth = o.steps_per_pass > 0 ? o.record_depth : 0;
+    if (gp.rec_depth == 0) gp.steps = 0;
+    const int64_t npix = int64_t(scene->s.w) * scene->s.h;
+    const int64_t total = int64_t(gp.rec_depth) * npix;
+    if (total >= (int64_t(1) << 31)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "record_depth * pixels must be < 2^31");
+    if (int64_t(gp.batch) * std::max(gp.steps, 1) >= (int64_t(1) << 31))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch * steps_per_pass must be < 2^31");
+    if (total > 1) {
+        // a multiplier coprime to the slot count: a bijection of the slots
+        int64_t m = (total * 5) / 8 + 1;
+        while (gcd64(m, total) != 1) m++;
+        gp.mul = m % total;
+    }
+    return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, &gp, stats,
+                           static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -38,7 +38,7 @@ EXPORTED = [
     'lmgpu_accel_intersect_counted',
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
-    'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train',
+    'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
 ]
 
 
@@ -103,8 +103,14 @@ class PTParams(ctypes.Structure):
 class PTStats(ctypes.Structure):
     _fields_ = [
         ('samples', ctypes.c_int64), ('spp', ctypes.c_int64), ('extension_rays', ctypes.c_int64),
-        ('shadow_rays', ctypes.c_int64), ('elapsed_s', ctypes.c_double), ('reserved', ctypes.c_int64 * 4),
+        ('shadow_rays', ctypes.c_int64), ('elapsed_s', ctypes.c_double), ('guide_records', ctypes.c_int64),
+        ('reserved', ctypes.c_int64 * 3),
     ]
+
+
+class GuideOpts(ctypes.Structure):
+    _fields_ = [('alpha', ctypes.c_float), ('learning_rate', ctypes.c_float), ('batch', ctypes.c_int32),
+                ('steps_per_pass', ctypes.c_int32), ('record_depth', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
 
 
 _lib = None
@@ -152,6 +158,9 @@ def lib():
         L.lmgpu_guide_infer.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_guide_sample.argtypes = [vp, vp, vp, vp, i64, vp]
         L.lmgpu_guide_pdf.argtypes = [vp, vp, vp, i64, vp]
+        L.lmgpu_guide_stats.argtypes = [vp, ctypes.POINTER(i64), f32p]
+        L.lmgpu_pt_render_guided.argtypes = [vp, ctypes.POINTER(PTParams), vp, ctypes.POINTER(GuideOpts), vp, vp,
+                                             ctypes.c_int, ctypes.POINTER(PTStats), vp]
         L.lmgpu_guide_train.argtypes = [vp, vp, i64, ctypes.c_float, f32p, vp, vp]
         for name in EXPORTED:
             if name not in ('lmgpu_last_error', 'lmgpu_version'):

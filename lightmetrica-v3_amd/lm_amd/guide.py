@@ -73,6 +73,13 @@ class Guide:
                   n, ctypes.c_void_p(_stream_ptr(stream)))
         return mix
 
+    def stats(self):
+        """(Adam steps taken, loss of the last step)."""
+        steps = ctypes.c_int64(0)
+        loss = ctypes.c_float(0.0)
+        _lib.call('lmgpu_guide_stats', self._h, ctypes.byref(steps), ctypes.byref(loss))
+        return steps.value, loss.value
+
     def train(self, records, lr: float = 1e-3, want_grad: bool = False, stream=None):
         """records: [n,16] float32 CUDA tensor. Returns (loss, grad or None)."""
         loss = ctypes.c_float(0.0)

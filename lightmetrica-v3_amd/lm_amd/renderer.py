@@ -113,6 +113,8 @@ class GpuScene:
         _lib.call('lmgpu_scene_create', ctypes.byref(h), ctypes.byref(d), ctypes.byref(opts))
         self._h = h
         self.w, self.h = int(flat.w), int(flat.h)
+        v = np.asarray(flat.verts, dtype=np.float32).reshape(-1, 3)
+        self._lo, self._hi = v.min(axis=0), v.max(axis=0)
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -131,12 +133,18 @@ class GpuScene:
         _lib.call('lmgpu_accel_info_get', a, ctypes.byref(i))
         return {f: getattr(i, f) for f, _ in _lib.AccelInfo._fields_}
 
+    def bounds(self):
+        """Scene bounding box (lo, hi) of the triangle vertices."""
+        return self._lo.copy(), self._hi.copy()
+
     def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: int = 5,
                    seed: int = 0, sample_offset: int = 0, render_time: float = 0.0,
-                   film_sum=None, film_sumsq=None, stream=None):
-        """lmgpu_pt_render. With torch CUDA tensors for film_sum / film_sumsq
-        ([h,w,3] float64) the film stays on the device and is accumulated
-        into; otherwise host arrays are returned (fresh sums)."""
+                   film_sum=None, film_sumsq=None, stream=None, guide=None, guide_opts: Optional[dict] = None):
+        """lmgpu_pt_render (lmgpu_pt_render_guided with a `guide`). With
+        torch CUDA tensors for film_sum / film_sumsq ([h,w,3] float64) the
+        film stays on the device and is accumulated into; otherwise host
+        arrays are returned (fresh sums). guide_opts keys: alpha,
+        learning_rate, batch, steps_per_pass, record_depth."""
         p = _lib.PTParams()
         p.spp = int(spp)
         p.max_verts = int(max_verts)
@@ -147,19 +155,28 @@ class GpuScene:
         p.guiding = 0
         p.render_time = float(render_time)
         st = _lib.PTStats()
+        if guide is not None:
+            o = _lib.GuideOpts(0.5, 1e-3, 1 << 18, 1, 8)
+            for k, v in (guide_opts or {}).items():
+                if k not in ('alpha', 'learning_rate', 'batch', 'steps_per_pass', 'record_depth'):
+                    raise ValueError(f'unknown guide option {k!r}')
+                setattr(o, k, v)
+            fn, pre = 'lmgpu_pt_render_guided', (self._h, ctypes.byref(p), guide.handle, ctypes.byref(o))
+        else:
+            fn, pre = 'lmgpu_pt_render', (self._h, ctypes.byref(p))
         on_device = film_sum is not None and hasattr(film_sum, 'data_ptr')
         if on_device:
             sq = ctypes.c_void_p(film_sumsq.data_ptr()) if film_sumsq is not None else None
-            _lib.call('lmgpu_pt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film_sum.data_ptr()), sq, 1,
-                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+            _lib.call(fn, *pre, ctypes.c_void_p(film_sum.data_ptr()), sq, 1, ctypes.byref(st),
+                      ctypes.c_void_p(_stream_ptr(stream)))
             s, q = film_sum, film_sumsq
         else:
             s = np.zeros((self.h, self.w, 3), dtype=np.float64)
             q = np.zeros((self.h, self.w, 3), dtype=np.float64)
-            _lib.call('lmgpu_pt_render', self._h, ctypes.byref(p), s.ctypes.data_as(ctypes.c_void_p),
-                      q.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+            _lib.call(fn, *pre, s.ctypes.data_as(ctypes.c_void_p), q.ctypes.data_as(ctypes.c_void_p), 0,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
         stats = {'samples': st.samples, 'spp': st.spp, 'extension_rays': st.extension_rays,
-                 'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s}
+                 'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s, 'guide_records': st.guide_records}
         return s, q, stats
 
 

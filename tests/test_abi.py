@@ -40,6 +40,8 @@ def test_host_side_errors_without_gpu():
     assert L.lmgpu_accel_build(None, None, 0, 0, None, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.lmgpu_scene_create(None, None, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.lmgpu_pt_render(None, None, None, None, 0, None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.lmgpu_pt_render_guided(None, None, None, None, None, None, 0, None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.lmgpu_guide_stats(None, None, None) == _lib.ERR_INVALID_ARGUMENT
     with pytest.raises(_lib.LmError):
         _lib.call('lmgpu_accel_info_get', None, None)
 
@@ -48,3 +50,5 @@ def test_structure_sizes_match_header():
     # lmgpu_ray 32 B, lmgpu_hit 16 B as the kernels assume
     assert ctypes.sizeof(_lib.BuildOpts) == 32
     assert ctypes.sizeof(_lib.PTParams) == 4 * 4 + 8 + 4 + 4 + 8 + 32
+    assert ctypes.sizeof(_lib.PTStats) == 4 * 8 + 8 + 4 * 8
+    assert ctypes.sizeof(_lib.GuideOpts) == 4 * 12
