@@ -211,6 +211,24 @@ typedef struct lmgpu_pt_stats {
 int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* film_sum, double* film_sumsq,
                     int film_on_device, lmgpu_pt_stats* stats, void* stream);
 
+/* renderer::raycast (src/renderer/renderer_raycast.cpp:48-97): one primary
+ * ray through every pixel centre; film[h][w][3] (FP64, y = 0 the bottom row)
+ * is set to bg_color on a miss (the envconst radiance when the scene has
+ * one), |shading normal| with visualize_normal, else the reflectance (Kd of
+ * diffuse, Ks of glossy, 0 for mirror and glass; `color` when use_color) times
+ * 0.2 + 0.8 |n . -d| unless use_constant_color. Synchronous for host films. */
+typedef struct lmgpu_raycast_params {
+    float bg_color[3];
+    float color[3];
+    int32_t use_color;
+    int32_t use_constant_color;
+    int32_t visualize_normal;
+    int32_t reserved[5];
+} lmgpu_raycast_params;
+
+int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params, double* film, int film_on_device,
+                         void* stream);
+
 /* ---- learned path guiding --------------------------------------------------
  * No counterpart in the reference: an MLP maps (position, shading normal,
  * outgoing direction) to a mixture of 8 von Mises-Fisher lobes over incident

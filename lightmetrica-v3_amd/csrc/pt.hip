@@ -644,6 +644,64 @@ __global__ __launch_bounds__(kBlock) void k_guide_collect(PathState ps, GuideDev
     }
 }
 
+// renderer::raycast (src/renderer/renderer_raycast.cpp:62-97): one primary
+// ray through each pixel centre, generated in the traversal's fetch; the
+// emit shades it: background on a miss (the env radiance when the scene has
+// an envconst light), |shading normal|, or reflectance times
+// 0.2 + 0.8 |n . -d| (Kd / Ks; 0 for mirror and glass), written with
+// set_pixel semantics into film[h][w][3].
+struct RaycastConsts {
+    f3 bg, color;
+    int use_color, constant, normal;
+};
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_raycast(SceneDev sc, unsigned long long* next, int refill, RaycastConsts rc,
+                                                    double* __restrict__ film, int32_t* status) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = sc.w * sc.h;
+    auto dir_of = [&](int64_t id) {
+        const int x = int(id % sc.w), y = int(id / sc.w);
+        return primary_dir(sc.cam, (x + .5f) / sc.w, (y + .5f) / sc.h);
+    };
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+        o = sc.cam.pos;
+        d = dir_of(id);
+        tmin = kEps;
+        tmax = kInf;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
+        f3 C = rc.bg;
+        if (r.prim >= 0) {
+            const ShadeTri st = sc.stri[r.prim];
+            const float b1 = r.u, b2 = r.v, b0 = 1.f - b1 - b2;
+            const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                     mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+            if (rc.normal) {
+                C = mk(fabsf(nsh.x), fabsf(nsh.y), fabsf(nsh.z));
+            } else {
+                int light;
+                const Mat m = load_mat(sc.mats, __float_as_int(st.p0.w), light);
+                const bool refl = m.type == LMGPU_MAT_DIFFUSE || m.type == LMGPU_MAT_GLOSSY;
+                C = rc.use_color ? rc.color : (refl ? m.c : mk(0, 0, 0));
+                if (!rc.constant) C = C * (.2f + .8f * fabsf(dot(nsh, -dir_of(id))));
+            }
+        } else if (sc.has_env) {
+            // the env endpoint of Scene_::intersect; it has no normal to visualise
+            C = rc.normal ? mk(0, 0, 0) : sc.env_Le;
+        }
+        film[3 * id + 0] = C.x;
+        film[3 * id + 1] = C.y;
+        film[3 * id + 2] = C.z;
+    };
+    trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit, status,
+                                                           nullptr, refill);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) {
+    if (threadIdx.x == 0) *p = 0ull;
+}
+
 __global__ void k_zero_i32(int32_t* p, int n) {
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
@@ -735,6 +793,7 @@ struct Scene {
     int enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const GuidedPass* gp, hipStream_t s);
     int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
                lmgpu_pt_stats* st, hipStream_t s);
+    int raycast(const lmgpu_raycast_params* p, double* film, bool on_device, hipStream_t s);
 };
 
 static inline float f_or(const float* p, int i) { return p ? p[i] : 0.f; }
@@ -1116,6 +1175,38 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     return LMGPU_OK;
 }
 
+int Scene::raycast(const lmgpu_raycast_params* p, double* film_in, bool on_device, hipStream_t s) {
+    if (!p) return fail(LMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!film_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film is NULL");
+    const int npix = w * h;
+    double* film = film_in;
+    if (!on_device) {
+        if (fsum.n != size_t(npix) * 3) LMGPU_TRY(fsum.alloc(size_t(npix) * 3));
+        film = fsum.p;
+    }
+    RaycastConsts rc;
+    rc.bg = mk(p->bg_color[0], p->bg_color[1], p->bg_color[2]);
+    rc.color = mk(p->color[0], p->color[1], p->color[2]);
+    rc.use_color = p->use_color != 0;
+    rc.constant = p->use_constant_color != 0;
+    rc.normal = p->visualize_normal != 0;
+    unsigned long long* next = accel.next_counter();
+    int32_t* status = accel.status_dev();
+    if (!next || !status) return fail(LMGPU_ERR_OUT_OF_MEMORY, "traversal counters");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, next);
+    // camera rays are coherent: keep 64 consecutive pixels per wave
+    const int refill = 64;
+    if (accel.width == 4)
+        hipLaunchKernelGGL(k_raycast<4>, dim3(pt_trace_grid<4>()), dim3(kBlock), 0, s, dev, next, refill, rc, film, status);
+    else if (accel.width == 8)
+        hipLaunchKernelGGL(k_raycast<8>, dim3(pt_trace_grid<8>()), dim3(kBlock), 0, s, dev, next, refill, rc, film, status);
+    else
+        hipLaunchKernelGGL(k_raycast<2>, dim3(pt_trace_grid<2>()), dim3(kBlock), 0, s, dev, next, refill, rc, film, status);
+    LMGPU_HIP(hipGetLastError());
+    if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+    return accel.check_status(s);
+}
+
 }  // namespace lmgpu
 
 using lmgpu::fail;
@@ -1159,6 +1250,13 @@ int lmgpu_pt_render(lmgpu_scene* scene, const lmgpu_pt_params* params, double* f
     lmgpu::clear_error();
     return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, nullptr, stats,
                            static_cast<hipStream_t>(stream));
+}
+
+int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params, double* film, int film_on_device,
+                         void* stream) {
+    if (!scene) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene is NULL");
+    lmgpu::clear_error();
+    return scene->s.raycast(params, film, film_on_device != 0, static_cast<hipStream_t>(stream));
 }
 
 static int64_t gcd64(int64_t a, int64_t b) {

@@ -39,6 +39,7 @@ EXPORTED = [
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
     'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
+    'lmgpu_raycast_render',
 ]
 
 
@@ -108,6 +109,12 @@ class PTStats(ctypes.Structure):
     ]
 
 
+class RaycastParams(ctypes.Structure):
+    _fields_ = [('bg_color', ctypes.c_float * 3), ('color', ctypes.c_float * 3), ('use_color', ctypes.c_int32),
+                ('use_constant_color', ctypes.c_int32), ('visualize_normal', ctypes.c_int32),
+                ('reserved', ctypes.c_int32 * 5)]
+
+
 class GuideOpts(ctypes.Structure):
     _fields_ = [('alpha', ctypes.c_float), ('learning_rate', ctypes.c_float), ('batch', ctypes.c_int32),
                 ('steps_per_pass', ctypes.c_int32), ('record_depth', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
@@ -159,6 +166,7 @@ def lib():
         L.lmgpu_guide_sample.argtypes = [vp, vp, vp, vp, i64, vp]
         L.lmgpu_guide_pdf.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_guide_stats.argtypes = [vp, ctypes.POINTER(i64), f32p]
+        L.lmgpu_raycast_render.argtypes = [vp, ctypes.POINTER(RaycastParams), vp, ctypes.c_int, vp]
         L.lmgpu_pt_render_guided.argtypes = [vp, ctypes.POINTER(PTParams), vp, ctypes.POINTER(GuideOpts), vp, vp,
                                              ctypes.c_int, ctypes.POINTER(PTStats), vp]
         L.lmgpu_guide_train.argtypes = [vp, vp, i64, ctypes.c_float, f32p, vp, vp]

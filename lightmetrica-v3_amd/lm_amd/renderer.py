@@ -180,6 +180,37 @@ class GpuScene:
         return s, q, stats
 
 
+class RendererRaycast:
+    """renderer::raycast_mi355x: drop-in for renderer::raycast
+    (src/renderer/renderer_raycast.cpp:48-97) on the GPU accel."""
+
+    def __init__(self, scene: GpuScene, output: Film, bg_color=(0.0, 0.0, 0.0), use_constant_color: bool = False,
+                 visualize_normal: bool = False, color=None):
+        if (output.w, output.h) != (scene.w, scene.h):
+            raise ValueError('film size differs from the scene film size')
+        self.scene = scene
+        self.film = output
+        p = _lib.RaycastParams()
+        for i in range(3):
+            p.bg_color[i] = float(bg_color[i])
+        if color is not None:
+            p.use_color = 1
+            for i in range(3):
+                p.color[i] = float(color[i])
+        p.use_constant_color = 1 if use_constant_color else 0
+        p.visualize_normal = 1 if visualize_normal else 0
+        self._p = p
+
+    def render(self, stream=None) -> dict:
+        import time
+        t0 = time.perf_counter()
+        img = np.zeros((self.scene.h, self.scene.w, 3), dtype=np.float64)
+        _lib.call('lmgpu_raycast_render', self.scene.handle, ctypes.byref(self._p),
+                  img.ctypes.data_as(ctypes.c_void_p), 0, ctypes.c_void_p(_stream_ptr(stream)))
+        self.film.data[:] = img
+        return {'elapsed': time.perf_counter() - t0}
+
+
 class RendererPT:
     """renderer::pt_mi355x: drop-in for renderer::pt (pixel sampling mode)."""
 
