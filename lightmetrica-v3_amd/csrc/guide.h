@@ -16,6 +16,11 @@ namespace lmgpu {
 
 constexpr int kLobes = 8;
 constexpr int kFreq = 12;
+// Ratio of consecutive encoding frequencies (pi * r^j, j < kFreq).
+#ifndef LMGPU_GUIDE_FREQ_RATIO
+#define LMGPU_GUIDE_FREQ_RATIO 2.0f
+#endif
+constexpr float kFreqRatio = LMGPU_GUIDE_FREQ_RATIO;
 constexpr int kIn = 80;      // 78 inputs + 2 zero pads
 constexpr int kHid = 64;
 constexpr int kOut = 40;     // logits[8], kappa[8], mu[8][3]
@@ -133,7 +138,7 @@ __device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box
             sincosf(f, &s, &c);
             out[o++] = f2bf(s);
             out[o++] = f2bf(c);
-            f *= 2.f;
+            f *= kFreqRatio;
         }
     }
     out[o++] = f2bf(n.x);
@@ -150,31 +155,54 @@ __device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box
 
 namespace lmgpu {
 
-// LDS pitches (in bf16 / f32 elements) of the per-wave tile buffers.
+// LDS pitches (in bf16 / f32 elements) of the per-wave tile buffers. Row
+// pitches of 176 B and 144 B put the 16 rows read by one 16-lane phase of a
+// ds_read_b128 on 16 distinct 4-bank groups.
 constexpr int kPitchX = 88;    // [32][88] bf16: inputs / even layers
 constexpr int kPitchH = 72;    // [32][72] bf16: odd layers
 constexpr int kPitchY = 68;    // [32][68] f32: outputs
-constexpr int kTileLdsBytes = 32 * kPitchX * 2 + 32 * kPitchH * 2 + 32 * kPitchY * 4;
+constexpr int kWaveLds = 32 * kPitchX * 2 + 32 * kPitchH * 2;   // 10240 B; the f32 output tile aliases it
+static_assert(32 * kPitchY * 4 <= kWaveLds, "output tile must fit the aliased buffers");
+
+// BF16 weights staged in LDS once per block, rows padded as the tiles:
+// W1 [64][88], W2..W4 [64][72].
+constexpr int kPW1 = kPitchX, kPWh = kPitchH;
+constexpr int kWOff2 = kHid * kPW1, kWOffH = kPWh * kHid;
+constexpr int kWeightLds = (kHid * kPW1 + 3 * kOutPad * kPWh) * 2;   // 38912 B
+
+__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ Wbf, uint16_t* wl, int tid, int nthr) {
+    for (int c = tid; c < kHid * (kIn / 8); c += nthr) {
+        const int r = c / (kIn / 8), k = (c % (kIn / 8)) * 8;
+        *reinterpret_cast<uint4*>(wl + r * kPW1 + k) = *reinterpret_cast<const uint4*>(Wbf + kOffW1 + r * kIn + k);
+    }
+    const int offs[3] = {kOffW2, kOffW3, kOffW4};
+    for (int l = 0; l < 3; l++)
+        for (int c = tid; c < kOutPad * (kHid / 8); c += nthr) {
+            const int r = c / (kHid / 8), k = (c % (kHid / 8)) * 8;
+            *reinterpret_cast<uint4*>(wl + kWOff2 + l * kWOffH + r * kPWh + k) =
+                *reinterpret_cast<const uint4*>(Wbf + offs[l] + r * kHid + k);
+        }
+}
 
 __device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
 // One dense layer for the 32 rows of a wave tile: out = act(in[32][K] W + b)
-// with in from LDS (bf16, pitch pin), weights Wt[N][K] bf16 in global
-// memory (output-neuron major) and N = 64. The MFMA result layout (lane l,
-// element r: row 8(r/4) + 4(l/32) + r%4, column 32t + l%32) is written
-// back to LDS by `store(row, col, value)`.
-template <int K, class Store>
+// with in from LDS (bf16, pitch pin), weights Wt[N][K] bf16 (output-neuron
+// major, row pitch PW, in LDS or global memory) and N = 64. The MFMA result
+// layout (lane l, element r: row 8(r/4) + 4(l/32) + r%4, column 32t + l%32)
+// is written back by `store(row, col, value)`.
+template <int K, int PW, int UNROLL, class Store>
 __device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin, const uint16_t* __restrict__ Wt,
                                         const float* __restrict__ bias, int lane, Store store) {
     f32x16 acc0 = {}, acc1 = {};
     const int r = lane & 31, h = lane >> 5;
-#pragma unroll
+#pragma unroll UNROLL
     for (int k0 = 0; k0 < K; k0 += 16) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r * pin + k0 + 8 * h);
-        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(Wt + r * K + k0 + 8 * h);
-        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(Wt + (32 + r) * K + k0 + 8 * h);
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(Wt + r * PW + k0 + 8 * h);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(Wt + (32 + r) * PW + k0 + 8 * h);
         acc0 = mfma_bf16(a, b0, acc0);
         acc1 = mfma_bf16(a, b1, acc1);
     }
@@ -188,33 +216,36 @@ __device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin
 }
 
 // Forward pass of the guide MLP for the 32 rows of a wave tile whose encoded
-// inputs are in lx ([32][kPitchX] bf16). Raw outputs land in ly
-// ([32][kPitchY] f32, columns 0..39). `save(layer, row, col, bf16)` sees
-// every hidden activation (layer 1..3) for training. Callers put a block
-// barrier between writing lx and calling this, and after it before reading ly.
+// inputs are in lx ([32][kPitchX] bf16), with the weights staged in LDS by
+// stage_weights (wl). Raw outputs land in ly ([32][kPitchY] f32, columns
+// 0..39). `save(layer, row, col, bf16)` sees every hidden activation
+// (layer 1..3) for training. Every wave of the block calls this the same
+// number of times (block barriers between the layers); callers put a block
+// barrier after staging the weights and writing lx, and after this call
+// before reading ly.
 template <class Save>
-__device__ __forceinline__ void mlp_tile_forward(const uint16_t* __restrict__ Wbf, const float* __restrict__ P,
-                                                 uint16_t* lx, uint16_t* lh, float* ly, int lane, Save save) {
-    dense64<kIn>(lx, kPitchX, Wbf + kOffW1, P + kOffB1, lane, [&](int row, int col, float v) {
+__device__ __forceinline__ void mlp_tile_forward(const uint16_t* wl, const float* __restrict__ P, uint16_t* lx,
+                                                 uint16_t* lh, float* ly, int lane, Save save) {
+    dense64<kIn, kPW1, kIn / 16>(lx, kPitchX, wl, P + kOffB1, lane, [&](int row, int col, float v) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lh[row * kPitchH + col] = b;
         save(1, row, col, b);
     });
     __syncthreads();
-    dense64<kHid>(lh, kPitchH, Wbf + kOffW2, P + kOffB2, lane, [&](int row, int col, float v) {
+    dense64<kHid, kPWh, kHid / 16>(lh, kPitchH, wl + kWOff2, P + kOffB2, lane, [&](int row, int col, float v) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lx[row * kPitchX + col] = b;
         save(2, row, col, b);
     });
     __syncthreads();
-    dense64<kHid>(lx, kPitchX, Wbf + kOffW3, P + kOffB3, lane, [&](int row, int col, float v) {
+    dense64<kHid, kPWh, kHid / 16>(lx, kPitchX, wl + kWOff2 + kWOffH, P + kOffB3, lane, [&](int row, int col, float v) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lh[row * kPitchH + col] = b;
         save(3, row, col, b);
     });
     __syncthreads();
-    dense64<kHid>(lh, kPitchH, Wbf + kOffW4, P + kOffB4, lane,
-                  [&](int row, int col, float v) { ly[row * kPitchY + col] = v; });
+    dense64<kHid, kPWh, kHid / 16>(lh, kPitchH, wl + kWOff2 + 2 * kWOffH, P + kOffB4, lane,
+                           [&](int row, int col, float v) { ly[row * kPitchY + col] = v; });
 }
 
 }  // namespace lmgpu

@@ -33,8 +33,6 @@ namespace {
 
 constexpr int kGB = 256;                  // 4 waves x 32 rows per block tile
 constexpr int kRowsPerBlock = 128;
-constexpr int kWaveLds = 32 * kPitchX * 2 + 32 * kPitchH * 2;   // 10240 B; the f32 output aliases it
-static_assert(32 * kPitchY * 4 <= kWaveLds, "output tile must fit the aliased buffers");
 
 struct Box {
     float lo[3], inv[3];
@@ -60,10 +58,12 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
                                                        const uint16_t* __restrict__ Wbf, const float* __restrict__ P,
                                                        Box box, float* __restrict__ mix, uint16_t* __restrict__ xT,
                                                        uint16_t* __restrict__ hT, float* __restrict__ yout) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
     const int64_t n = batch_rows(ndev, off, n_max);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
+    uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
+    stage_weights(Wbf, wl, threadIdx.x, kGB);
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
     float* ly = reinterpret_cast<float*>(lx);
     for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < n; base += int64_t(gridDim.x) * kRowsPerBlock) {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
                 for (int k = 0; k < kIn; k++) xT[int64_t(k) * B + i] = row[k];
         }
         __syncthreads();
-        mlp_tile_forward(Wbf, P, lx, lh, ly, lane, [&](int layer, int row, int col, uint16_t v) {
+        mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int row, int col, uint16_t v) {
             if (TRAIN) {
                 const int64_t i = i0 + row;
                 if (i < B) hT[(int64_t(layer - 1) * kHid + col) * B + i] = v;
@@ -203,9 +203,16 @@ __global__ __launch_bounds__(kGB) void k_guide_backward(int64_t n_max, const int
                                                         const uint16_t* __restrict__ WnBf,
                                                         const float* __restrict__ zero64,
                                                         const uint16_t* __restrict__ hT, uint16_t* __restrict__ dT) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
+    constexpr int kWnLds = 3 * kHid * kPWh * 2;   // W2..W4 as [k][n], rows padded to kPWh
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kWnLds + 4 * kWaveLds];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
+    uint16_t* wn = reinterpret_cast<uint16_t*>(lds);
+    for (int c = threadIdx.x; c < 3 * kHid * (kHid / 8); c += kGB) {
+        const int r = c / (kHid / 8), k = (c % (kHid / 8)) * 8;   // r over the 3 x 64 rows
+        *reinterpret_cast<uint4*>(wn + r * kPWh + k) = *reinterpret_cast<const uint4*>(WnBf + r * kHid + k);
+    }
+    __syncthreads();
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWnLds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
     const int64_t rows = round128(batch_rows(ndev, off, n_max));
     for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < rows; base += int64_t(gridDim.x) * kRowsPerBlock) {
@@ -221,10 +228,10 @@ __global__ __launch_bounds__(kGB) void k_guide_backward(int64_t n_max, const int
         uint16_t* out = lh;
         int pin = kPitchX, pout = kPitchH;
         for (int l = 4; l >= 2; l--) {
-            const uint16_t* Wn = WnBf + (l - 2) * kHid * kHid;
+            const uint16_t* Wn = wn + (l - 2) * kHid * kPWh;
             const uint16_t* hprev = hT + int64_t(l - 2) * kHid * B;   // h_{l-1}
             uint16_t* dsave = dT + int64_t(l - 2) * kHid * B;          // d_{l-1}
-            dense64<kHid>(in, pin, Wn, zero64, lane, [&](int row, int col, float v) {
+            dense64<kHid, kPWh, 1>(in, pin, Wn, zero64, lane, [&](int row, int col, float v) {
                 const int64_t i = i0 + row;
                 float g = 0.f;
                 if (i < B && bf2f(hprev[int64_t(col) * B + i]) > 0.f) g = v;
@@ -447,8 +454,9 @@ int Guide::get_params(float* host) const {
     return LMGPU_OK;
 }
 
+// persistent grid over 128-row tiles: two blocks per CU (LDS bound)
 static int grid_for(int64_t n) {
-    return int(std::max<int64_t>(1, std::min<int64_t>((n + kRowsPerBlock - 1) / kRowsPerBlock, 256 * 4)));
+    return int(std::max<int64_t>(1, std::min<int64_t>((n + kRowsPerBlock - 1) / kRowsPerBlock, 256 * 2)));
 }
 
 int Guide::infer(const float* query, int qs, float* mix, int64_t n, hipStream_t s) const {

@@ -379,7 +379,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                     const bool spec = is_specular(m);
                     const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
                     // guide mixture of this queue entry (guided vertices only)
+#ifdef LMGPU_GUIDE_DIFFUSE_ONLY
+                    const float* gmix = (GUIDED && m.type == LMGPU_MAT_DIFFUSE) ? gd.mix + int64_t(i) * kOut : nullptr;
+#else
                     const float* gmix = (GUIDED && !spec) ? gd.mix + int64_t(i) * kOut : nullptr;
+#endif
                     const int slot = (gmix && nv - 1 < gd.rec_depth) ? (nv - 1) * c.npix + p : -1;
                     if (nee) {
                         // path::sample_direct (path.h:509-528)
@@ -544,10 +548,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
 // the decoded mixture goes to mix[i]. Entries without a hit get a zero query.
 __global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, int in_q, const uint16_t* __restrict__ Wbf,
                                                         const float* __restrict__ P, GBox box, float* __restrict__ mix) {
-    constexpr int kWaveLds = 32 * kPitchX * 2 + 32 * kPitchH * 2;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[4 * kWaveLds];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + wave * kWaveLds);
+    uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
+    stage_weights(Wbf, wl, threadIdx.x, 256);
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
     float* ly = reinterpret_cast<float*>(lx);
     const int n = q.counts[in_q];
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, i
             }
         }
         __syncthreads();
-        mlp_tile_forward(Wbf, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
+        mlp_tile_forward(wl, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
         __syncthreads();
         if (lane < 32) {
             const int i = i0 + lane;
@@ -1032,7 +1037,7 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
         else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
         if (gp)
-            hipLaunchKernelGGL(k_pt_guide_infer, dim3(std::max(1, std::min((npix + 127) / 128, 2048))), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_pt_guide_infer, dim3(std::max(1, std::min((npix + 127) / 128, 512))), dim3(256), 0, s,
                                dev, q, in_q, gp->g->Wbf.p, gp->g->P.p, box, gmix.p);
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, counts.p, in_q ^ 1);
         if (gp) hipLaunchKernelGGL(k_shade<true>, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q, gd);

@@ -1,0 +1,98 @@
+"""Equal-time comparison of guided and plain wavefront PT on the slit-lit room
+(config 4: light through a 0.3-wide wall slit, GGX floor alpha 0.1, 200K
+clutter triangles, max 16 vertices).
+
+Renders a converged plain-PT reference (FP64 film, `--ref-spp` samples per
+pixel, in chunks that print progress), then plain PT and guided PT (a fresh
+guide trained online during the render) for `--time` seconds each with the
+time scheduler, and reports the relative MSE of both against the reference,
+rMSE = mean((I - R)^2 / (R^2 + 1e-2)), as one JSON line.
+
+    python3 lightmetrica-v3_amd/lm_amd/equal_time.py --width 1920 --height 1080 --time 30 --ref-spp 4096
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run(width=1920, height=1080, seconds=30.0, ref_spp=4096, max_verts=16, chunk=128, guide_opts=None,
+        save_dir=None):
+    import numpy as np
+    import torch
+    from lm_amd import scenes, stats
+    from lm_amd.guide import Guide
+    from lm_amd.renderer import GpuScene
+
+    dev = torch.device('cuda:0')
+    t0 = time.time()
+    flat = scenes.config4(width=width, height=height).flatten()
+    sc = GpuScene(flat)
+    log(f'config 4 {width}x{height}: {flat.num_triangles} triangles, scene ready in {time.time() - t0:.1f} s')
+    # reference: plain PT, FP64 device film, rendered in chunks
+    fs = torch.zeros((height, width, 3), dtype=torch.float64, device=dev)
+    done = 0
+    t0 = time.time()
+    while done < ref_spp:
+        n = min(chunk, ref_spp - done)
+        sc.render_raw(spp=n, max_verts=max_verts, seed=987654321, sample_offset=done, film_sum=fs)
+        torch.cuda.synchronize()
+        done += n
+        log(f'reference {done}/{ref_spp} spp, {time.time() - t0:.1f} s')
+    ref_s = time.time() - t0
+    ref = (fs / ref_spp).cpu().numpy()
+    # plain PT, equal time
+    ps, _, pst = sc.render_raw(render_time=seconds, max_verts=max_verts, seed=5)
+    plain = ps / pst['spp']
+    e_plain = stats.rel_mse(plain, ref)
+    log(f'plain: {pst["spp"]} spp in {pst["elapsed"]:.2f} s, rMSE {e_plain:.4e}')
+    # guided PT, equal time, fresh guide trained during the render
+    lo, hi = sc.bounds()
+    g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
+    gs, _, gst = sc.render_raw(render_time=seconds, max_verts=max_verts, seed=6, guide=g, guide_opts=guide_opts)
+    guided = gs / gst['spp']
+    e_guided = stats.rel_mse(guided, ref)
+    steps, loss = g.stats()
+    log(f'guided: {gst["spp"]} spp in {gst["elapsed"]:.2f} s, rMSE {e_guided:.4e}, {steps} Adam steps, '
+        f'{gst["guide_records"]} training records')
+    if save_dir:
+        os.makedirs(save_dir, exist_ok=True)
+        for name, img in (('reference', ref), ('plain', plain), ('guided', guided)):
+            np.save(os.path.join(save_dir, f'c4_{width}x{height}_{name}.npy'), img.astype(np.float32))
+    return {
+        'scene': f'config 4 (slit-lit room), {width}x{height}, {flat.num_triangles} triangles, max {max_verts} vertices',
+        'seconds': seconds,
+        'reference': {'spp': ref_spp, 'seconds': round(ref_s, 1)},
+        'plain': {'spp': int(pst['spp']), 'seconds': round(pst['elapsed'], 2), 'rmse': e_plain,
+                  'ms_per_pass': round(pst['elapsed'] / pst['spp'] * 1e3, 2)},
+        'guided': {'spp': int(gst['spp']), 'seconds': round(gst['elapsed'], 2), 'rmse': e_guided,
+                   'ms_per_pass': round(gst['elapsed'] / gst['spp'] * 1e3, 2), 'adam_steps': int(steps),
+                   'training_records': int(gst['guide_records']), 'options': guide_opts or 'defaults'},
+        'rmse_ratio_plain_over_guided': e_plain / e_guided,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--width', type=int, default=1920)
+    ap.add_argument('--height', type=int, default=1080)
+    ap.add_argument('--time', type=float, default=30.0)
+    ap.add_argument('--ref-spp', type=int, default=4096)
+    ap.add_argument('--guide-opts', default='', help='JSON dict of lmgpu_guide_opts fields')
+    ap.add_argument('--save-dir', default='')
+    a = ap.parse_args()
+    opts = json.loads(a.guide_opts) if a.guide_opts else None
+    out = run(a.width, a.height, a.time, a.ref_spp, guide_opts=opts, save_dir=a.save_dir or None)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

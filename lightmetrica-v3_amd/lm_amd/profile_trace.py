@@ -1,10 +1,13 @@
 """Profile driver: a fixed number of closest-hit launches over the config-1
-incoherent rays (and, optionally, one wavefront path-tracing pass), preceded
-by a plain device copy of known size that calibrates the HBM byte counters.
+incoherent rays, preceded by a plain device copy of known size that
+calibrates the HBM byte counters; optionally one 1080p config-2
+path-tracing pass (--pt) and 4 plain + 4 guided 1080p passes of config 4
+(--c4).
 
 Run under rocprofv3, e.g.
-  rocprofv3 --kernel-trace --stats -d OUT -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py
-  rocprofv3 --pmc FETCH_SIZE -d OUT -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py
+  rocprofv3 --kernel-trace --stats -d OUT -o kt -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py --pt
+  rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py
+and summarise with lightmetrica-v3_amd/lm_amd/prof_summary.py OUT/kt_results.db.
 """
 import argparse
 import os
@@ -23,6 +26,8 @@ def main():
     ap.add_argument('--leaf', type=int, default=2)
     ap.add_argument('--rays', default='incoherent')
     ap.add_argument('--pt', action='store_true', help='also run one 1080p config-2 PT pass')
+    ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
+    ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
     args = ap.parse_args()
     import torch
     from lm_amd import scenes
@@ -36,16 +41,17 @@ def main():
     torch.cuda.synchronize()
     print(f'copy calibration: {COPY_BYTES} bytes read + written', flush=True)
     del src, dst
-    scene, rays = scenes.config1()
-    flat = scene.flatten()
-    acc = Accel(args.width, args.leaf)
-    acc.build(flat)
-    r = torch.from_numpy(rays_from_dict(rays[args.rays])).to(dev)
-    h = torch.zeros((r.shape[0], 4), dtype=torch.float32, device=dev)
-    for i in range(args.launches):
-        acc.intersect_device(r, h)
-    torch.cuda.synchronize()
-    print(f'{args.launches} x k_trace over {r.shape[0]} {args.rays} rays', flush=True)
+    if not args.no_trace:
+        scene, rays = scenes.config1()
+        flat = scene.flatten()
+        acc = Accel(args.width, args.leaf)
+        acc.build(flat)
+        r = torch.from_numpy(rays_from_dict(rays[args.rays])).to(dev)
+        h = torch.zeros((r.shape[0], 4), dtype=torch.float32, device=dev)
+        for i in range(args.launches):
+            acc.intersect_device(r, h)
+        torch.cuda.synchronize()
+        print(f'{args.launches} x k_trace over {r.shape[0]} {args.rays} rays', flush=True)
     if args.pt:
         from lm_amd.renderer import GpuScene
         s2 = scenes.config2().flatten()
@@ -53,6 +59,20 @@ def main():
         sc.render_raw(spp=1, max_verts=16, seed=1)
         torch.cuda.synchronize()
         print('1 x 1080p config-2 PT pass', flush=True)
+    if args.c4:
+        from lm_amd.guide import Guide
+        from lm_amd.renderer import GpuScene
+        s4 = scenes.config4().flatten()
+        sc = GpuScene(s4, width=args.width, leaf_size=args.leaf)
+        fs = torch.zeros((s4.h, s4.w, 3), dtype=torch.float64, device=dev)
+        _, _, st = sc.render_raw(spp=4, max_verts=16, seed=1, film_sum=fs)
+        torch.cuda.synchronize()
+        print('4 x 1080p config-4 plain PT passes', st, flush=True)
+        lo, hi = sc.bounds()
+        g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
+        _, _, st = sc.render_raw(spp=4, max_verts=16, seed=2, film_sum=fs, guide=g)
+        torch.cuda.synchronize()
+        print('4 x 1080p config-4 guided PT passes', st, flush=True)
 
 
 if __name__ == '__main__':

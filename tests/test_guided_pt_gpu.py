@@ -100,21 +100,27 @@ def test_guided_c4_welch_vs_plain(device):
 
 
 def test_guided_c4_lowers_error(device):
-    """At equal samples per pixel the trained guide lowers the relative MSE
-    against a 2048-spp plain image on the slit-lit room (config 4 at
-    192x108); the guide frozen at its random initialisation does not."""
+    """What the trained network gains over the method without it: on the
+    slit-lit room (config 4 at 192x108) a guide trained online for 384
+    passes, then frozen, lowers the relative MSE against a 2048-spp plain
+    image at equal samples per pixel; the untrained guide (frozen at its
+    random initialisation) does not beat plain PT."""
     flat = c4_flat()
     sc = GpuScene(flat)
     ref = c4_reference(sc)
-    spp = 64
+    spp = 128
     ps, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=21)
     e_plain = stats.rel_mse(ps / spp, ref)
-    # train on a first run, then render with the trained guide still learning
     lo, hi = sc.bounds()
+    g0 = Guide(lo - 1e-3, hi + 1e-3, seed=2)
+    us, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=23, guide=g0, guide_opts={'steps_per_pass': 0})
+    e_untrained = stats.rel_mse(us / spp, ref)
     g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-    sc.render_raw(spp=spp, max_verts=16, seed=22, guide=g, guide_opts={'batch': 16384})
-    gs, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=23, guide=g, guide_opts={'batch': 16384})
+    opts = {'steps_per_pass': 4, 'batch': 16384, 'learning_rate': 5e-3}
+    sc.render_raw(spp=384, max_verts=16, seed=22, guide=g, guide_opts=opts)
+    gs, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=23, guide=g, guide_opts={'steps_per_pass': 0})
     e_guided = stats.rel_mse(gs / spp, ref)
-    print(f'C4 192x108 {spp} spp rMSE: plain {e_plain:.4e} guided {e_guided:.4e} '
-          f'ratio {e_plain / e_guided:.2f}')
-    assert e_guided < e_plain
+    print(f'C4 192x108 {spp} spp rMSE: plain {e_plain:.4e}, untrained guide {e_untrained:.4e}, '
+          f'trained guide {e_guided:.4e}; gain {e_plain / e_guided:.2f}, Adam steps {g.stats()[0]}')
+    assert e_plain / e_guided > 1.15
+    assert e_untrained > e_guided
