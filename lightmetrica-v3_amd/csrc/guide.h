@@ -16,11 +16,7 @@ namespace lmgpu {
 
 constexpr int kLobes = 8;
 constexpr int kFreq = 12;
-// Ratio of consecutive encoding frequencies (pi * r^j, j < kFreq).
-#ifndef LMGPU_GUIDE_FREQ_RATIO
-#define LMGPU_GUIDE_FREQ_RATIO 2.0f
-#endif
-constexpr float kFreqRatio = LMGPU_GUIDE_FREQ_RATIO;
+constexpr float kFreqRatio = 2.f;   // encoding frequencies pi * 2^j, j < kFreq
 constexpr int kIn = 80;      // 78 inputs + 2 zero pads
 constexpr int kHid = 64;
 constexpr int kOut = 40;     // logits[8], kappa[8], mu[8][3]

@@ -379,11 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
                     const bool spec = is_specular(m);
                     const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
                     // guide mixture of this queue entry (guided vertices only)
-#ifdef LMGPU_GUIDE_DIFFUSE_ONLY
-                    const float* gmix = (GUIDED && m.type == LMGPU_MAT_DIFFUSE) ? gd.mix + int64_t(i) * kOut : nullptr;
-#else
                     const float* gmix = (GUIDED && !spec) ? gd.mix + int64_t(i) * kOut : nullptr;
-#endif
                     const int slot = (gmix && nv - 1 < gd.rec_depth) ? (nv - 1) * c.npix + p : -1;
                     if (nee) {
                         // path::sample_direct (path.h:509-528)
