@@ -6,7 +6,11 @@ subdivision levels, + 10K lognormal clutter = 1,320,720 triangles) for the
 uniform over the normal's hemisphere). A step is one closest-hit pass over
 the whole ray batch, rays and BVH resident in HBM. Also reported: coherent
 1920x1080 primaries, the GPU BVH build time, and wavefront path-tracing
-samples/s at 1920x1080 on config 2 (max 16 vertices, NEE + MIS).
+samples/s at 1920x1080 on config 2 (max 16 vertices, NEE + MIS), and the
+batched rendering of config 3 (64 jobs of 10K-2M triangles, 256-1024^2
+pixels, 16-256 spp, one wavefront loop over a device work queue with
+autotuned BVH width / path capacity / block size): jobs/s and p50 / p99 job
+latency.
 
 Prints one JSON line. `roofline` is for the dominant kernel (closest-hit
 traversal, k_trace): algorithmic bytes per launch = rays x B_ray, with
@@ -78,6 +82,7 @@ def main():
     ap.add_argument('--pt-spp', type=int, default=4)
     ap.add_argument('--no-pt', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-c3', action='store_true')
     ap.add_argument('--cpu-rays', type=int, default=2_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -159,6 +164,29 @@ def main():
                              'extension_rays': st['extension_rays'], 'shadow_rays': st['shadow_rays'],
                              'ms_per_spp': round(dt / args.pt_spp * 1e3, 2)}
         log('pt', extra['pt_1080p'])
+
+    if not args.no_c3:
+        from lm_amd.batch import Batch
+        log('generating config-3 jobs (64 scenes)')
+        jobs = scenes.config3_jobs()
+        b = Batch()
+        t0 = time.time()
+        for j in jobs:
+            b.add_job(scenes.config3_scene(j).flatten(), spp=j['spp'], max_verts=16, seed=j['seed'])
+        gen_s = time.time() - t0
+        st = b.run()
+        lat = np.array([b.latency_ms(i) for i in range(len(jobs))])
+        extra['c3_batch'] = {
+            'jobs': len(jobs), 'jobs_s': round(len(jobs) / st['elapsed_s'], 3),
+            'p50_latency_ms': round(float(np.percentile(lat, 50)), 1),
+            'p99_latency_ms': round(float(np.percentile(lat, 99)), 1),
+            'samples_s': round(st['samples'] / st['elapsed_s'], 1), 'render_s': round(st['elapsed_s'], 3),
+            'build_s': round(st['build_s'], 3), 'tune_s': round(st['tune_s'], 3),
+            'triangles': int(sum(j['tris'] for j in jobs)), 'samples': int(st['samples']),
+            'autotuned': {'bvh_width': st['width'], 'path_capacity': st['path_capacity'], 'block': st['block']},
+            'scene_generation_s': round(gen_s, 1)}
+        log('c3', extra['c3_batch'])
+        del b
 
     cpu = None
     if not args.no_cpu:

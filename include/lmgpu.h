@@ -229,6 +229,54 @@ typedef struct lmgpu_raycast_params {
 int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params, double* film, int film_on_device,
                          void* stream);
 
+/* ---- batched rendering ------------------------------------------------------
+ * Many renderer::pt jobs (each its own scene, film size, spp and seed) in
+ * one wavefront loop over a pool of path slots fed from a device work queue;
+ * the jobs' BVHs share one device arena. A job renders what lmgpu_pt_render
+ * renders for the same scene, parameters and BVH width. No counterpart in
+ * the reference (its renderers run one scene at a time). */
+typedef struct lmgpu_batch lmgpu_batch;
+
+typedef struct lmgpu_job_desc {
+    const lmgpu_scene_desc* scene;   /* copied by lmgpu_batch_add_job */
+    int32_t spp;
+    int32_t max_verts;
+    int32_t sampling_mode;
+    int32_t rr_depth;
+    uint64_t seed;
+    int32_t reserved[4];
+} lmgpu_job_desc;
+
+typedef struct lmgpu_batch_opts {
+    int32_t width;          /* BVH width 2, 4, 8; 0 = autotune */
+    int32_t path_capacity;  /* paths in flight (queue capacity), 1024..2^24; 0 = autotune */
+    int32_t block;          /* block size of the generate / shade kernels, 128 or 256; 0 = autotune */
+    int32_t leaf_size;      /* 0 = 2 */
+    int32_t reserved[4];
+} lmgpu_batch_opts;
+
+typedef struct lmgpu_batch_stats {
+    int64_t jobs, samples, extension_rays, shadow_rays, iterations;
+    double elapsed_s;       /* device time of the render loop */
+    double build_s;         /* BVH builds and arena assembly in this call */
+    double tune_s;          /* autotuning trials in this call (builds excluded) */
+    int32_t width, path_capacity, block, reserved0;
+    int64_t reserved[4];
+} lmgpu_batch_stats;
+
+int lmgpu_batch_create(lmgpu_batch** out);
+int lmgpu_batch_destroy(lmgpu_batch* batch);
+/* Copies the job's scene and parameters; returns its index in job_id. */
+int lmgpu_batch_add_job(lmgpu_batch* batch, const lmgpu_job_desc* job, int32_t* job_id);
+/* Renders every job (fresh films); autotunes the fields of opts left 0 on
+ * the first run and reuses that choice afterwards. Synchronous. */
+int lmgpu_batch_run(lmgpu_batch* batch, const lmgpu_batch_opts* opts, lmgpu_batch_stats* stats, void* stream);
+/* FP64 sum / sum of squares ([h][w][3], host) of a job's samples in the
+ * last run, and its latency: device time from the start of the run to the
+ * accumulation of its last sample. */
+int lmgpu_batch_job_film(const lmgpu_batch* batch, int32_t job, double* film_sum, double* film_sumsq,
+                         double* latency_ms);
+
 /* ---- learned path guiding --------------------------------------------------
  * No counterpart in the reference: an MLP maps (position, shading normal,
  * outgoing direction) to a mixture of 8 von Mises-Fisher lobes over incident

@@ -340,13 +340,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const WNode<W>* __restric
                                                         int32_t* __restrict__ status,
                                                         unsigned long long* __restrict__ counters, int refill) {
     __shared__ int lds[kLdsDepth][kTraceBlock];
-    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(rays + id);
         const float4 a = rp[0], c = rp[1];
         o = mk(a.x, a.y, a.z);
         d = mk(c.x, c.y, c.z);
         tmin = a.w;
         tmax = c.w;
+        return 0;
     };
     auto emit = [&](int64_t id, const TraceResult& r) {
         if (ANY) {

@@ -1,0 +1,822 @@
+// Batched multi-job renderer::pt (C ABI: lmgpu_batch_*).
+//
+// Many render jobs of uneven scene size, resolution and spp are rendered by
+// one wavefront loop over a fixed pool of path slots:
+//   * the jobs' BVHs live in one node / triangle arena (child indices made
+//     global, so one traversal launch serves every job; a ray starts at its
+//     job's root), their shading data (triangles, materials, lights) in
+//     arenas addressed through per-job base pointers;
+//   * one device work queue enumerates (job, sample, pixel) items job by job;
+//     free path slots take the next items, finished paths add their sample to
+//     their job's FP64 film (atomics) and count it, and the job that counts
+//     its last sample records the device clock (job latency);
+//   * an iteration is regenerate -> extend (persistent closest hit) -> shade
+//     (shade_vertex, the transport of renderer::pt) -> connect (persistent
+//     any hit); two iterations (both queue parities) are captured once as a
+//     hipGraph and replayed until every work item has been accumulated.
+// A job renders exactly what lmgpu_pt_render renders for the same scene,
+// seed and BVH: same Philox streams, same per-vertex code.
+//
+// BVH width, path-pool capacity (queue capacity) and the block size of the
+// regenerate / shade kernels are chosen by an autotuner when the caller
+// leaves them 0: a coordinate search over {2,4,8} x {2^19,2^20,2^21} x
+// {128,256} timing one sample per pixel of every job.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lmgpu_internal.h"
+#include "pt_device.h"
+#include "pt_internal.h"
+
+namespace lmgpu {
+
+namespace {
+
+struct JobDev {
+    SceneDev sc;          // job-local pointers into the shading arenas
+    int root;             // root of the job's BVH in the node arena
+    int npix, spp, max_verts, mode, rr_depth;
+    uint32_t seed_lo, seed_hi;
+    int64_t work_off;     // first work item of the job
+    int64_t film_off;     // first double of the job's film (3 per pixel)
+};
+
+constexpr int32_t kFree = 0, kActive = 1, kFinishing = 2;
+
+struct PoolDev {
+    int32_t *job, *pixel, *sample, *state;
+    float4 *thr, *L, *prev;
+    int4* info;
+};
+
+struct BatchQ {
+    lmgpu_ray* ext[2];
+    int32_t* ext_path[2];
+    lmgpu_hit* hit;
+    lmgpu_ray* sh;
+    float4* sh_contrib;   // xyz contribution, w = path slot bits
+    int32_t* counts;      // [0],[1] extension queues, [2] shadow queue, [5] stack overflow flag
+};
+
+struct BatchCtl {
+    unsigned long long work_next;   // next work item to hand out
+    unsigned long long remaining;   // work items not yet accumulated
+    unsigned long long ext_rays, shadow_rays;
+    unsigned long long t_start;     // device clock at the start of the run
+};
+
+__device__ __forceinline__ uint64_t job_seed(const JobDev& J) { return (uint64_t(J.seed_hi) << 32) | J.seed_lo; }
+
+// Frees finishing paths (their sample goes to the job film) and hands the
+// free slots new work items, appending their camera rays to queue in_q.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jobs, const int64_t* __restrict__ work_offs,
+                                                   int njobs, int64_t total, PoolDev P, BatchQ q, int in_q, int cap,
+                                                   BatchCtl* ctl, unsigned long long* job_done,
+                                                   unsigned long long* job_time, double* film_sum, double* film_sq) {
+    const int stride = gridDim.x * BLOCK;
+    const int n_iter = (cap + stride - 1) / stride;
+    const int lane = __lane_id();
+    for (int it = 0; it < n_iter; it++) {
+        const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        bool want = false;
+        if (s < cap) {
+            int32_t st = P.state[s];
+            if (st == kFinishing) {
+                const int j = P.job[s];
+                const JobDev& J = jobs[j];
+                const float4 L = P.L[s];
+                const int64_t f = J.film_off + 3 * int64_t(P.pixel[s]);
+                atomicAdd(film_sum + f + 0, double(L.x));
+                atomicAdd(film_sum + f + 1, double(L.y));
+                atomicAdd(film_sum + f + 2, double(L.z));
+                if (film_sq) {
+                    atomicAdd(film_sq + f + 0, double(L.x) * double(L.x));
+                    atomicAdd(film_sq + f + 1, double(L.y) * double(L.y));
+                    atomicAdd(film_sq + f + 2, double(L.z) * double(L.z));
+                }
+                const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
+                if (atomicAdd(job_done + j, 1ull) + 1ull == need) job_time[j] = wall_clock64();
+                atomicAdd(&ctl->remaining, ~0ull);   // - 1
+                st = kFree;
+                P.state[s] = kFree;
+            }
+            want = st == kFree;
+        }
+        const unsigned long long mask = __ballot(want);
+        unsigned long long base = 0;
+        if (mask) {
+            const int leader = __ffsll(mask) - 1;
+            if (lane == leader) base = atomicAdd(&ctl->work_next, (unsigned long long)__popcll(mask));
+            base = (unsigned long long)__shfl((long long)base, leader);
+        }
+        bool emit = false;
+        lmgpu_ray ray;
+        if (want) {
+            const unsigned long long w = base + __popcll(mask & ((1ull << lane) - 1ull));
+            if (w < (unsigned long long)total) {
+                // job of item w: last job whose first item is <= w
+                int lo = 0, hi = njobs - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (work_offs[mid] <= int64_t(w)) lo = mid;
+                    else hi = mid - 1;
+                }
+                const JobDev& J = jobs[lo];
+                const int64_t local = int64_t(w) - J.work_off;
+                const int sample = int(local / J.npix), pix = int(local % J.npix);
+                P.job[s] = lo;
+                P.pixel[s] = pix;
+                P.sample[s] = sample;
+                P.state[s] = kActive;
+                P.thr[s] = make_float4(1.f, 1.f, 1.f, 0.f);
+                P.L[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                P.prev[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                P.info[s] = make_int4(1, 0, 2, 0);   // camera vertex: degenerate, not NEE-samplable
+                ray = camera_ray(J.sc, pix, uint32_t(sample), job_seed(J));
+                emit = true;
+            }
+        }
+        const int k = wave_append(q.counts + in_q, emit);
+        if (emit) {
+            q.ext[in_q][k] = ray;
+            q.ext_path[in_q][k] = s;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_b_extend(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+                                                     const JobDev* __restrict__ jobs, PoolDev P, BatchQ q, int in_q,
+                                                     unsigned long long* next, int refill, BatchCtl* ctl) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = q.counts[in_q];
+    const lmgpu_ray* rays = q.ext[in_q];
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
+        const float4* rp = reinterpret_cast<const float4*>(rays + id);
+        const float4 a = rp[0], b = rp[1];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+        return jobs[P.job[q.ext_path[in_q][id]]].root;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
+        reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
+    };
+    trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes, tris, n, next, lds, fetch, emit, q.counts + 5,
+                                                           nullptr, refill);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->ext_rays, (unsigned long long)n);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jobs, PoolDev P, BatchQ q, int in_q) {
+    const int n = q.counts[in_q];
+    const int out_q = in_q ^ 1;
+    const int stride = gridDim.x * BLOCK;
+    const int n_iter = (n + stride - 1) / stride;
+    const GuideDev nog{};
+    for (int it = 0; it < n_iter; it++) {
+        const int i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        VertexOut out;
+        int s = 0;
+        if (i < n) {
+            s = q.ext_path[in_q][i];
+            const JobDev& J = jobs[P.job[s]];
+            const float4* rp = reinterpret_cast<const float4*>(q.ext[in_q] + i);
+            const float4 ra = rp[0], rb = rp[1];
+            const float4 h = reinterpret_cast<const float4*>(q.hit)[i];
+            if (__float_as_int(h.x) == -2) atomicOr(q.counts + 5, 1);
+            const float4 thr4 = P.thr[s], L4 = P.L[s];
+            PathRegs R{mk(thr4.x, thr4.y, thr4.z), mk(L4.x, L4.y, L4.z), P.prev[s], P.info[s]};
+            PTConsts c;
+            c.max_verts = J.max_verts;
+            c.mode = J.mode;
+            c.rr_depth = J.rr_depth;
+            c.seed_lo = J.seed_lo;
+            c.seed_hi = J.seed_hi;
+            c.npix = J.npix;
+            shade_vertex<false>(J.sc, c, uint32_t(P.pixel[s]), uint32_t(P.sample[s]), job_seed(J), s,
+                                mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), h, R, nog, nullptr, out);
+            if (out.ext) {
+                P.prev[s] = R.prev;
+                P.info[s] = R.info;
+                P.thr[s] = make_float4(R.thr.x, R.thr.y, R.thr.z, 0.f);
+            } else {
+                // the path ends here; its NEE of this vertex lands in connect
+                P.state[s] = kFinishing;
+            }
+            P.L[s] = make_float4(R.L.x, R.L.y, R.L.z, 0.f);
+        }
+        const int se = wave_append(q.counts + out_q, out.ext);
+        if (out.ext) {
+            q.ext[out_q][se] = out.ext_ray;
+            q.ext_path[out_q][se] = s;
+        }
+        const int ss = wave_append(q.counts + 2, out.sh);
+        if (out.sh) {
+            q.sh[ss] = out.sh_ray;
+            q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(s));
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_b_connect(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+                                                      const JobDev* __restrict__ jobs, PoolDev P, BatchQ q,
+                                                      unsigned long long* next, int refill, BatchCtl* ctl) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = q.counts[2];
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
+        const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
+        const float4 a = rp[0], b = rp[1];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+        return jobs[P.job[__float_as_int(q.sh_contrib[id].w)]].root;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
+        if (r.prim < 0 && !r.overflow) {
+            const float4 c = q.sh_contrib[id];
+            const int s = __float_as_int(c.w);
+            float4 L = P.L[s];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            P.L[s] = L;
+        }
+    };
+    trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes, tris, n, next, lds, fetch, emit, q.counts + 5, nullptr,
+                                                          refill);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
+}
+
+// Makes a job's BVH node indices global: internal children += node_off,
+// leaf triangle ranges += tri_off (empty slots unchanged).
+template <int W>
+__global__ void k_b_fix_nodes(WNode<W>* nodes, int64_t n, int node_off, int tri_off) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    WNode<W>& nd = nodes[i];
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        if (nd.count[c] == 0) nd.child[c] += node_off;
+        else if (nd.count[c] > 0) nd.child[c] += tri_off;
+    }
+}
+
+__global__ void k_b_init(BatchCtl* ctl, int64_t total, int32_t* counts) {
+    if (threadIdx.x == 0) {
+        ctl->work_next = 0;
+        ctl->remaining = (unsigned long long)total;
+        ctl->ext_rays = 0;
+        ctl->shadow_rays = 0;
+        ctl->t_start = wall_clock64();
+    }
+    if (threadIdx.x < 16) counts[threadIdx.x] = 0;
+}
+
+__global__ void k_b_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_b_fill_u64(unsigned long long* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_b_fill_f64(double* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_b_zero2(unsigned long long* next) {
+    if (threadIdx.x < 2) next[threadIdx.x] = 0ull;
+}
+
+__global__ void k_b_reset(int32_t* counts, int out_q) {
+    if (threadIdx.x == 0) {
+        counts[out_q] = 0;
+        counts[2] = 0;
+    }
+}
+
+inline unsigned nblk(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+template <class K>
+int occupancy_grid(K kernel, int block) {
+    int dev = 0, cus = 0, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0);
+    return std::max(1, cus) * std::max(1, occ);
+}
+
+}  // namespace
+
+// Host copy of one job (the caller's buffers are only valid during add_job).
+struct BatchJob {
+    std::vector<float> verts, normals, mat_param, prim_Ke;
+    std::vector<int32_t> tri_prim, mat_type, light_prims;
+    lmgpu_scene_desc desc{};
+    int spp = 1, max_verts = 2, mode = 2, rr_depth = 5;
+    uint64_t seed = 0;
+};
+
+struct Batch {
+    std::vector<BatchJob> jobs;
+    // arena (valid for arena_width / arena_leaf)
+    int arena_width = 0, arena_leaf = 0;
+    DevBuf<float4> nodes;
+    DevBuf<TriV> tris;
+    DevBuf<ShadeTri> stri;
+    DevBuf<DevMat> mats;
+    DevBuf<DevLight> lights;
+    DevBuf<float> cdf;
+    DevBuf<int32_t> ltri;
+    std::vector<JobDev> jobdev;   // per job, work_off / film_off filled per run
+    // run state
+    DevBuf<JobDev> djobs;
+    DevBuf<int64_t> dwork;
+    DevBuf<unsigned long long> job_done, job_time;
+    DevBuf<double> film, film_sq;
+    int64_t film_doubles = 0;
+    std::vector<double> latency_ms;
+    bool have_results = false;
+    // path pool
+    int cap = 0;
+    DevBuf<int32_t> pjob, ppix, psample, pstate;
+    DevBuf<float4> pthr, pL, pprev;
+    DevBuf<int4> pinfo;
+    DevBuf<lmgpu_ray> ext0, ext1, sh;
+    DevBuf<int32_t> path0, path1, counts;
+    DevBuf<lmgpu_hit> hit;
+    DevBuf<float4> shc;
+    DevBuf<BatchCtl> ctl;
+    DevBuf<unsigned long long> next;
+    // autotuned choice
+    int tuned_width = 0, tuned_cap = 0, tuned_block = 0;
+
+    int build_arena(int width, int leaf, hipStream_t s);
+    int ensure_pool(int cap);
+    int run_once(int width, int block, int spp_cap, bool keep_results, double* elapsed, int64_t* iters,
+                 hipStream_t s);
+};
+
+int Batch::build_arena(int width, int leaf, hipStream_t s) {
+    if (arena_width == width && arena_leaf == leaf) return LMGPU_OK;
+    arena_width = 0;
+    // element counts per job: nodes, tris, shading tris, materials, lights, cdf entries, light tris
+    struct Part {
+        int64_t v[7];
+    };
+    std::vector<Part> parts;
+    std::vector<Scene*> scenes;
+    auto cleanup = [&]() {
+        for (Scene* sc : scenes) delete sc;
+        scenes.clear();
+    };
+    lmgpu_build_opts o{};
+    o.width = width;
+    o.leaf_size = leaf;
+    o.builder = LMGPU_BUILDER_LBVH;
+    int64_t tot[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < jobs.size(); j++) {
+        Scene* sc = new (std::nothrow) Scene();
+        if (!sc) {
+            cleanup();
+            return fail(LMGPU_ERR_OUT_OF_MEMORY, "job scene");
+        }
+        scenes.push_back(sc);
+        const int rc = sc->create(&jobs[j].desc, &o);
+        if (rc != LMGPU_OK) {
+            cleanup();
+            return fail(rc, "job " + std::to_string(j) + ": " + lmgpu_last_error());
+        }
+        Part p{{sc->accel.num_nodes, int64_t(sc->accel.tris.n), int64_t(sc->stri.n), int64_t(sc->mats.n),
+                int64_t(sc->lights.n), int64_t(sc->light_cdf.n), int64_t(sc->light_tri.n)}};
+        parts.push_back(p);
+        for (int k = 0; k < 7; k++) tot[k] += p.v[k];
+        if (tot[0] >= (int64_t(1) << 31) || tot[1] >= (int64_t(1) << 27)) {
+            cleanup();
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch too large for one arena (nodes or triangles)");
+        }
+    }
+    const int f4_per_node = 2 * width;   // 32 * W bytes
+    int rc = LMGPU_OK;
+    if ((rc = nodes.alloc(size_t(tot[0]) * f4_per_node)) || (rc = tris.alloc(tot[1])) || (rc = stri.alloc(tot[2])) ||
+        (rc = mats.alloc(tot[3])) || (rc = lights.alloc(tot[4])) || (rc = cdf.alloc(tot[5])) ||
+        (rc = ltri.alloc(tot[6]))) {
+        cleanup();
+        return rc;
+    }
+    jobdev.assign(jobs.size(), JobDev{});
+    int64_t off[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < jobs.size(); j++) {
+        Scene* sc = scenes[j];
+        const Part& p = parts[j];
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) : hipSuccess;
+        };
+        hipError_t e = hipSuccess;
+        if (e == hipSuccess) e = cp(nodes.p + off[0] * f4_per_node, sc->accel.nodes.p, size_t(p.v[0]) * 32 * width);
+        if (e == hipSuccess) e = cp(tris.p + off[1], sc->accel.tris.p, sizeof(TriV) * p.v[1]);
+        if (e == hipSuccess) e = cp(stri.p + off[2], sc->stri.p, sizeof(ShadeTri) * p.v[2]);
+        if (e == hipSuccess) e = cp(mats.p + off[3], sc->mats.p, sizeof(DevMat) * p.v[3]);
+        if (e == hipSuccess) e = cp(lights.p + off[4], sc->lights.p, sizeof(DevLight) * p.v[4]);
+        if (e == hipSuccess) e = cp(cdf.p + off[5], sc->light_cdf.p, sizeof(float) * p.v[5]);
+        if (e == hipSuccess) e = cp(ltri.p + off[6], sc->light_tri.p, sizeof(int32_t) * p.v[6]);
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(LMGPU_ERR_DEVICE, std::string("arena copy: ") + hipGetErrorString(e));
+        }
+        void* nbase = nodes.p + off[0] * f4_per_node;
+        if (width == 2)
+            hipLaunchKernelGGL(k_b_fix_nodes<2>, dim3(nblk(p.v[0], 256)), dim3(256), 0, s,
+                               static_cast<WNode<2>*>(nbase), p.v[0], int(off[0]), int(off[1]));
+        else if (width == 4)
+            hipLaunchKernelGGL(k_b_fix_nodes<4>, dim3(nblk(p.v[0], 256)), dim3(256), 0, s,
+                               static_cast<WNode<4>*>(nbase), p.v[0], int(off[0]), int(off[1]));
+        else
+            hipLaunchKernelGGL(k_b_fix_nodes<8>, dim3(nblk(p.v[0], 256)), dim3(256), 0, s,
+                               static_cast<WNode<8>*>(nbase), p.v[0], int(off[0]), int(off[1]));
+        JobDev& J = jobdev[j];
+        J.sc = sc->dev;
+        J.sc.nodes2 = nullptr;
+        J.sc.nodes4 = nullptr;
+        J.sc.nodes8 = nullptr;
+        J.sc.tris = tris.p;
+        J.sc.stri = stri.p + off[2];
+        J.sc.mats = mats.p + off[3];
+        J.sc.lights = lights.p + off[4];
+        J.sc.light_cdf = cdf.p + off[5];
+        J.sc.light_tri = ltri.p + off[6];
+        J.root = int(off[0]);
+        for (int k = 0; k < 7; k++) off[k] += p.v[k];
+        // the per-job scene is no longer needed once its data is in the arena
+        LMGPU_HIP(hipStreamSynchronize(s));
+        delete sc;
+        scenes[j] = nullptr;
+    }
+    LMGPU_HIP(hipGetLastError());
+    LMGPU_HIP(hipStreamSynchronize(s));
+    cleanup();
+    arena_width = width;
+    arena_leaf = leaf;
+    return LMGPU_OK;
+}
+
+int Batch::ensure_pool(int c) {
+    if (cap == c) return LMGPU_OK;
+    cap = 0;
+    LMGPU_TRY(pjob.alloc(c)); LMGPU_TRY(ppix.alloc(c)); LMGPU_TRY(psample.alloc(c)); LMGPU_TRY(pstate.alloc(c));
+    LMGPU_TRY(pthr.alloc(c)); LMGPU_TRY(pL.alloc(c)); LMGPU_TRY(pprev.alloc(c)); LMGPU_TRY(pinfo.alloc(c));
+    LMGPU_TRY(ext0.alloc(c)); LMGPU_TRY(ext1.alloc(c)); LMGPU_TRY(sh.alloc(c));
+    LMGPU_TRY(path0.alloc(c)); LMGPU_TRY(path1.alloc(c)); LMGPU_TRY(hit.alloc(c)); LMGPU_TRY(shc.alloc(c));
+    if (!counts.p) LMGPU_TRY(counts.alloc(16));
+    if (!ctl.p) LMGPU_TRY(ctl.alloc(1));
+    if (!next.p) LMGPU_TRY(next.alloc(2));
+    cap = c;
+    return LMGPU_OK;
+}
+
+// Renders every job once with the current arena and pool: at most spp_cap
+// samples per pixel per job when spp_cap > 0 (autotuning trials).
+int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double* elapsed, int64_t* iters,
+                    hipStream_t s) {
+    const int nj = int(jobs.size());
+    int64_t total = 0, film_n = 0;
+    std::vector<int64_t> work(nj);
+    for (int j = 0; j < nj; j++) {
+        JobDev& J = jobdev[j];
+        const BatchJob& B = jobs[j];
+        J.npix = B.desc.w * B.desc.h;
+        J.spp = spp_cap > 0 ? std::min(B.spp, spp_cap) : B.spp;
+        J.max_verts = B.max_verts;
+        J.mode = B.mode;
+        J.rr_depth = B.rr_depth;
+        J.seed_lo = uint32_t(B.seed);
+        J.seed_hi = uint32_t(B.seed >> 32);
+        J.work_off = total;
+        J.film_off = film_n;
+        work[j] = total;
+        total += int64_t(J.npix) * J.spp;
+        film_n += int64_t(J.npix) * 3;
+    }
+    if (djobs.n != size_t(nj)) LMGPU_TRY(djobs.alloc(nj));
+    if (dwork.n != size_t(nj)) LMGPU_TRY(dwork.alloc(nj));
+    if (job_done.n != size_t(nj)) {
+        LMGPU_TRY(job_done.alloc(nj));
+        LMGPU_TRY(job_time.alloc(nj));
+    }
+    if (film_doubles != film_n) {
+        LMGPU_TRY(film.alloc(film_n));
+        LMGPU_TRY(film_sq.alloc(film_n));
+        film_doubles = film_n;
+    }
+    LMGPU_HIP(hipStreamSynchronize(s));
+    LMGPU_HIP(hipMemcpy(djobs.p, jobdev.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice));
+    LMGPU_HIP(hipMemcpy(dwork.p, work.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_done.p, int64_t(nj));
+    hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_time.p, int64_t(nj));
+    hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film.p, film_n);
+    hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film_sq.p, film_n);
+    hipLaunchKernelGGL(k_b_fill_i32, dim3(nblk(cap, 256)), dim3(256), 0, s, pstate.p, int64_t(cap), kFree);
+    hipLaunchKernelGGL(k_b_init, dim3(1), dim3(64), 0, s, ctl.p, total, counts.p);
+
+    PoolDev P{pjob.p, ppix.p, psample.p, pstate.p, pthr.p, pL.p, pprev.p, pinfo.p};
+    BatchQ q{{ext0.p, ext1.p}, {path0.p, path1.p}, hit.p, sh.p, shc.p, counts.p};
+    const int refill = 32;
+    // two iterations (queue parities 0 and 1) captured as one graph
+    hipStream_t cs;
+    LMGPU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int par = 0; par < 2; par++) {
+        const int in_q = par;
+        const unsigned rg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)));
+        if (block == 128)
+            hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
+                               ctl.p, job_done.p, job_time.p, film.p, film_sq.p);
+        else
+            hipLaunchKernelGGL(k_b_regen<256>, dim3(rg), dim3(256), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
+                               ctl.p, job_done.p, job_time.p, film.p, film_sq.p);
+        hipLaunchKernelGGL(k_b_zero2, dim3(1), dim3(64), 0, cs, next.p);
+        if (width == 2)
+            hipLaunchKernelGGL(k_b_extend<2>, dim3(occupancy_grid(k_b_extend<2>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<2>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+        else if (width == 4)
+            hipLaunchKernelGGL(k_b_extend<4>, dim3(occupancy_grid(k_b_extend<4>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<4>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+        else
+            hipLaunchKernelGGL(k_b_extend<8>, dim3(occupancy_grid(k_b_extend<8>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<8>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+        hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, cs, counts.p, in_q ^ 1);
+        const unsigned sg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 2048 * 256 / block)));
+        if (block == 128) hipLaunchKernelGGL(k_b_shade<128>, dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q);
+        else hipLaunchKernelGGL(k_b_shade<256>, dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q);
+        if (width == 2)
+            hipLaunchKernelGGL(k_b_connect<2>, dim3(occupancy_grid(k_b_connect<2>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<2>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+        else if (width == 4)
+            hipLaunchKernelGGL(k_b_connect<4>, dim3(occupancy_grid(k_b_connect<4>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<4>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+        else
+            hipLaunchKernelGGL(k_b_connect<8>, dim3(occupancy_grid(k_b_connect<8>, kBlock)), dim3(kBlock), 0, cs,
+                               reinterpret_cast<const WNode<8>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+    }
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fail(LMGPU_ERR_DEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t ge;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail(LMGPU_ERR_DEVICE, std::string("batch graph instantiate: ") + hipGetErrorString(e));
+    // every work item needs at most max_verts + 1 iterations in its slot
+    int max_v = 2;
+    for (const BatchJob& B : jobs) max_v = std::max(max_v, B.max_verts);
+    const int64_t iter_cap = (total / cap + 2) * (max_v + 2) + 64;
+    hipEvent_t e0, e1;
+    LMGPU_HIP(hipEventCreate(&e0));
+    LMGPU_HIP(hipEventCreate(&e1));
+    LMGPU_HIP(hipEventRecord(e0, s));
+    int64_t it = 0;
+    int rc = LMGPU_OK;
+    while (true) {
+        for (int r = 0; r < 8; r++) {
+            if ((e = hipGraphLaunch(ge, s)) != hipSuccess) break;
+            it += 2;
+        }
+        if (e != hipSuccess) {
+            rc = fail(LMGPU_ERR_DEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
+            break;
+        }
+        BatchCtl hc;
+        if ((e = hipMemcpyAsync(&hc, ctl.p, sizeof(hc), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess) {
+            rc = fail(LMGPU_ERR_DEVICE, std::string("batch progress: ") + hipGetErrorString(e));
+            break;
+        }
+        if (hc.remaining == 0) break;
+        if (it > iter_cap) {
+            rc = fail(LMGPU_ERR_FAILED_TO_RENDER, "batch did not finish within its iteration bound");
+            break;
+        }
+    }
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ge);
+    if (rc != LMGPU_OK) return rc;
+    int32_t cnt[16];
+    LMGPU_HIP(hipMemcpy(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during rendering");
+    *elapsed = ms * 1e-3;
+    *iters = it;
+    if (keep_results) {
+        // job latency = device clock at the job's last sample - clock at the start
+        std::vector<unsigned long long> t(nj);
+        BatchCtl hc;
+        LMGPU_HIP(hipMemcpy(t.data(), job_time.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&hc, ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        int dev = 0, khz = 0;
+        LMGPU_HIP(hipGetDevice(&dev));
+        LMGPU_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        latency_ms.assign(nj, 0.0);
+        for (int j = 0; j < nj; j++)
+            latency_ms[j] = khz > 0 ? double(t[j] - hc.t_start) / double(khz) : 0.0;
+        have_results = true;
+    }
+    return LMGPU_OK;
+}
+
+}  // namespace lmgpu
+
+using lmgpu::fail;
+
+extern "C" {
+
+struct lmgpu_batch {
+    lmgpu::Batch b;
+};
+
+int lmgpu_batch_create(lmgpu_batch** out) {
+    if (!out) return fail(LMGPU_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = new (std::nothrow) lmgpu_batch();
+    if (!*out) return fail(LMGPU_ERR_OUT_OF_MEMORY, "batch handle");
+    return LMGPU_OK;
+}
+
+int lmgpu_batch_destroy(lmgpu_batch* b) {
+    delete b;
+    return LMGPU_OK;
+}
+
+int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_id) {
+    if (!b || !job || !job->scene) return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL argument");
+    const lmgpu_scene_desc& d = *job->scene;
+    if (d.num_tris <= 0 || !d.verts || !d.normals || !d.tri_prim || d.num_prims <= 0 || !d.mat_type || !d.mat_param)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "job scene needs triangles, normals, primitive ids and materials");
+    if (d.w <= 0 || d.h <= 0 || int64_t(d.w) * d.h > (int64_t(1) << 26))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "job film size out of range");
+    if (job->spp < 1) return fail(LMGPU_ERR_INVALID_ARGUMENT, "job spp must be positive");
+    if (job->max_verts < 2 || job->max_verts > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "max_verts must be in [2, 64]");
+    if (job->sampling_mode < 0 || job->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
+    if (d.num_area_lights < 0 || (d.num_area_lights > 0 && !d.light_prims))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad light list");
+    lmgpu::BatchJob J;
+    const int64_t N = d.num_tris;
+    const int P = d.num_prims;
+    J.verts.assign(d.verts, d.verts + 9 * N);
+    J.normals.assign(d.normals, d.normals + 9 * N);
+    J.tri_prim.assign(d.tri_prim, d.tri_prim + N);
+    J.mat_type.assign(d.mat_type, d.mat_type + P);
+    J.mat_param.assign(d.mat_param, d.mat_param + 8 * P);
+    if (d.prim_Ke) J.prim_Ke.assign(d.prim_Ke, d.prim_Ke + 3 * P);
+    else J.prim_Ke.assign(3 * P, 0.f);
+    if (d.num_area_lights > 0) J.light_prims.assign(d.light_prims, d.light_prims + d.num_area_lights);
+    J.desc = d;
+    J.spp = job->spp;
+    J.max_verts = job->max_verts;
+    J.mode = job->sampling_mode;
+    J.rr_depth = job->rr_depth;
+    J.seed = job->seed;
+    b->b.jobs.push_back(std::move(J));
+    // point the stored descriptor at the host copies (after the move)
+    lmgpu::BatchJob& S = b->b.jobs.back();
+    S.desc.verts = S.verts.data();
+    S.desc.normals = S.normals.data();
+    S.desc.tri_prim = S.tri_prim.data();
+    S.desc.mat_type = S.mat_type.data();
+    S.desc.mat_param = S.mat_param.data();
+    S.desc.prim_Ke = S.prim_Ke.data();
+    S.desc.light_prims = S.light_prims.empty() ? nullptr : S.light_prims.data();
+    b->b.arena_width = 0;   // the arena must be rebuilt
+    b->b.have_results = false;
+    if (job_id) *job_id = int32_t(b->b.jobs.size() - 1);
+    return LMGPU_OK;
+}
+
+int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_stats* stats, void* stream) {
+    if (!bh) return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch is NULL");
+    lmgpu::clear_error();
+    lmgpu::Batch& b = bh->b;
+    if (b.jobs.empty()) return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch has no jobs");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    lmgpu_batch_opts o{};
+    if (opts) o = *opts;
+    if (o.width != 0 && o.width != 2 && o.width != 4 && o.width != 8)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "width must be 0 (autotune), 2, 4 or 8");
+    if (o.path_capacity != 0 && (o.path_capacity < 1024 || o.path_capacity > (1 << 24)))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "path_capacity must be 0 (autotune) or in [1024, 2^24]");
+    if (o.block != 0 && o.block != 128 && o.block != 256)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "block must be 0 (autotune), 128 or 256");
+    const int leaf = o.leaf_size > 0 ? o.leaf_size : 2;
+    if (leaf > 15) return fail(LMGPU_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 15]");
+    double build_s = 0, tune_s = 0;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    int width = o.width, capv = o.path_capacity, block = o.block;
+    if (!width || !capv || !block) {
+        if (b.tuned_width && (!o.width || o.width == b.tuned_width)) {
+            width = width ? width : b.tuned_width;
+            capv = capv ? capv : b.tuned_cap;
+            block = block ? block : b.tuned_block;
+        } else {
+            // coordinate search on one sample per pixel of every job
+            const auto t0 = now();
+            double tb = 0;
+            auto trial = [&](int w, int c, int bl, double* sec) -> int {
+                const auto tb0 = now();
+                LMGPU_TRY(b.build_arena(w, leaf, s));
+                tb += std::chrono::duration<double>(now() - tb0).count();
+                LMGPU_TRY(b.ensure_pool(c));
+                int64_t it;
+                return b.run_once(w, bl, 1, false, sec, &it, s);
+            };
+            const int widths[3] = {4, 8, 2}, caps[3] = {1 << 20, 1 << 19, 1 << 21}, blocks[2] = {256, 128};
+            int bw = width ? width : 4, bc = capv ? capv : (1 << 20), bb = block ? block : 256;
+            double best = 1e30, sec;
+            if (!width)
+                for (int w : widths) {
+                    LMGPU_TRY(trial(w, bc, bb, &sec));
+                    if (sec < best) { best = sec; bw = w; }
+                }
+            if (!capv)
+                for (int c : caps) {
+                    LMGPU_TRY(trial(bw, c, bb, &sec));
+                    if (sec < best) { best = sec; bc = c; }
+                }
+            if (!block)
+                for (int bl : blocks) {
+                    LMGPU_TRY(trial(bw, bc, bl, &sec));
+                    if (sec < best) { best = sec; bb = bl; }
+                }
+            width = bw;
+            capv = bc;
+            block = bb;
+            b.tuned_width = bw;
+            b.tuned_cap = bc;
+            b.tuned_block = bb;
+            build_s += tb;
+            tune_s = std::chrono::duration<double>(now() - t0).count() - tb;
+        }
+    }
+    const auto tb0 = now();
+    LMGPU_TRY(b.build_arena(width, leaf, s));
+    build_s += std::chrono::duration<double>(now() - tb0).count();
+    LMGPU_TRY(b.ensure_pool(capv));
+    double sec = 0;
+    int64_t iters = 0;
+    LMGPU_TRY(b.run_once(width, block, 0, true, &sec, &iters, s));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        lmgpu::BatchCtl hc;
+        LMGPU_HIP(hipMemcpy(&hc, b.ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        int64_t samples = 0;
+        for (const auto& J : b.jobs) samples += int64_t(J.desc.w) * J.desc.h * J.spp;
+        stats->jobs = int64_t(b.jobs.size());
+        stats->samples = samples;
+        stats->extension_rays = int64_t(hc.ext_rays);
+        stats->shadow_rays = int64_t(hc.shadow_rays);
+        stats->iterations = iters;
+        stats->elapsed_s = sec;
+        stats->build_s = build_s;
+        stats->tune_s = tune_s;
+        stats->width = width;
+        stats->path_capacity = capv;
+        stats->block = block;
+    }
+    return LMGPU_OK;
+}
+
+int lmgpu_batch_job_film(const lmgpu_batch* bh, int32_t job, double* film_sum, double* film_sumsq,
+                         double* latency_ms) {
+    if (!bh) return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch is NULL");
+    const lmgpu::Batch& b = bh->b;
+    if (job < 0 || size_t(job) >= b.jobs.size()) return fail(LMGPU_ERR_INVALID_ARGUMENT, "job index out of range");
+    if (!b.have_results) return fail(LMGPU_ERR_INVALID_ARGUMENT, "the batch has not been run since its last change");
+    const lmgpu::JobDev& J = b.jobdev[size_t(job)];
+    const size_t n = size_t(J.npix) * 3;
+    if (film_sum) LMGPU_HIP(hipMemcpy(film_sum, b.film.p + J.film_off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (film_sumsq)
+        LMGPU_HIP(hipMemcpy(film_sumsq, b.film_sq.p + J.film_off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (latency_ms) *latency_ms = b.latency_ms[size_t(job)];
+    return LMGPU_OK;
+}
+
+}  // extern "C"

@@ -296,8 +296,9 @@ __device__ __forceinline__ TraceResult trace(const WNode<W>* __restrict__ nodes,
 // slowest ray. refill = 64 keeps 64 consecutive rays per wave (best for
 // coherent rays), small values keep lanes busy (best for incoherent rays).
 // Each loop iteration advances every active lane by one step: one wide node
-// or one leaf. `fetch(id, o, d, tmin, tmax)` loads ray `id`; `emit(id, res)`
-// consumes its result.
+// or one leaf. `fetch(id, o, d, tmin, tmax)` loads ray `id` and returns the
+// node its traversal starts from (0, or a root inside a multi-BVH arena);
+// `emit(id, res)` consumes its result.
 template <int W, bool ANY, bool COUNT, int LDS_DEPTH, int PRIV_DEPTH, int BLOCK, class Fetch, class Emit>
 __device__ __forceinline__ void trace_persistent(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
                                                  int64_t n, unsigned long long* next, int (*lds)[BLOCK],
@@ -332,14 +333,13 @@ __device__ __forceinline__ void trace_persistent(const WNode<W>* __restrict__ no
                     rid = id;
                     f3 o, d;
                     float tmax;
-                    fetch(id, o, d, tmin, tmax);
+                    cur = fetch(id, o, d, tmin, tmax);
                     r = ray_prepare(o, d);
                     res.prim = -1;
                     res.t = tmax;
                     res.u = res.v = 0.f;
                     res.best = -1;
                     res.overflow = false;
-                    cur = 0;
                     st.sp = 0;
                 } else {
                     exhausted = true;

@@ -32,143 +32,22 @@
 #include <string>
 #include <vector>
 
-#include "bvh.h"
-#include "guide.h"
 #include "guide_internal.h"
 #include "lmgpu_internal.h"
-#include "shade.h"
+#include "pt_device.h"
+#include "pt_internal.h"
 
 namespace lmgpu {
-
-// Triangle shading record, original (flattened) order: positions, vertex
-// normals; p0.w = primitive index bits.
-struct alignas(16) ShadeTri {
-    float4 p0, p1, p2, n0, n1, n2;
-};
-
-struct alignas(16) DevMat {
-    float4 a;   // rgb, type (as float)
-    float4 b;   // ax, ay, ni, light index (as float, -1 = none)
-};
-
-struct alignas(16) DevLight {
-    float4 Ke_invA;   // Ke rgb, 1/area
-    int32_t cdf_off, count, pad0, pad1;
-};
-
-struct SceneDev {
-    const WNode<4>* nodes4;
-    const WNode<2>* nodes2;
-    const WNode<8>* nodes8;
-    const TriV* tris;
-    const ShadeTri* stri;
-    const DevMat* mats;
-    const DevLight* lights;
-    const float* light_cdf;    // per light: count+1 entries, c[0] = 0, normalised
-    const int32_t* light_tri;  // per light: original triangle indices (face order)
-    int num_area_lights;
-    int num_lights;            // area + env
-    int has_env;
-    f3 env_Le;
-    Camera cam;
-    int w, h;
-};
-
-// Path state, SoA over pixels.
-struct PathState {
-    float4* thr;    // throughput xyz
-    float4* L;      // radiance xyz
-    float4* prev;   // shading normal of the previous vertex xyz, pdf_bsdf (projected SA) w
-    int4* info;     // nv, comp, flags (bit0 prev samplable by NEE, bit1 prev degenerate), -
-};
-
-struct Queues {
-    lmgpu_ray* ext[2];
-    lmgpu_hit* hit;
-    int32_t* ext_path[2];
-    lmgpu_ray* sh;
-    float4* sh_contrib;   // xyz contribution, w = path index bits
-    int32_t* counts;      // [0],[1] extension queues, [2] shadow queue, [3] sample index, [4..] stats
-};
-
-struct PTConsts {
-    int max_verts;
-    int mode;
-    int rr_depth;
-    uint32_t seed_lo, seed_hi;
-    int npix;
-};
-
-// Training record of one guided vertex of one path (slot v * npix + p for
-// the v-th surface vertex of path p).
-struct alignas(16) GRec {
-    float4 a;   // position, mixture pdf of the sampled direction (solid angle; 0 = empty slot)
-    float4 b;   // shading normal
-    float4 c;   // direction toward the previous vertex
-    float4 d;   // sampled direction
-    float4 e;   // path radiance when the direction was sampled, plus this vertex's NEE
-    float4 f;   // per channel: mixture pdf / throughput before the direction was sampled
-};
-
-struct GuideDev {
-    const float* mix;   // decoded mixture per extension-queue entry of the bounce [npix][40]
-    GRec* rec;          // [rec_depth][npix]
-    int32_t* sh_rec;    // record slot of each shadow-queue entry (-1: none)
-    float alpha;        // probability of sampling the guide at a non-specular vertex
-    int rec_depth;      // surface vertices per path that record (0: no recording)
-};
-
-struct GBox {
-    float lo[3], inv[3];
-};
-
-constexpr int kBlock = 256;
-constexpr int kLds = 24;
-constexpr int kPriv = 72;
-
-__device__ __forceinline__ Mat load_mat(const DevMat* mats, int prim, int& light) {
-    const DevMat d = mats[prim];
-    Mat m;
-    m.type = int(d.a.w);
-    m.c = mk(d.a.x, d.a.y, d.a.z);
-    m.ax = d.b.x;
-    m.ay = d.b.y;
-    m.ni = d.b.z;
-    light = int(d.b.w);
-    return m;
-}
-
-// Per-wave append: returns this lane's slot (lanes with want = false get -1).
-__device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0) return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll(mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned long long lower = mask & ((1ull << lane) - 1ull);
-    return want ? base + __popcll(lower) : -1;
-}
 
 __global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathState ps, Queues q, PTConsts c) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= c.npix) return;
     const uint32_t sample = uint32_t(q.counts[3]);
-    const int x = p % sc.w, y = p / sc.w;
-    const rand4 r = rng(uint32_t(p), sample, 0u, 0u, (uint64_t(c.seed_hi) << 32) | c.seed_lo);
-    const float dx = 1.f / sc.w, dy = 1.f / sc.h;
-    const f3 d = primary_dir(sc.cam, dx * x + dx * r.u[2], dy * y + dy * r.u[3]);
     ps.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.prev[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.info[p] = make_int4(1, 0, 2, 0);   // nv = 1, comp 0, prev degenerate (camera), not NEE-samplable
-    lmgpu_ray ray;
-    ray.org[0] = sc.cam.pos.x; ray.org[1] = sc.cam.pos.y; ray.org[2] = sc.cam.pos.z;
-    ray.tmin = kEps;
-    ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
-    ray.tmax = kInf;
-    q.ext[0][p] = ray;
+    q.ext[0][p] = camera_ray(sc, p, sample, (uint64_t(c.seed_hi) << 32) | c.seed_lo);
     q.ext_path[0][p] = p;
     if (p == 0) {
         q.counts[0] = c.npix;
@@ -183,27 +62,20 @@ __global__ void k_reset_counts(int32_t* counts, int out_q) {
     }
 }
 
-template <int W>
-__device__ __forceinline__ const WNode<W>* nodes_of(const SceneDev& sc);
-template <>
-__device__ __forceinline__ const WNode<2>* nodes_of<2>(const SceneDev& sc) { return sc.nodes2; }
-template <>
-__device__ __forceinline__ const WNode<4>* nodes_of<4>(const SceneDev& sc) { return sc.nodes4; }
-template <>
-__device__ __forceinline__ const WNode<8>* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[in_q];
     const lmgpu_ray* rays = q.ext[in_q];
-    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(rays + id);
         const float4 a = rp[0], b = rp[1];
         o = mk(a.x, a.y, a.z);
         d = mk(b.x, b.y, b.z);
         tmin = a.w;
         tmax = b.w;
+        return 0;
     };
     auto emit = [&](int64_t id, const TraceResult& r) {
         reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
@@ -223,13 +95,14 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Q
                                                    int refill, GRec* __restrict__ rec, const int32_t* __restrict__ sh_rec) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
-    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
         const float4 a = rp[0], b = rp[1];
         o = mk(a.x, a.y, a.z);
         d = mk(b.x, b.y, b.z);
         tmin = a.w;
         tmax = b.w;
+        return 0;
     };
     auto emit = [&](int64_t id, const TraceResult& r) {
         if (r.prim < 0 && !r.overflow) {
@@ -262,32 +135,9 @@ __global__ void k_bounce_begin(unsigned long long* next) {
     if (threadIdx.x < 2) next[threadIdx.x] = 0ull;
 }
 
-// pdf of light endpoint `light` seen from the previous vertex (prev normal
-// np, degenerate flag) along d (unit, from the previous vertex to the light),
-// including the uniform selection probability (path::pdf_direct, path.h:563-583).
-__device__ __forceinline__ float pdf_light_area(const SceneDev& sc, int light, f3 np, bool prev_degen, f3 d,
-                                                float dist2, f3 nL) {
-    const float psel = 1.f / sc.num_lights;
-    const float cL = dot(-d, nL);
-    if (cL <= 0.f) return 0.f;
-    const float c1 = prev_degen ? 1.f : fabsf(dot(np, d));
-    const float G = c1 * fabsf(cL) / dist2;
-    if (G == 0.f) return 0.f;
-    return psel * sc.lights[light].Ke_invA.w / G;
-}
 
-__device__ __forceinline__ float pdf_light_env(const SceneDev& sc, f3 np, bool prev_degen, f3 d) {
-    const float psel = 1.f / sc.num_lights;
-    const float pSA = 1.f / (4.f * kPi);
-    if (prev_degen) return psel * pSA;
-    const float J = fabsf(dot(np, d));
-    return J == 0.f ? 0.f : psel * pSA / J;
-}
-
-// Guided vertices (GUIDED, diffuse or glossy) sample the next direction from
-// the one-sample MIS mixture alpha * guide + (1 - alpha) * BSDF, whose
-// density (solid angle) p_mix enters both the throughput f |cos| / p_mix and,
-// as p_mix / |cos|, every MIS weight against light sampling.
+// One bounce of shading over the current extension queue (shade_vertex per
+// entry), appending the next extension rays and the shadow rays.
 template <bool GUIDED>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Queues q, PTConsts c, int in_q, GuideDev gd) {
     const int n = q.counts[in_q];
@@ -300,241 +150,35 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
     const int n_iter = (n + stride - 1) / stride;
     for (int it = 0; it < n_iter; it++) {
         const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
-        bool active = i < n;
-        bool emit_ext = false, emit_sh = false;
-        lmgpu_ray ext_ray, sh_ray;
-        float4 sh_c = make_float4(0, 0, 0, 0);
-        int p = 0, sh_slot = -1;
-        if (active) {
+        VertexOut out;
+        int p = 0;
+        if (i < n) {
             p = q.ext_path[in_q][i];
             const float4* rp = reinterpret_cast<const float4*>(q.ext[in_q] + i);
             const float4 ra = rp[0], rb = rp[1];
-            const f3 o = mk(ra.x, ra.y, ra.z), wo = mk(rb.x, rb.y, rb.z);
             const float4 h = reinterpret_cast<const float4*>(q.hit)[i];
-            const int tri = __float_as_int(h.x);
-            float4 thr4 = ps.thr[p];
-            float4 L4 = ps.L[p];
-            const float4 pv = ps.prev[p];
-            int4 info = ps.info[p];
-            f3 thr = mk(thr4.x, thr4.y, thr4.z);
-            f3 L = mk(L4.x, L4.y, L4.z);
-            const f3 np = mk(pv.x, pv.y, pv.z);
-            const bool prev_nee = info.z & 1;
-            const bool prev_degen = info.z & 2;
-            const bool direct_ok = c.mode == LMGPU_SAMPLING_NEE ? !prev_nee : true;
-            const bool use_mis = c.mode != LMGPU_SAMPLING_NAIVE && prev_nee;
-            if (tri == -2) atomicOr(q.counts + 5, 1);
-            if (tri < 0) {
-                // miss: environment light endpoint (Scene_::intersect, scene.cpp:321-332)
-                if (sc.has_env && direct_ok) {
-                    float mis = 1.f;
-                    if (use_mis) mis = balance_heuristic(pv.w, pdf_light_env(sc, np, prev_degen, wo));
-                    L = L + thr * sc.env_Le * mis;
-                }
-            } else {
-                const ShadeTri st = sc.stri[tri];
-                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
-                const f3 P0 = mk(st.p0.x, st.p0.y, st.p0.z), P1 = mk(st.p1.x, st.p1.y, st.p1.z),
-                         P2 = mk(st.p2.x, st.p2.y, st.p2.z);
-                const f3 pos = P0 * b0 + P1 * b1 + P2 * b2;
-                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
-                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
-                const f3 gn = normalize(cross(P1 - P0, P2 - P0));
-                const Geom g = make_on_surface(pos, nsh, gn);
-                const int prim = __float_as_int(st.p0.w);
-                int light;
-                const Mat m = load_mat(sc.mats, prim, light);
-                // emitter hit (renderer_pt.cpp:246-271)
-                if (light >= 0 && direct_ok) {
-                    const f3 woL = -wo;
-                    if (dot(woL, g.n) > 0.f) {
-                        const DevLight dl = sc.lights[light];
-                        float mis = 1.f;
-                        if (use_mis) {
-                            const f3 dd = pos - o;
-                            const float dist2 = dot(dd, dd);
-                            mis = balance_heuristic(pv.w, pdf_light_area(sc, light, np, prev_degen, wo, dist2, g.n));
-                        }
-                        L = L + thr * mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) * mis;
-                    }
-                }
-                // Russian roulette and component sampling at this vertex
-                const int nv = info.x;
-                const rand4 r0 = rng(uint32_t(p), sample, uint32_t(nv), 0u, seed);
-                bool alive = true;
-                if (nv > c.rr_depth) {
-                    const float qq = fmaxf(.2f, 1.f - maxc(thr));
-                    if (r0.u[0] < qq) alive = false;
-                    else thr = thr / (1.f - qq);
-                }
-                const f3 wi = -wo;
-                int comp = 0;
-                if (alive) {
-                    float cw;
-                    comp = mat_sample_component(m, g, wi, r0.u[1], cw);
-                    thr = thr * cw;
-                }
-                const int nv2 = nv + 1;
-                if (alive && nv2 < c.max_verts) {
-                    const bool spec = is_specular(m);
-                    const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
-                    // guide mixture of this queue entry (guided vertices only)
-                    const float* gmix = (GUIDED && !spec) ? gd.mix + int64_t(i) * kOut : nullptr;
-                    const int slot = (gmix && nv - 1 < gd.rec_depth) ? (nv - 1) * c.npix + p : -1;
-                    if (nee) {
-                        // path::sample_direct (path.h:509-528)
-                        const rand4 r1 = rng(uint32_t(p), sample, uint32_t(nv), 1u, seed);
-                        const int li = min(max(int(r1.u[0] * sc.num_lights), 0), sc.num_lights - 1);
-                        const float psel = 1.f / sc.num_lights;
-                        f3 dirL;   // from the shading point toward the light
-                        float tmaxL = 0.f;
-                        f3 weight = mk(0, 0, 0);
-                        float p_light = 0.f;
-                        bool ok = false;
-                        if (li < sc.num_area_lights) {
-                            const DevLight dl = sc.lights[li];
-                            // Dist::sample: upper_bound on the CDF
-                            const float* cdf = sc.light_cdf + dl.cdf_off;
-                            int lo = 0, hi = dl.count + 1;
-                            const float u = r1.u[1];
-                            while (lo < hi) {
-                                const int mid = (lo + hi) >> 1;
-                                if (cdf[mid] <= u) lo = mid + 1;
-                                else hi = mid;
-                            }
-                            const int k = min(max(lo - 1, 0), dl.count - 1);
-                            const int lt = sc.light_tri[dl.cdf_off - li + k];
-                            const ShadeTri ls = sc.stri[lt];
-                            const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z),
-                                     C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
-                            const float s = sqrtf(fmaxf(0.f, r1.u[2]));
-                            const float bu = 1.f - s, bv = r1.u[3] * s;
-                            const f3 pL = A * (1.f - bu - bv) + B * bu + C * bv;
-                            const f3 nL = normalize(cross(B - A, C - A));
-                            const f3 wL = normalize(pos - pL);   // from the light to the point
-                            if (dot(wL, nL) > 0.f) {
-                                const f3 dd = pL - pos;
-                                const float d2 = dot(dd, dd);
-                                const float G = fabsf(dot(g.n, wL)) * fabsf(dot(nL, -wL)) / d2;
-                                if (G != 0.f) {
-                                    const float pA = dl.Ke_invA.w / G;
-                                    weight = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) / pA / psel;
-                                    p_light = psel * pA;
-                                    dirL = normalize(pL - pos);
-                                    tmaxL = sqrtf(d2) * (1.f - kEps);
-                                    ok = true;
-                                }
-                            }
-                        } else {
-                            const f3 wL = sample_uniform_sphere(r1.u[2], r1.u[3]);
-                            const float J = fabsf(dot(g.n, -wL));
-                            if (J != 0.f) {
-                                const float pL = (1.f / (4.f * kPi)) / J;
-                                weight = sc.env_Le / pL / psel;
-                                p_light = psel * pL;
-                                dirL = -wL;
-                                tmaxL = kInf - 1.f;
-                                ok = true;
-                            }
-                        }
-                        if (ok) {
-                            const f3 fs = mat_eval(m, g, wi, dirL) * sn_correction(g, wi, dirL);
-                            if (!is_zero(fs)) {
-                                float mis = 1.f;
-                                if (c.mode == LMGPU_SAMPLING_MIS) {
-                                    float pdir = mat_pdf(m, g, wi, dirL);
-                                    if (GUIDED && gmix) {
-                                        float pg = mixture_pdf(gmix, dirL);
-                                        if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
-                                        pdir = gd.alpha * pg / fabsf(dot(dirL, g.n)) + (1.f - gd.alpha) * pdir;
-                                    }
-                                    mis = balance_heuristic(p_light, pdir);
-                                }
-                                const f3 C = thr * fs * weight * mis;
-                                if (!is_zero(C)) {
-                                    emit_sh = true;
-                                    sh_slot = slot;
-                                    sh_ray.org[0] = pos.x; sh_ray.org[1] = pos.y; sh_ray.org[2] = pos.z;
-                                    sh_ray.tmin = kEps;
-                                    sh_ray.dir[0] = dirL.x; sh_ray.dir[1] = dirL.y; sh_ray.dir[2] = dirL.z;
-                                    sh_ray.tmax = tmaxL;
-                                    sh_c = make_float4(C.x, C.y, C.z, __int_as_float(p));
-                                }
-                            }
-                        }
-                    }
-                    // path::sample_direction (path.h:410-419)
-                    f3 wo2, sw;
-                    bool sampled;
-                    float pdir = 0.f, pmix = 0.f;
-                    if (GUIDED && gmix) {
-                        const rand4 r2 = rng(uint32_t(p), sample, uint32_t(nv), 2u, seed);
-                        if (r2.u[0] < gd.alpha) {
-                            wo2 = mixture_sample(gmix, r2.u[1], r2.u[2], r2.u[3]);
-                            sampled = true;
-                        } else {
-                            sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
-                        }
-                        sw = mk(0, 0, 0);
-                        if (sampled) {
-                            const float cs = fabsf(dot(wo2, g.n));
-                            float pg = mixture_pdf(gmix, wo2);
-                            if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
-                            pmix = gd.alpha * pg + (1.f - gd.alpha) * mat_pdf(m, g, wi, wo2) * cs;
-                            if (pmix > 0.f && cs > 0.f) {
-                                sw = mat_eval(m, g, wi, wo2) * (cs / pmix);
-                                pdir = pmix / cs;
-                            }
-                        }
-                    } else {
-                        sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
-                        if (sampled) pdir = mat_pdf(m, g, wi, wo2);
-                    }
-                    if (sampled) {
-                        sw = sw * sn_correction(g, wi, wo2);
-                        const f3 thr_before = thr;
-                        thr = thr * sw;
-                        if (!is_zero(thr)) {
-                            emit_ext = true;
-                            ext_ray.org[0] = pos.x; ext_ray.org[1] = pos.y; ext_ray.org[2] = pos.z;
-                            ext_ray.tmin = kEps;
-                            ext_ray.dir[0] = wo2.x; ext_ray.dir[1] = wo2.y; ext_ray.dir[2] = wo2.z;
-                            ext_ray.tmax = kInf;
-                            ps.prev[p] = make_float4(g.n.x, g.n.y, g.n.z, pdir);
-                            info = make_int4(nv2, comp, nee ? 1 : 0, 0);
-                            ps.info[p] = info;
-                            ps.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.f);
-                            if (GUIDED && slot >= 0) {
-                                // incident radiance along wo2 is (L_final - e) / throughput after this
-                                // vertex; the record keeps p_mix / throughput before it, so that
-                                // (L_final - e) * f is the integrand f |cos| L_i estimate
-                                GRec r;
-                                r.a = make_float4(pos.x, pos.y, pos.z, pmix);
-                                r.b = make_float4(g.n.x, g.n.y, g.n.z, 0.f);
-                                r.c = make_float4(wi.x, wi.y, wi.z, 0.f);
-                                r.d = make_float4(wo2.x, wo2.y, wo2.z, 0.f);
-                                r.e = make_float4(L.x, L.y, L.z, 0.f);
-                                r.f = make_float4(thr_before.x > 0.f ? pmix / thr_before.x : 0.f,
-                                                  thr_before.y > 0.f ? pmix / thr_before.y : 0.f,
-                                                  thr_before.z > 0.f ? pmix / thr_before.z : 0.f, 0.f);
-                                gd.rec[slot] = r;
-                            }
-                        }
-                    }
-                }
+            if (__float_as_int(h.x) == -2) atomicOr(q.counts + 5, 1);
+            const float4 thr4 = ps.thr[p], L4 = ps.L[p];
+            PathRegs P{mk(thr4.x, thr4.y, thr4.z), mk(L4.x, L4.y, L4.z), ps.prev[p], ps.info[p]};
+            shade_vertex<GUIDED>(sc, c, uint32_t(p), sample, seed, p, mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), h,
+                                 P, gd, GUIDED ? gd.mix + int64_t(i) * kOut : nullptr, out);
+            if (out.ext) {
+                ps.prev[p] = P.prev;
+                ps.info[p] = P.info;
+                ps.thr[p] = make_float4(P.thr.x, P.thr.y, P.thr.z, 0.f);
             }
-            ps.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+            ps.L[p] = make_float4(P.L.x, P.L.y, P.L.z, 0.f);
         }
-        const int se = wave_append(q.counts + out_q, emit_ext);
-        if (emit_ext) {
-            q.ext[out_q][se] = ext_ray;
+        const int se = wave_append(q.counts + out_q, out.ext);
+        if (out.ext) {
+            q.ext[out_q][se] = out.ext_ray;
             q.ext_path[out_q][se] = p;
         }
-        const int ss = wave_append(q.counts + 2, emit_sh);
-        if (emit_sh) {
-            q.sh[ss] = sh_ray;
-            q.sh_contrib[ss] = sh_c;
-            if (GUIDED) gd.sh_rec[ss] = sh_slot;
+        const int ss = wave_append(q.counts + 2, out.sh);
+        if (out.sh) {
+            q.sh[ss] = out.sh_ray;
+            q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(p));
+            if (GUIDED) gd.sh_rec[ss] = out.sh_slot;
         }
     }
 }
@@ -665,11 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_raycast(SceneDev sc, unsigned long l
         const int x = int(id % sc.w), y = int(id / sc.w);
         return primary_dir(sc.cam, (x + .5f) / sc.w, (y + .5f) / sc.h);
     };
-    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) {
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         o = sc.cam.pos;
         d = dir_of(id);
         tmin = kEps;
         tmax = kInf;
+        return 0;
     };
     auto emit = [&](int64_t id, const TraceResult& r) {
         f3 C = rc.bg;
@@ -741,61 +386,6 @@ __global__ void k_zero_f64(double* p, int64_t n) {
 // ---------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------
-
-// Guided rendering of one render call (opts resolved, workspace sized).
-struct GuidedPass {
-    Guide* g = nullptr;
-    uint64_t uid = 0;   // guide identity and buffer generation (graph key)
-    int gen = 0;
-    float alpha = 0.5f, lr = 1e-3f;
-    int batch = 0, steps = 0, rec_depth = 0;
-    int64_t mul = 1;   // slot visiting order of k_guide_collect
-};
-
-struct Scene {
-    Accel accel;
-    DevBuf<ShadeTri> stri;
-    DevBuf<DevMat> mats;
-    DevBuf<DevLight> lights;
-    DevBuf<float> light_cdf;
-    DevBuf<int32_t> light_tri;
-    SceneDev dev{};
-    int w = 0, h = 0;
-    // render workspace
-    int ws_npix = 0;
-    DevBuf<float4> thr, L, prev, shc;
-    DevBuf<int4> info;
-    DevBuf<lmgpu_ray> ext0, ext1, sh;
-    DevBuf<lmgpu_hit> hit;
-    DevBuf<int32_t> path0, path1, counts;
-    DevBuf<unsigned long long> next;   // per-bounce ray counters of the persistent trace kernels
-    DevBuf<double> fsum, fsq;   // device film when the caller passes host pointers
-    // guided rendering workspace
-    DevBuf<float> gmix, gtrain;
-    DevBuf<GRec> grec;
-    DevBuf<int32_t> gshrec, gcount;
-    int grec_slots = 0;
-    hipGraphExec_t graph = nullptr;
-    int refill_primary = 64, refill_secondary = 16;
-    int graph_key[4] = {0, 0, 0, 0};
-    GuidedPass graph_guide;
-    uint64_t graph_seed = 0;
-    int graph_sumsq = -1;
-    double* graph_sum_ptr = nullptr;
-    double* graph_sq_ptr = nullptr;
-
-    ~Scene() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-    }
-
-    int create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts);
-    int ensure_workspace(int npix);
-    int ensure_guided(int npix, const GuidedPass& gp);
-    int enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const GuidedPass* gp, hipStream_t s);
-    int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
-               lmgpu_pt_stats* st, hipStream_t s);
-    int raycast(const lmgpu_raycast_params* p, double* film, bool on_device, hipStream_t s);
-};
 
 static inline float f_or(const float* p, int i) { return p ? p[i] : 0.f; }
 

@@ -40,6 +40,7 @@ EXPORTED = [
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
     'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
     'lmgpu_raycast_render',
+    'lmgpu_batch_create', 'lmgpu_batch_destroy', 'lmgpu_batch_add_job', 'lmgpu_batch_run', 'lmgpu_batch_job_film',
 ]
 
 
@@ -115,6 +116,25 @@ class RaycastParams(ctypes.Structure):
                 ('reserved', ctypes.c_int32 * 5)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [('scene', ctypes.c_void_p), ('spp', ctypes.c_int32), ('max_verts', ctypes.c_int32),
+                ('sampling_mode', ctypes.c_int32), ('rr_depth', ctypes.c_int32), ('seed', ctypes.c_uint64),
+                ('reserved', ctypes.c_int32 * 4)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [('width', ctypes.c_int32), ('path_capacity', ctypes.c_int32), ('block', ctypes.c_int32),
+                ('leaf_size', ctypes.c_int32), ('reserved', ctypes.c_int32 * 4)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [('jobs', ctypes.c_int64), ('samples', ctypes.c_int64), ('extension_rays', ctypes.c_int64),
+                ('shadow_rays', ctypes.c_int64), ('iterations', ctypes.c_int64), ('elapsed_s', ctypes.c_double),
+                ('build_s', ctypes.c_double), ('tune_s', ctypes.c_double), ('width', ctypes.c_int32),
+                ('path_capacity', ctypes.c_int32), ('block', ctypes.c_int32), ('reserved0', ctypes.c_int32),
+                ('reserved', ctypes.c_int64 * 4)]
+
+
 class GuideOpts(ctypes.Structure):
     _fields_ = [('alpha', ctypes.c_float), ('learning_rate', ctypes.c_float), ('batch', ctypes.c_int32),
                 ('steps_per_pass', ctypes.c_int32), ('record_depth', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
@@ -167,6 +187,11 @@ def lib():
         L.lmgpu_guide_pdf.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_guide_stats.argtypes = [vp, ctypes.POINTER(i64), f32p]
         L.lmgpu_raycast_render.argtypes = [vp, ctypes.POINTER(RaycastParams), vp, ctypes.c_int, vp]
+        L.lmgpu_batch_create.argtypes = [ctypes.POINTER(vp)]
+        L.lmgpu_batch_destroy.argtypes = [vp]
+        L.lmgpu_batch_add_job.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int32)]
+        L.lmgpu_batch_run.argtypes = [vp, ctypes.POINTER(BatchOpts), ctypes.POINTER(BatchStats), vp]
+        L.lmgpu_batch_job_film.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.lmgpu_pt_render_guided.argtypes = [vp, ctypes.POINTER(PTParams), vp, ctypes.POINTER(GuideOpts), vp, vp,
                                              ctypes.c_int, ctypes.POINTER(PTStats), vp]
         L.lmgpu_guide_train.argtypes = [vp, vp, i64, ctypes.c_float, f32p, vp, vp]

@@ -70,6 +70,43 @@ class Film:
         self.splat_pixel(*self.raster_to_pixel(rp), v)
 
 
+def scene_desc(flat):
+    """lmgpu_scene_desc of a lm_amd.scenes.FlatScene; returns (keepalive
+    arrays, descriptor). The descriptor points into the arrays."""
+    keep = {
+        'verts': np.ascontiguousarray(flat.verts, dtype=np.float32),
+        'normals': np.ascontiguousarray(flat.normals, dtype=np.float32),
+        'tri_prim': np.ascontiguousarray(flat.tri_prim, dtype=np.int32),
+        'mat_type': np.ascontiguousarray(flat.mat_type, dtype=np.int32),
+        'mat_param': np.ascontiguousarray(flat.mat_param, dtype=np.float32),
+        'prim_Ke': np.ascontiguousarray(flat.prim_Ke, dtype=np.float32),
+        'light_prims': np.ascontiguousarray(flat.light_prims, dtype=np.int32),
+    }
+    k = keep
+    d = _lib.SceneDesc()
+    d.num_tris = flat.num_triangles
+    d.verts = k['verts'].ctypes.data
+    d.normals = k['normals'].ctypes.data
+    d.tri_prim = k['tri_prim'].ctypes.data
+    d.num_prims = int(k['mat_type'].shape[0])
+    d.mat_type = k['mat_type'].ctypes.data
+    d.mat_param = k['mat_param'].ctypes.data
+    d.prim_Ke = k['prim_Ke'].ctypes.data
+    d.num_area_lights = int(k['light_prims'].shape[0])
+    d.light_prims = k['light_prims'].ctypes.data if k['light_prims'].size else None
+    d.has_env = 1 if flat.has_env else 0
+    for i in range(3):
+        d.env_Le[i] = float(flat.env_Le[i])
+        d.cam_pos[i] = float(flat.camera.position[i])
+        d.cam_center[i] = float(flat.camera.center[i])
+        d.cam_up[i] = float(flat.camera.up[i])
+    d.vfov = float(flat.camera.vfov)
+    d.aspect = float(flat.camera.aspect)
+    d.w = int(flat.w)
+    d.h = int(flat.h)
+    return keep, d
+
+
 class GpuScene:
     """Device scene (geometry, accel, materials, lights, camera) built from a
     lm_amd.scenes.FlatScene through lmgpu_scene_create."""
@@ -77,37 +114,7 @@ class GpuScene:
     def __init__(self, flat, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1,
                  radius: int = 16):
         self.flat = flat
-        self._keep = {
-            'verts': np.ascontiguousarray(flat.verts, dtype=np.float32),
-            'normals': np.ascontiguousarray(flat.normals, dtype=np.float32),
-            'tri_prim': np.ascontiguousarray(flat.tri_prim, dtype=np.int32),
-            'mat_type': np.ascontiguousarray(flat.mat_type, dtype=np.int32),
-            'mat_param': np.ascontiguousarray(flat.mat_param, dtype=np.float32),
-            'prim_Ke': np.ascontiguousarray(flat.prim_Ke, dtype=np.float32),
-            'light_prims': np.ascontiguousarray(flat.light_prims, dtype=np.int32),
-        }
-        k = self._keep
-        d = _lib.SceneDesc()
-        d.num_tris = flat.num_triangles
-        d.verts = k['verts'].ctypes.data
-        d.normals = k['normals'].ctypes.data
-        d.tri_prim = k['tri_prim'].ctypes.data
-        d.num_prims = int(k['mat_type'].shape[0])
-        d.mat_type = k['mat_type'].ctypes.data
-        d.mat_param = k['mat_param'].ctypes.data
-        d.prim_Ke = k['prim_Ke'].ctypes.data
-        d.num_area_lights = int(k['light_prims'].shape[0])
-        d.light_prims = k['light_prims'].ctypes.data if k['light_prims'].size else None
-        d.has_env = 1 if flat.has_env else 0
-        for i in range(3):
-            d.env_Le[i] = float(flat.env_Le[i])
-            d.cam_pos[i] = float(flat.camera.position[i])
-            d.cam_center[i] = float(flat.camera.center[i])
-            d.cam_up[i] = float(flat.camera.up[i])
-        d.vfov = float(flat.camera.vfov)
-        d.aspect = float(flat.camera.aspect)
-        d.w = int(flat.w)
-        d.h = int(flat.h)
+        self._keep, d = scene_desc(flat)
         opts = _lib.BuildOpts(width, leaf_size, refill, builder, radius)
         h = ctypes.c_void_p()
         _lib.call('lmgpu_scene_create', ctypes.byref(h), ctypes.byref(d), ctypes.byref(opts))
