@@ -252,7 +252,8 @@ typedef struct lmgpu_batch_opts {
     int32_t path_capacity;  /* paths in flight (queue capacity), 1024..2^24; 0 = autotune */
     int32_t block;          /* block size of the generate / shade kernels, 128 or 256; 0 = autotune */
     int32_t leaf_size;      /* 0 = 2 */
-    int32_t reserved[4];
+    int32_t film_sumsq;     /* 1: also accumulate per-pixel sums of squares (lmgpu_batch_job_film) */
+    int32_t reserved[3];
 } lmgpu_batch_opts;
 
 typedef struct lmgpu_batch_stats {
@@ -271,9 +272,10 @@ int lmgpu_batch_add_job(lmgpu_batch* batch, const lmgpu_job_desc* job, int32_t* 
 /* Renders every job (fresh films); autotunes the fields of opts left 0 on
  * the first run and reuses that choice afterwards. Synchronous. */
 int lmgpu_batch_run(lmgpu_batch* batch, const lmgpu_batch_opts* opts, lmgpu_batch_stats* stats, void* stream);
-/* FP64 sum / sum of squares ([h][w][3], host) of a job's samples in the
- * last run, and its latency: device time from the start of the run to the
- * accumulation of its last sample. */
+/* FP64 sum / sum of squares ([h][w][3], host; the latter needs
+ * film_sumsq in the run) of a job's samples in the last run, and its
+ * latency: device time from the start of the run to the accumulation of
+ * its last sample. Either pointer may be NULL. */
 int lmgpu_batch_job_film(const lmgpu_batch* batch, int32_t job, double* film_sum, double* film_sumsq,
                          double* latency_ms);
 

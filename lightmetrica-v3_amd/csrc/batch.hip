@@ -86,12 +86,14 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
     const int lane = __lane_id();
     for (int it = 0; it < n_iter; it++) {
         const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool want = false;
+        bool want = false, fin = false;
+        int fj = -1;
         if (s < cap) {
             int32_t st = P.state[s];
             if (st == kFinishing) {
-                const int j = P.job[s];
-                const JobDev& J = jobs[j];
+                fin = true;
+                fj = P.job[s];
+                const JobDev& J = jobs[fj];
                 const float4 L = P.L[s];
                 const int64_t f = J.film_off + 3 * int64_t(P.pixel[s]);
                 atomicAdd(film_sum + f + 0, double(L.x));
@@ -102,13 +104,31 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
                     atomicAdd(film_sq + f + 1, double(L.y) * double(L.y));
                     atomicAdd(film_sq + f + 2, double(L.z) * double(L.z));
                 }
-                const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
-                if (atomicAdd(job_done + j, 1ull) + 1ull == need) job_time[j] = wall_clock64();
-                atomicAdd(&ctl->remaining, ~0ull);   // - 1
                 st = kFree;
                 P.state[s] = kFree;
             }
             want = st == kFree;
+        }
+        // completion counters, one atomic per wave (remaining) and per job
+        // present in the wave (job_done); the job whose count reaches its
+        // sample total records the device clock
+        const unsigned long long fmask = __ballot(fin);
+        if (fmask) {
+            if (lane == __ffsll(fmask) - 1)
+                atomicAdd(&ctl->remaining, (unsigned long long)(-(long long)__popcll(fmask)));
+            unsigned long long todo = fmask;
+            while (todo) {
+                const int l0 = __ffsll(todo) - 1;
+                const int jj = __shfl(fj, l0);
+                const unsigned long long same = __ballot(fin && fj == jj);
+                if (lane == l0) {
+                    const JobDev& J = jobs[jj];
+                    const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
+                    const unsigned long long k = (unsigned long long)__popcll(same);
+                    if (atomicAdd(job_done + jj, k) + k == need) job_time[jj] = wall_clock64();
+                }
+                todo &= ~same;
+            }
         }
         const unsigned long long mask = __ballot(want);
         unsigned long long base = 0;
@@ -279,9 +299,12 @@ __global__ void k_b_init(BatchCtl* ctl, int64_t total, int32_t* counts) {
         ctl->remaining = (unsigned long long)total;
         ctl->ext_rays = 0;
         ctl->shadow_rays = 0;
-        ctl->t_start = wall_clock64();
     }
     if (threadIdx.x < 16) counts[threadIdx.x] = 0;
+}
+
+__global__ void k_b_clock(BatchCtl* ctl) {
+    if (threadIdx.x == 0) ctl->t_start = wall_clock64();
 }
 
 __global__ void k_b_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -368,8 +391,9 @@ struct Batch {
 
     int build_arena(int width, int leaf, hipStream_t s);
     int ensure_pool(int cap);
-    int run_once(int width, int block, int spp_cap, bool keep_results, double* elapsed, int64_t* iters,
-                 hipStream_t s);
+    bool sumsq = false;   // the last run accumulated sums of squares
+    int run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, double* elapsed,
+                 int64_t* iters, hipStream_t s);
 };
 
 int Batch::build_arena(int width, int leaf, hipStream_t s) {
@@ -491,8 +515,8 @@ int Batch::ensure_pool(int c) {
 
 // Renders every job once with the current arena and pool: at most spp_cap
 // samples per pixel per job when spp_cap > 0 (autotuning trials).
-int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double* elapsed, int64_t* iters,
-                    hipStream_t s) {
+int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, double* elapsed,
+                    int64_t* iters, hipStream_t s) {
     const int nj = int(jobs.size());
     int64_t total = 0, film_n = 0;
     std::vector<int64_t> work(nj);
@@ -529,7 +553,8 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_done.p, int64_t(nj));
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_time.p, int64_t(nj));
     hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film.p, film_n);
-    hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film_sq.p, film_n);
+    if (with_sumsq) hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film_sq.p, film_n);
+    double* fsq = with_sumsq ? film_sq.p : nullptr;
     hipLaunchKernelGGL(k_b_fill_i32, dim3(nblk(cap, 256)), dim3(256), 0, s, pstate.p, int64_t(cap), kFree);
     hipLaunchKernelGGL(k_b_init, dim3(1), dim3(64), 0, s, ctl.p, total, counts.p);
 
@@ -546,10 +571,10 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double
         const unsigned rg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)));
         if (block == 128)
             hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
-                               ctl.p, job_done.p, job_time.p, film.p, film_sq.p);
+                               ctl.p, job_done.p, job_time.p, film.p, fsq);
         else
             hipLaunchKernelGGL(k_b_regen<256>, dim3(rg), dim3(256), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
-                               ctl.p, job_done.p, job_time.p, film.p, film_sq.p);
+                               ctl.p, job_done.p, job_time.p, film.p, fsq);
         hipLaunchKernelGGL(k_b_zero2, dim3(1), dim3(64), 0, cs, next.p);
         if (width == 2)
             hipLaunchKernelGGL(k_b_extend<2>, dim3(occupancy_grid(k_b_extend<2>, kBlock)), dim3(kBlock), 0, cs,
@@ -592,6 +617,7 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double
     LMGPU_HIP(hipEventCreate(&e0));
     LMGPU_HIP(hipEventCreate(&e1));
     LMGPU_HIP(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_b_clock, dim3(1), dim3(64), 0, s, ctl.p);
     int64_t it = 0;
     int rc = LMGPU_OK;
     while (true) {
@@ -641,6 +667,7 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, double
         for (int j = 0; j < nj; j++)
             latency_ms[j] = khz > 0 ? double(t[j] - hc.t_start) / double(khz) : 0.0;
         have_results = true;
+        sumsq = with_sumsq;
     }
     return LMGPU_OK;
 }
@@ -746,7 +773,7 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
                 tb += std::chrono::duration<double>(now() - tb0).count();
                 LMGPU_TRY(b.ensure_pool(c));
                 int64_t it;
-                return b.run_once(w, bl, 1, false, sec, &it, s);
+                return b.run_once(w, bl, 1, false, o.film_sumsq != 0, sec, &it, s);
             };
             const int widths[3] = {4, 8, 2}, caps[3] = {1 << 20, 1 << 19, 1 << 21}, blocks[2] = {256, 128};
             int bw = width ? width : 4, bc = capv ? capv : (1 << 20), bb = block ? block : 256;
@@ -782,7 +809,7 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
     LMGPU_TRY(b.ensure_pool(capv));
     double sec = 0;
     int64_t iters = 0;
-    LMGPU_TRY(b.run_once(width, block, 0, true, &sec, &iters, s));
+    LMGPU_TRY(b.run_once(width, block, 0, true, o.film_sumsq != 0, &sec, &iters, s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         lmgpu::BatchCtl hc;
@@ -810,6 +837,7 @@ int lmgpu_batch_job_film(const lmgpu_batch* bh, int32_t job, double* film_sum, d
     const lmgpu::Batch& b = bh->b;
     if (job < 0 || size_t(job) >= b.jobs.size()) return fail(LMGPU_ERR_INVALID_ARGUMENT, "job index out of range");
     if (!b.have_results) return fail(LMGPU_ERR_INVALID_ARGUMENT, "the batch has not been run since its last change");
+    if (film_sumsq && !b.sumsq) return fail(LMGPU_ERR_INVALID_ARGUMENT, "the last run did not accumulate sums of squares");
     const lmgpu::JobDev& J = b.jobdev[size_t(job)];
     const size_t n = size_t(J.npix) * 3;
     if (film_sum) LMGPU_HIP(hipMemcpy(film_sum, b.film.p + J.film_off, sizeof(double) * n, hipMemcpyDeviceToHost));

@@ -124,7 +124,7 @@ class JobDesc(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [('width', ctypes.c_int32), ('path_capacity', ctypes.c_int32), ('block', ctypes.c_int32),
-                ('leaf_size', ctypes.c_int32), ('reserved', ctypes.c_int32 * 4)]
+                ('leaf_size', ctypes.c_int32), ('film_sumsq', ctypes.c_int32), ('reserved', ctypes.c_int32 * 3)]
 
 
 class BatchStats(ctypes.Structure):

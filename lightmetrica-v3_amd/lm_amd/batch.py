@@ -37,8 +37,10 @@ class Batch:
         self.sizes.append((int(flat.h), int(flat.w)))
         return jid.value
 
-    def run(self, width: int = 0, path_capacity: int = 0, block: int = 0, leaf_size: int = 0, stream=None) -> dict:
-        o = _lib.BatchOpts(int(width), int(path_capacity), int(block), int(leaf_size))
+    def run(self, width: int = 0, path_capacity: int = 0, block: int = 0, leaf_size: int = 0, sumsq: bool = False,
+            stream=None) -> dict:
+        o = _lib.BatchOpts(int(width), int(path_capacity), int(block), int(leaf_size), 1 if sumsq else 0)
+        self._sumsq = bool(sumsq)
         st = _lib.BatchStats()
         _lib.call('lmgpu_batch_run', self._h, ctypes.byref(o), ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
         return {f: getattr(st, f) for f, _ in _lib.BatchStats._fields_ if not f.startswith('reserved')}
@@ -50,11 +52,12 @@ class Batch:
         return lat.value
 
     def film(self, job: int):
-        """(sum [h,w,3], sumsq [h,w,3], latency_ms) of a job in the last run."""
+        """(sum [h,w,3], sumsq [h,w,3] or None, latency_ms) of a job in the
+        last run (sumsq when the run was made with sumsq=True)."""
         h, w = self.sizes[job]
         s = np.zeros((h, w, 3), dtype=np.float64)
-        q = np.zeros((h, w, 3), dtype=np.float64)
+        q = np.zeros((h, w, 3), dtype=np.float64) if getattr(self, '_sumsq', False) else None
         lat = ctypes.c_double(0.0)
         _lib.call('lmgpu_batch_job_film', self._h, int(job), s.ctypes.data_as(ctypes.c_void_p),
-                  q.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lat))
+                  q.ctypes.data_as(ctypes.c_void_p) if q is not None else None, ctypes.byref(lat))
         return s, q, lat.value

@@ -28,6 +28,7 @@ def main():
     ap.add_argument('--pt', action='store_true', help='also run one 1080p config-2 PT pass')
     ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
     ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
+    ap.add_argument('--c3', type=int, default=0, help='also render the first N config-3 jobs as one batch')
     args = ap.parse_args()
     import torch
     from lm_amd import scenes
@@ -59,6 +60,14 @@ def main():
         sc.render_raw(spp=1, max_verts=16, seed=1)
         torch.cuda.synchronize()
         print('1 x 1080p config-2 PT pass', flush=True)
+    if args.c3:
+        from lm_amd.batch import Batch
+        jobs = scenes.config3_jobs()[:args.c3]
+        b = Batch()
+        for j in jobs:
+            b.add_job(scenes.config3_scene(j).flatten(), spp=j['spp'], max_verts=16, seed=j['seed'])
+        st = b.run(width=args.width, path_capacity=1 << 21, block=128, leaf_size=args.leaf)
+        print(f'batch of {len(jobs)} config-3 jobs', st, flush=True)
     if args.c4:
         from lm_amd.guide import Guide
         from lm_amd.renderer import GpuScene

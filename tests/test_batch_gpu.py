@@ -53,7 +53,7 @@ def test_batch_jobs_equal_single_renders(device, width, cap, block):
     b = Batch()
     for flat, p in jobs:
         b.add_job(flat, **p)
-    st = b.run(width=width, path_capacity=cap, block=block)
+    st = b.run(width=width, path_capacity=cap, block=block, sumsq=True)
     print(st)
     assert st['width'] == width and st['path_capacity'] == cap and st['block'] == block
     ref = single_renders(jobs, width)
