@@ -330,7 +330,8 @@ typedef struct lmgpu_guide_opts {
     int32_t batch;            /* records per Adam step, rounded up to 128 (2^18) */
     int32_t steps_per_pass;   /* Adam steps per pass (1); 0 = frozen guide, nothing recorded */
     int32_t record_depth;     /* recording surface vertices per path (8, at most 16) */
-    int32_t reserved[7];
+    float weight_clamp;       /* clamp of training weights relative to the batch mean (0 = 50) */
+    int32_t reserved[6];
 } lmgpu_guide_opts;
 
 /* As lmgpu_pt_render; opts may be NULL (the defaults above). */

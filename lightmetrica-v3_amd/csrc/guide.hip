@@ -488,7 +488,7 @@ int Guide::ensure_train(int64_t B) {
 }
 
 int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* ndev, int64_t off, float lr,
-                         float* grad_host, hipStream_t s) {
+                         float* grad_host, hipStream_t s, float clamp) {
     if (B > train_cap || n > B) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training batch exceeds the workspace");
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, scal.p, int64_t(4));
     hipLaunchKernelGGL(k_zero_f32, dim3(nb(kNParams, 256)), dim3(256), 0, s, G.p, int64_t(kNParams));
@@ -497,7 +497,8 @@ int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* 
     for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
     hipLaunchKernelGGL(k_guide_forward<true>, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, rec, kRecordFloats,
                        Wbf.p, P.p, b, nullptr, xT.p, hT.p, y.p);
-    hipLaunchKernelGGL(k_guide_loss, dim3(nb(B, 256)), dim3(256), 0, s, n, ndev, off, B, y.p, rec, w.p, scal.p, wclamp,
+    hipLaunchKernelGGL(k_guide_loss, dim3(nb(B, 256)), dim3(256), 0, s, n, ndev, off, B, y.p, rec, w.p, scal.p,
+                       clamp > 0.f ? clamp : wclamp,
                        dT.p + 3 * kHid * B, scal.p + 1);
     hipLaunchKernelGGL(k_guide_backward, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, WnBf.p, zero64.p, hT.p,
                        dT.p);

@@ -42,8 +42,9 @@ struct Guide {
     // buffer whose total count *ndev is in device memory. B >= n is the
     // padded row capacity (multiple of 128, prepared by ensure_train).
     // Enqueues kernels only (graph-capturable) unless grad_host is given.
+    // wclamp <= 0: the guide's own clamp of normalised training weights
     int enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* ndev, int64_t off, float lr,
-                      float* grad_host, hipStream_t s);
+                      float* grad_host, hipStream_t s, float wclamp = 0.f);
 };
 
 }  // namespace lmgpu

@@ -648,7 +648,7 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
                            reinterpret_cast<unsigned long long*>(counts.p + 14));
         for (int t = 0; t < gp->steps; t++)
             LMGPU_TRY(gp->g->enqueue_train(gtrain.p + int64_t(t) * gp->batch * kRecordFloats, gp->batch, gp->batch,
-                                           gcount.p, int64_t(t) * gp->batch, gp->lr, nullptr, s));
+                                           gcount.p, int64_t(t) * gp->batch, gp->lr, nullptr, s, gp->wclamp));
     }
     hipLaunchKernelGGL(k_next_sample, dim3(1), dim3(64), 0, s, counts.p);
     LMGPU_HIP(hipGetLastError());
@@ -703,7 +703,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     const GuidedPass none;
     const GuidedPass& gk = gp ? *gp : none;
     const bool guide_same = graph_guide.g == gk.g && graph_guide.uid == gk.uid && graph_guide.gen == gk.gen &&
-                            graph_guide.alpha == gk.alpha && graph_guide.lr == gk.lr &&
+                            graph_guide.alpha == gk.alpha && graph_guide.lr == gk.lr && graph_guide.wclamp == gk.wclamp &&
                             graph_guide.batch == gk.batch && graph_guide.steps == gk.steps &&
                             graph_guide.rec_depth == gk.rec_depth && graph_guide.mul == gk.mul;
     const bool key_same = graph && graph_key[0] == c.max_verts && graph_key[1] == c.mode && graph_key[2] == c.rr_depth &&
@@ -877,10 +877,12 @@ int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lm
     if (o.steps_per_pass > 0 && (o.batch < 1 || o.batch > (1 << 22)))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch must be in [1, 2^22]");
     if (o.record_depth < 0 || o.record_depth > 16) return fail(LMGPU_ERR_INVALID_ARGUMENT, "record_depth must be in [0, 16]");
+    if (!(o.weight_clamp >= 0.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "weight_clamp must be >= 0");
     lmgpu::GuidedPass gp;
     gp.g = &guide->g;
     gp.alpha = o.alpha;
     gp.lr = o.learning_rate;
+    gp.wclamp = o.weight_clamp > 0.f ? o.weight_clamp : 50.f;
     gp.steps = o.steps_per_pass;
     gp.batch = o.steps_per_pass > 0 ? (o.batch + 127) / 128 * 128 : 0;
     gp.rec_depth = o.steps_per_pass > 0 ? o.record_depth : 0;

@@ -19,7 +19,7 @@ struct GuidedPass {
     Guide* g = nullptr;
     uint64_t uid = 0;   // guide identity and buffer generation (graph key)
     int gen = 0;
-    float alpha = 0.5f, lr = 1e-3f;
+    float alpha = 0.5f, lr = 1e-3f, wclamp = 50.f;
     int batch = 0, steps = 0, rec_depth = 0;
     int64_t mul = 1;   // slot visiting order of k_guide_collect
 };

@@ -137,7 +137,8 @@ class BatchStats(ctypes.Structure):
 
 class GuideOpts(ctypes.Structure):
     _fields_ = [('alpha', ctypes.c_float), ('learning_rate', ctypes.c_float), ('batch', ctypes.c_int32),
-                ('steps_per_pass', ctypes.c_int32), ('record_depth', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
+                ('steps_per_pass', ctypes.c_int32), ('record_depth', ctypes.c_int32), ('weight_clamp', ctypes.c_float),
+                ('reserved', ctypes.c_int32 * 6)]
 
 
 _lib = None

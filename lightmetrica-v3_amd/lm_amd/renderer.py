@@ -165,7 +165,7 @@ class GpuScene:
         if guide is not None:
             o = _lib.GuideOpts(0.5, 1e-3, 1 << 18, 1, 8)
             for k, v in (guide_opts or {}).items():
-                if k not in ('alpha', 'learning_rate', 'batch', 'steps_per_pass', 'record_depth'):
+                if k not in ('alpha', 'learning_rate', 'batch', 'steps_per_pass', 'record_depth', 'weight_clamp'):
                     raise ValueError(f'unknown guide option {k!r}')
                 setattr(o, k, v)
             fn, pre = 'lmgpu_pt_render_guided', (self._h, ctypes.byref(p), guide.handle, ctypes.byref(o))
