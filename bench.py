@@ -10,7 +10,10 @@ samples/s at 1920x1080 on config 2 (max 16 vertices, NEE + MIS), and the
 batched rendering of config 3 (64 jobs of 10K-2M triangles, 256-1024^2
 pixels, 16-256 spp, one wavefront loop over a device work queue with
 autotuned BVH width / path capacity / block size): jobs/s and p50 / p99 job
-latency.
+latency; and a reduced equal-time comparison of guided and plain PT on the
+slit-lit room (config 4 at 960x540, 10 s each against a 2048-spp plain
+reference; the 1920x1080 / 30 s / 4096-spp run is
+lightmetrica-v3_amd/lm_amd/equal_time.py).
 
 Prints one JSON line. `roofline` is for the dominant kernel (closest-hit
 traversal, k_trace): algorithmic bytes per launch = rays x B_ray, with
@@ -83,6 +86,7 @@ def main():
     ap.add_argument('--no-pt', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-c3', action='store_true')
+    ap.add_argument('--no-c4', action='store_true')
     ap.add_argument('--cpu-rays', type=int, default=2_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -187,6 +191,22 @@ def main():
             'scene_generation_s': round(gen_s, 1)}
         log('c3', extra['c3_batch'])
         del b
+
+    if not args.no_c4:
+        from lm_amd import equal_time
+        log('config-4 equal-time comparison (960x540, 10 s each)')
+        et = equal_time.run(width=960, height=540, seconds=10.0, ref_spp=2048,
+                            variants=[{'steps_per_pass': 4, 'learning_rate': 0.005}])
+        v = et['variants'][0]
+        extra['c4_equal_time'] = {
+            'resolution': '960x540', 'seconds_each': 10.0, 'reference_spp': 2048,
+            'plain': {'spp': et['plain']['spp'], 'rmse': et['plain']['rmse']},
+            'guided_defaults': {'spp': et['guided']['spp'], 'rmse': et['guided']['rmse'],
+                                'ratio': round(et['rmse_ratio_plain_over_guided'], 3)},
+            'guided_4_steps_lr_5e-3': {'spp': v['spp'], 'rmse': v['rmse'],
+                                       'ratio': round(v['rmse_ratio_plain_over_guided'], 3)},
+            'full_size_result': 'profiles/guided_c4_equal_time.md'}
+        log('c4', extra['c4_equal_time'])
 
     cpu = None
     if not args.no_cpu:

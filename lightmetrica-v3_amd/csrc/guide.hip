@@ -285,21 +285,34 @@ __global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* n
     }
 }
 
-// Bias gradients: db_l[n] = sum_i d_l[n][i]; one block per (layer, n).
+// Bias gradients: db_l[n] = sum_i d_l[n][i]; blocks (layer * 64 + n, chunk)
+// read 8 bf16 per lane and add their partial sum atomically (G starts at 0).
 __global__ __launch_bounds__(256) void k_guide_db(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
                                                   const uint16_t* __restrict__ dT, float* __restrict__ grad) {
     __shared__ float red[4];
     const int layer = blockIdx.x / kHid + 1, n = blockIdx.x % kHid;
-    const uint16_t* d = dT + (int64_t(layer - 1) * kHid + n) * B;
-    const int64_t rows = round128(batch_rows(ndev, off, n_max));
+    if (layer == 4 && n >= kOut) return;   // whole block: padded output rows
+    const uint4* d = reinterpret_cast<const uint4*>(dT + int64_t(blockIdx.x) * B);
+    const int64_t nvec = round128(batch_rows(ndev, off, n_max)) / 8;
     float s = 0.f;
-    for (int64_t i = threadIdx.x; i < rows; i += 256) s += bf2f(d[i]);
+    for (int64_t v = int64_t(blockIdx.y) * 256 + threadIdx.x; v < nvec; v += int64_t(gridDim.y) * 256) {
+        const uint4 q = d[v];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float lo, hi;
+            const uint32_t a = w[k] << 16, b = w[k] & 0xffff0000u;
+            memcpy(&lo, &a, 4);
+            memcpy(&hi, &b, 4);
+            s += lo + hi;
+        }
+    }
     for (int sh = 32; sh > 0; sh >>= 1) s += __shfl_xor(s, sh);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         const int poff = layer == 1 ? kOffB1 : (layer == 2 ? kOffB2 : (layer == 3 ? kOffB3 : kOffB4));
-        if (layer != 4 || n < kOut) grad[poff + n] = red[0] + red[1] + red[2] + red[3];
+        atomicAdd(&grad[poff + n], red[0] + red[1] + red[2] + red[3]);
     }
 }
 
@@ -505,7 +518,8 @@ int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* 
     const int slice = 4096;
     hipLaunchKernelGGL(k_guide_dw, dim3(nb(B, slice), 18), dim3(64), 0, s, n, ndev, off, B, slice, xT.p, hT.p, dT.p,
                        G.p);
-    hipLaunchKernelGGL(k_guide_db, dim3(4 * kHid), dim3(256), 0, s, n, ndev, off, B, dT.p, G.p);
+    hipLaunchKernelGGL(k_guide_db, dim3(4 * kHid, unsigned(std::min<int64_t>(16, (B / 8 + 255) / 256))), dim3(256), 0,
+                       s, n, ndev, off, B, dT.p, G.p);
     if (grad_host) LMGPU_HIP(hipMemcpyAsync(grad_host, G.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_guide_adam, dim3(nb(kNParams, 256)), dim3(256), 0, s, n, ndev, off, stepc.p, P.p, M.p, V.p,
                        G.p, lr, 0.9f, 0.999f, 1e-8f, Wbf.p, WnBf.p);

@@ -180,6 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
             q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(p));
             if (GUIDED) gd.sh_rec[ss] = out.sh_slot;
         }
+        if (GUIDED) {
+            const int rs = wave_append(gd.rec_count, out.rec_slot >= 0);
+            if (out.rec_slot >= 0) gd.rec_list[rs] = out.rec_slot;
+        }
     }
 }
 
@@ -238,42 +242,39 @@ __global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, i
     }
 }
 
-// After a pass: turns the filled record slots into training records
-// (pos, nrm, wo, wi, p_mix, f |cos| L_i estimate) appended to train[cap][16]
-// when their target weight is positive, and empties the slots. Slots are
-// visited in the order j * mul mod total (mul coprime to total) so that a
-// capped batch spreads over the film and the path depths.
-__global__ __launch_bounds__(kBlock) void k_guide_collect(PathState ps, GuideDev gd, int npix, int64_t total,
-                                                          int64_t mul, float* __restrict__ train, int cap,
+// After a pass: turns the records written in this pass (the slots listed in
+// rec_list) into training records (pos, nrm, wo, wi, p_mix, f |cos| L_i
+// estimate) appended to train[cap][16] when their target weight is
+// positive. The list is visited in the order j * P mod n (P a prime above
+// any list length, so a bijection) so that a capped batch spreads over the
+// film and the path depths.
+__global__ __launch_bounds__(kBlock) void k_guide_collect(PathState ps, GuideDev gd, int npix,
+                                                          float* __restrict__ train, int cap,
                                                           int32_t* __restrict__ count,
                                                           unsigned long long* __restrict__ recorded) {
+    constexpr int64_t kScramble = 2654435761;   // prime > 2^31
+    const int64_t n = *gd.rec_count;
     const int64_t stride = int64_t(gridDim.x) * kBlock;
-    const int64_t n_iter = (total + stride - 1) / stride;
+    const int64_t n_iter = (n + stride - 1) / stride;
     for (int64_t it = 0; it < n_iter; it++) {
         const int64_t j = it * stride + int64_t(blockIdx.x) * kBlock + threadIdx.x;
         bool keep = false;
         float out[16];
-        if (j < total) {
-            const int64_t slot = (j * mul) % total;
-            GRec& r = gd.rec[slot];
-            const float4 a = r.a;
-            if (a.w > 0.f) {
-                const int p = int(slot % npix);
-                const float4 Lf = ps.L[p];
-                const float4 e = r.e, f = r.f;
-                const float lx = fmaxf(0.f, Lf.x - e.x) * f.x, ly = fmaxf(0.f, Lf.y - e.y) * f.y,
-                            lz = fmaxf(0.f, Lf.z - e.z) * f.z;
-                const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / a.w;
-                r.a.w = 0.f;
-                if (wgt > 0.f && wgt < 3e38f) {
-                    const float4 b = r.b, c = r.c, d = r.d;
-                    keep = true;
-                    out[0] = a.x; out[1] = a.y; out[2] = a.z;
-                    out[3] = b.x; out[4] = b.y; out[5] = b.z;
-                    out[6] = c.x; out[7] = c.y; out[8] = c.z;
-                    out[9] = d.x; out[10] = d.y; out[11] = d.z;
-                    out[12] = a.w; out[13] = lx; out[14] = ly; out[15] = lz;
-                }
+        if (j < n) {
+            const int64_t slot = gd.rec_list[(j * kScramble) % n];
+            const GRec r = gd.rec[slot];
+            const int p = int(slot % npix);
+            const float4 Lf = ps.L[p];
+            const float lx = fmaxf(0.f, Lf.x - r.e.x) * r.f.x, ly = fmaxf(0.f, Lf.y - r.e.y) * r.f.y,
+                        lz = fmaxf(0.f, Lf.z - r.e.z) * r.f.z;
+            const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / r.a.w;
+            if (wgt > 0.f && wgt < 3e38f) {
+                keep = true;
+                out[0] = r.a.x; out[1] = r.a.y; out[2] = r.a.z;
+                out[3] = r.b.x; out[4] = r.b.y; out[5] = r.b.z;
+                out[6] = r.c.x; out[7] = r.c.y; out[8] = r.c.z;
+                out[9] = r.d.x; out[10] = r.d.y; out[11] = r.d.z;
+                out[12] = r.a.w; out[13] = lx; out[14] = ly; out[15] = lz;
             }
         }
         const unsigned long long mask = __ballot(keep);
@@ -555,8 +556,7 @@ int Scene::ensure_guided(int npix, const GuidedPass& gp) {
     if (slots > grec_slots) {
         drop_graph();
         LMGPU_TRY(grec.alloc(slots));
-        // every slot starts empty (mixture pdf 0)
-        LMGPU_HIP(hipMemset(grec.p, 0, sizeof(GRec) * slots));
+        LMGPU_TRY(grec_list.alloc(slots));
         grec_slots = int(slots);
     }
     const size_t tsz = size_t(gp.batch) * std::max(gp.steps, 1) * kRecordFloats;
@@ -566,7 +566,7 @@ int Scene::ensure_guided(int npix, const GuidedPass& gp) {
     }
     if (!gcount.p) {
         drop_graph();
-        LMGPU_TRY(gcount.alloc(1));
+        LMGPU_TRY(gcount.alloc(2));
     }
     if (gp.steps > 0) LMGPU_TRY(gp.g->ensure_train(gp.batch));
     return LMGPU_OK;
@@ -600,6 +600,8 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         gd.mix = gmix.p;
         gd.rec = grec.p;
         gd.sh_rec = gshrec.p;
+        gd.rec_list = grec_list.p;
+        gd.rec_count = gcount.p + 1;
         gd.alpha = gp->alpha;
         gd.rec_depth = gp->rec_depth;
         for (int a = 0; a < 3; a++) {
@@ -612,6 +614,7 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
     const int npix = c.npix;
     const unsigned gb = unsigned((npix + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_generate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
+    if (gp && gp->rec_depth > 0) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, gcount.p + 1, 1);
     const int sg = shade_grid(npix);
     const int tg = accel.width == 4 ? pt_trace_grid<4>() : (accel.width == 8 ? pt_trace_grid<8>() : pt_trace_grid<2>());
     for (int b = 0; b < c.max_verts - 1; b++) {
@@ -643,8 +646,8 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         const int64_t total = int64_t(gp->rec_depth) * npix;
         const int cap = gp->batch * gp->steps;
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, gcount.p, 1);
-        hipLaunchKernelGGL(k_guide_collect, dim3(unsigned(std::min<int64_t>((total + kBlock - 1) / kBlock, 4096))),
-                           dim3(kBlock), 0, s, ps, gd, npix, total, gp->mul, gtrain.p, cap, gcount.p,
+        hipLaunchKernelGGL(k_guide_collect, dim3(unsigned(std::min<int64_t>((total + kBlock - 1) / kBlock, 2048))),
+                           dim3(kBlock), 0, s, ps, gd, npix, gtrain.p, cap, gcount.p,
                            reinterpret_cast<unsigned long long*>(counts.p + 14));
         for (int t = 0; t < gp->steps; t++)
             LMGPU_TRY(gp->g->enqueue_train(gtrain.p + int64_t(t) * gp->batch * kRecordFloats, gp->batch, gp->batch,
@@ -705,7 +708,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     const bool guide_same = graph_guide.g == gk.g && graph_guide.uid == gk.uid && graph_guide.gen == gk.gen &&
                             graph_guide.alpha == gk.alpha && graph_guide.lr == gk.lr && graph_guide.wclamp == gk.wclamp &&
                             graph_guide.batch == gk.batch && graph_guide.steps == gk.steps &&
-                            graph_guide.rec_depth == gk.rec_depth && graph_guide.mul == gk.mul;
+                            graph_guide.rec_depth == gk.rec_depth;
     const bool key_same = graph && graph_key[0] == c.max_verts && graph_key[1] == c.mode && graph_key[2] == c.rr_depth &&
                           graph_seed == p->seed && graph_sum_ptr == sum && graph_sq_ptr == sumsq && guide_same;
     if (!key_same) {
@@ -850,15 +853,6 @@ int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params,
     return scene->s.raycast(params, film, film_on_device != 0, static_cast<hipStream_t>(stream));
 }
 
-static int64_t gcd64(int64_t a, int64_t b) {
-    while (b) {
-        const int64_t t = a % b;
-        a = b;
-        b = t;
-    }
-    return a;
-}
-
 int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lmgpu_guide* guide,
                            const lmgpu_guide_opts* opts, double* film_sum, double* film_sumsq, int film_on_device,
                            lmgpu_pt_stats* stats, void* stream) {
@@ -892,12 +886,6 @@ int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lm
     if (total >= (int64_t(1) << 31)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "record_depth * pixels must be < 2^31");
     if (int64_t(gp.batch) * std::max(gp.steps, 1) >= (int64_t(1) << 31))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch * steps_per_pass must be < 2^31");
-    if (total > 1) {
-        // a multiplier coprime to the slot count: a bijection of the slots
-        int64_t m = (total * 5) / 8 + 1;
-        while (gcd64(m, total) != 1) m++;
-        gp.mul = m % total;
-    }
     return scene->s.render(params, film_sum, film_sumsq, film_on_device != 0, &gp, stats,
                            static_cast<hipStream_t>(stream));
 }

@@ -85,6 +85,8 @@ struct GuideDev {
     const float* mix;   // decoded mixture per extension-queue entry of the bounce [npix][40]
     GRec* rec;          // [rec_depth][npix]
     int32_t* sh_rec;    // record slot of each shadow-queue entry (-1: none)
+    int32_t* rec_list;  // slots written in this pass
+    int32_t* rec_count; // number of entries of rec_list
     float alpha;        // probability of sampling the guide at a non-specular vertex
     int rec_depth;      // surface vertices per path that record (0: no recording)
 };
@@ -183,6 +185,7 @@ struct VertexOut {
     lmgpu_ray ext_ray, sh_ray;
     f3 sh_C = {0, 0, 0};
     int sh_slot = -1;   // guide record slot the NEE contribution belongs to (-1: none)
+    int rec_slot = -1;  // guide record slot written by this vertex (-1: none)
 };
 
 // One path vertex of Renderer_PT::render (src/renderer/renderer_pt.cpp:
@@ -407,6 +410,7 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
                                           thr_before.y > 0.f ? pmix / thr_before.y : 0.f,
                                           thr_before.z > 0.f ? pmix / thr_before.z : 0.f, 0.f);
                         gd.rec[slot] = r;
+                out.rec_slot = slot;
                     }
                 }
             }

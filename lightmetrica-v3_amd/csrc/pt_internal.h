@@ -21,7 +21,6 @@ struct GuidedPass {
     int gen = 0;
     float alpha = 0.5f, lr = 1e-3f, wclamp = 50.f;
     int batch = 0, steps = 0, rec_depth = 0;
-    int64_t mul = 1;   // slot visiting order of k_guide_collect
 };
 
 struct Scene {
@@ -45,7 +44,7 @@ struct Scene {
     // guided rendering workspace
     DevBuf<float> gmix, gtrain;
     DevBuf<GRec> grec;
-    DevBuf<int32_t> gshrec, gcount;
+    DevBuf<int32_t> gshrec, gcount, grec_list;   // gcount[0] training records, gcount[1] written slots
     int grec_slots = 0;
     hipGraphExec_t graph = nullptr;
     int refill_primary = 64, refill_secondary = 16;

@@ -25,7 +25,10 @@ def log(*a):
 
 
 def run(width=1920, height=1080, seconds=30.0, ref_spp=4096, max_verts=16, chunk=128, guide_opts=None,
-        save_dir=None):
+        save_dir=None, variants=None):
+    """Returns the result dict of the first guide variant (guide_opts) and, under
+    'variants', one entry per element of `variants` (lists of option dicts,
+    each rendered with a fresh guide for `seconds`)."""
     import numpy as np
     import torch
     from lm_amd import scenes, stats
@@ -56,13 +59,24 @@ def run(width=1920, height=1080, seconds=30.0, ref_spp=4096, max_verts=16, chunk
     log(f'plain: {pst["spp"]} spp in {pst["elapsed"]:.2f} s, rMSE {e_plain:.4e}')
     # guided PT, equal time, fresh guide trained during the render
     lo, hi = sc.bounds()
-    g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-    gs, _, gst = sc.render_raw(render_time=seconds, max_verts=max_verts, seed=6, guide=g, guide_opts=guide_opts)
-    guided = gs / gst['spp']
-    e_guided = stats.rel_mse(guided, ref)
-    steps, loss = g.stats()
-    log(f'guided: {gst["spp"]} spp in {gst["elapsed"]:.2f} s, rMSE {e_guided:.4e}, {steps} Adam steps, '
-        f'{gst["guide_records"]} training records')
+
+    def guided_run(opts):
+        g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
+        gs, _, gst = sc.render_raw(render_time=seconds, max_verts=max_verts, seed=6, guide=g, guide_opts=opts)
+        img = gs / gst['spp']
+        e = stats.rel_mse(img, ref)
+        steps, _ = g.stats()
+        log(f'guided {opts or "defaults"}: {gst["spp"]} spp in {gst["elapsed"]:.2f} s, rMSE {e:.4e}, '
+            f'ratio {e_plain / e:.3f}, {steps} Adam steps, {gst["guide_records"]} training records')
+        return img, e, gst, steps
+
+    guided, e_guided, gst, steps = guided_run(guide_opts)
+    extra = []
+    for v in variants or []:
+        _, ev, vst, vsteps = guided_run(v)
+        extra.append({'options': v, 'spp': int(vst['spp']), 'seconds': round(vst['elapsed'], 2), 'rmse': ev,
+                      'ms_per_pass': round(vst['elapsed'] / vst['spp'] * 1e3, 2), 'adam_steps': int(vsteps),
+                      'rmse_ratio_plain_over_guided': e_plain / ev})
     if save_dir:
         os.makedirs(save_dir, exist_ok=True)
         for name, img in (('reference', ref), ('plain', plain), ('guided', guided)):
@@ -77,6 +91,7 @@ def run(width=1920, height=1080, seconds=30.0, ref_spp=4096, max_verts=16, chunk
                    'ms_per_pass': round(gst['elapsed'] / gst['spp'] * 1e3, 2), 'adam_steps': int(steps),
                    'training_records': int(gst['guide_records']), 'options': guide_opts or 'defaults'},
         'rmse_ratio_plain_over_guided': e_plain / e_guided,
+        'variants': extra,
     }
 
 
@@ -88,9 +103,11 @@ def main():
     ap.add_argument('--ref-spp', type=int, default=4096)
     ap.add_argument('--guide-opts', default='', help='JSON dict of lmgpu_guide_opts fields')
     ap.add_argument('--save-dir', default='')
+    ap.add_argument('--variants', default='', help='JSON list of further guide option dicts')
     a = ap.parse_args()
     opts = json.loads(a.guide_opts) if a.guide_opts else None
-    out = run(a.width, a.height, a.time, a.ref_spp, guide_opts=opts, save_dir=a.save_dir or None)
+    out = run(a.width, a.height, a.time, a.ref_spp, guide_opts=opts, save_dir=a.save_dir or None,
+              variants=json.loads(a.variants) if a.variants else None)
     print(json.dumps(out), flush=True)
 
 

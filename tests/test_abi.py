@@ -52,3 +52,15 @@ def test_structure_sizes_match_header():
     assert ctypes.sizeof(_lib.PTParams) == 4 * 4 + 8 + 4 + 4 + 8 + 32
     assert ctypes.sizeof(_lib.PTStats) == 4 * 8 + 8 + 4 * 8
     assert ctypes.sizeof(_lib.GuideOpts) == 4 * 12
+
+
+def test_host_components_built():
+    """The C++ host components (liblmhost.so over the C ABI) and their test
+    program are built and load (their GPU run is in test_raycast_gpu.py)."""
+    pkg = os.path.join(ROOT, 'lightmetrica-v3_amd')
+    lib = os.path.join(pkg, 'liblmhost.so')
+    exe = os.path.join(pkg, 'bin', 'test_host')
+    if not os.path.exists(lib):
+        pytest.skip('liblmhost.so not built')
+    ctypes.CDLL(lib)
+    assert os.access(exe, os.X_OK)
