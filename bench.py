@@ -58,21 +58,34 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(flat, rays, n_rays):
+    """The oracle's FP64 restatement of accel::sahbvh and renderer::pt on all
+    host cores: incoherent (the headline unit) and coherent closest-hit
+    Mrays/s on the config-1 rays, the SAH build time, and renderer::pt
+    samples/s on config 0 (256x256, 16 spp, 8 vertices)."""
     from oracle import oracle as orc
     import numpy as np
+    from lm_amd import scenes
     cores = os.cpu_count() or 1
     t0 = time.time()
     acc = orc.Accel(flat.verts, cores)
     build_s = time.time() - t0
-    sub = {k: np.ascontiguousarray(v[:n_rays]) for k, v in rays.items()}
-    acc.intersect(sub, nthreads=cores)  # warm
+    out = {}
+    for label in ('incoherent', 'coherent'):
+        sub = {k: np.ascontiguousarray(v[:n_rays]) for k, v in rays[label].items()}
+        acc.intersect(sub, nthreads=cores)  # warm
+        t0 = time.time()
+        acc.intersect(sub, nthreads=cores)
+        out[label] = n_rays / (time.time() - t0) / 1e6
+    f0 = scenes.config0().flatten()
     t0 = time.time()
-    acc.intersect(sub, nthreads=cores)
-    dt = time.time() - t0
-    return {'value': n_rays / dt / 1e6, 'unit': 'Mrays/s', 'cores': cores, 'kind': 'port',
+    r = orc.pt_render(f0, spp=16, max_verts=8, seed=1, nthreads=cores, sumsq=False)
+    pt_s = r['stats']['samples'] / (time.time() - t0)
+    return {'value': out['incoherent'], 'unit': 'Mrays/s', 'cores': cores, 'kind': 'port',
             'sample': f'FP64 accel::sahbvh restatement (oracle), OpenMP over {n_rays} of the 2,000,000 '
                       f'incoherent config-1 rays; SAH build of 1,320,720 tris {build_s:.1f} s excluded',
-            'build_s': build_s}
+            'build_s': round(build_s, 2), 'coherent_mrays_s': round(out['coherent'], 2),
+            'pt_c0_samples_s': round(pt_s, 1),
+            'pt_c0_sample': 'renderer::pt restatement (oracle), config 0 256x256 16 spp, max 8 vertices, MIS'}
 
 
 def main():
