@@ -139,6 +139,19 @@ int lmgpu_accel_status(const lmgpu_accel* accel, void* stream);
 int lmgpu_accel_intersect_counted(const lmgpu_accel* accel, const lmgpu_ray* rays, lmgpu_hit* hits,
                                   int64_t n, int64_t* node_visits, int64_t* tri_tests, void* stream);
 
+/* Serialized form of a built accel, the counterpart of the reference's
+ * serialization of accel::sahbvh (src/accel/accel_sahbvh.cpp:122-124,
+ * cereal; own format): a 128-byte header ("LMGPUBVH", version, width, leaf
+ * size, depth, counts, SAH cost, bounds) followed by the nodes and the
+ * triangles as laid out on the device. With buf == NULL, *size receives the
+ * byte count; else *size is the buffer size on input and the bytes written
+ * on output. */
+int lmgpu_accel_serialize(const lmgpu_accel* accel, void* buf, size_t* size);
+/* Replaces the accel's BVH with a serialized one, validated on the host
+ * (header, sizes, tree structure, leaf ranges, triangle ids) before upload;
+ * opts may give the traversal refill threshold. */
+int lmgpu_accel_deserialize(lmgpu_accel* accel, const void* buf, size_t size, const lmgpu_build_opts* opts);
+
 /* ---- scene + renderer::pt ------------------------------------------------------ */
 
 /* Material type codes: material::diffuse, ::glossy, ::mirror, ::glass. */

@@ -144,3 +144,145 @@ int lmgpu_accel_status(const lmgpu_accel* accel, void* stream) {
 }
 
 }  // extern "C"
+
+// ---- serialization ------------------------------------------------------------
+// A built accel as bytes: a 128-byte little-endian header followed by the
+// wide nodes and the triangles exactly as laid out on the device. Loading
+// validates the whole structure on the host first (a malformed BVH would
+// send the traversal out of bounds or round a cycle).
+
+namespace {
+
+struct SerialHeader {
+    char magic[8];          // "LMGPUBVH"
+    uint32_t version;       // 1
+    uint32_t width, leaf_size, depth;
+    int64_t num_tris, num_nodes, num_leaves;
+    double sah_cost;
+    float root_lo[3], root_hi[3];
+    uint8_t pad[128 - 8 - 16 - 24 - 8 - 24];
+};
+static_assert(sizeof(SerialHeader) == 128, "header layout");
+
+constexpr uint32_t kSerialVersion = 1;
+
+template <int W>
+int validate_nodes(const std::vector<unsigned char>& nb, int64_t num_nodes, int64_t num_tris, int* depth_out) {
+    const auto* nodes = reinterpret_cast<const lmgpu::WNode<W>*>(nb.data());
+    std::vector<unsigned char> seen(size_t(num_nodes), 0);
+    std::vector<std::pair<int64_t, int>> st{{0, 1}};
+    int64_t visited = 0;
+    int depth = 0;
+    while (!st.empty()) {
+        const auto [i, d] = st.back();
+        st.pop_back();
+        if (seen[size_t(i)]) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH is not a tree");
+        seen[size_t(i)] = 1;
+        visited++;
+        depth = std::max(depth, d);
+        const lmgpu::WNode<W>& n = nodes[i];
+        for (int c = 0; c < W; c++) {
+            const int cnt = n.count[c], ch = n.child[c];
+            if (cnt == 0) {
+                if (ch <= 0 || ch >= num_nodes) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: child out of range");
+                st.push_back({ch, d + 1});
+            } else if (cnt > 0) {
+                if (cnt > 15 || ch < 0 || int64_t(ch) + cnt > num_tris)
+                    return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: leaf range out of range");
+            }
+        }
+    }
+    if (visited != num_nodes) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH has unreachable nodes");
+    *depth_out = depth;
+    return LMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmgpu_accel_serialize(const lmgpu_accel* accel, void* buf, size_t* size) {
+    if (!accel || !size) return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL argument");
+    const lmgpu::Accel& a = accel->a;
+    if (!a.built) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel is not built");
+    const size_t nbytes = size_t(a.num_nodes) * 32 * a.width, tbytes = size_t(a.num_tris) * sizeof(lmgpu::TriV);
+    const size_t total = sizeof(SerialHeader) + nbytes + tbytes;
+    if (!buf) {
+        *size = total;
+        return LMGPU_OK;
+    }
+    if (*size < total) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialization buffer too small");
+    SerialHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "LMGPUBVH", 8);
+    h.version = kSerialVersion;
+    h.width = uint32_t(a.width);
+    h.leaf_size = uint32_t(a.leaf_size);
+    h.depth = uint32_t(a.depth);
+    h.num_tris = a.num_tris;
+    h.num_nodes = a.num_nodes;
+    h.num_leaves = a.num_leaves;
+    h.sah_cost = a.sah_cost;
+    for (int k = 0; k < 3; k++) {
+        h.root_lo[k] = a.root_lo[k];
+        h.root_hi[k] = a.root_hi[k];
+    }
+    auto* out = static_cast<unsigned char*>(buf);
+    std::memcpy(out, &h, sizeof(h));
+    LMGPU_HIP(hipMemcpy(out + sizeof(h), a.nodes.p, nbytes, hipMemcpyDeviceToHost));
+    LMGPU_HIP(hipMemcpy(out + sizeof(h) + nbytes, a.tris.p, tbytes, hipMemcpyDeviceToHost));
+    *size = total;
+    return LMGPU_OK;
+}
+
+int lmgpu_accel_deserialize(lmgpu_accel* accel, const void* buf, size_t size, const lmgpu_build_opts* opts) {
+    if (!accel || !buf) return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL argument");
+    lmgpu::clear_error();
+    if (size < sizeof(SerialHeader)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized accel too short");
+    SerialHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, "LMGPUBVH", 8) != 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "not a serialized lmgpu accel");
+    if (h.version != kSerialVersion) return fail(LMGPU_ERR_UNSUPPORTED, "unsupported serialized accel version");
+    if (h.width != 2 && h.width != 4 && h.width != 8) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad BVH width");
+    if (h.num_tris <= 0 || h.num_nodes <= 0 || h.num_tris >= (int64_t(1) << 27) || h.num_nodes >= (int64_t(1) << 31))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad BVH sizes");
+    const size_t nbytes = size_t(h.num_nodes) * 32 * h.width, tbytes = size_t(h.num_tris) * sizeof(lmgpu::TriV);
+    if (size != sizeof(SerialHeader) + nbytes + tbytes) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized accel size mismatch");
+    const auto* in = static_cast<const unsigned char*>(buf) + sizeof(SerialHeader);
+    std::vector<unsigned char> nb(in, in + nbytes);
+    int depth = 0;
+    int rc = h.width == 2 ? validate_nodes<2>(nb, h.num_nodes, h.num_tris, &depth)
+             : h.width == 4 ? validate_nodes<4>(nb, h.num_nodes, h.num_tris, &depth)
+                            : validate_nodes<8>(nb, h.num_nodes, h.num_tris, &depth);
+    if (rc != LMGPU_OK) return rc;
+    // every triangle's original index (v0.w bits) must name a triangle
+    const auto* tv = reinterpret_cast<const lmgpu::TriV*>(in + nbytes);
+    for (int64_t t = 0; t < h.num_tris; t++) {
+        int32_t id;
+        std::memcpy(&id, &tv[t].v0.w, 4);
+        if (id < 0 || id >= h.num_tris) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: triangle id out of range");
+    }
+    lmgpu::Accel& a = accel->a;
+    a.release();
+    LMGPU_TRY(a.nodes.alloc(size_t(h.num_nodes) * 2 * h.width));
+    LMGPU_TRY(a.tris.alloc(size_t(h.num_tris)));
+    LMGPU_HIP(hipMemcpy(a.nodes.p, in, nbytes, hipMemcpyHostToDevice));
+    LMGPU_HIP(hipMemcpy(a.tris.p, in + nbytes, tbytes, hipMemcpyHostToDevice));
+    a.width = int(h.width);
+    a.leaf_size = int(h.leaf_size);
+    a.refill = (opts && opts->refill > 0) ? std::min(opts->refill, 64) : 32;
+    a.num_tris = h.num_tris;
+    a.num_nodes = h.num_nodes;
+    a.num_leaves = h.num_leaves;
+    a.depth = depth;
+    a.sah_cost = h.sah_cost;
+    a.build_ms = 0;
+    for (int k = 0; k < 3; k++) {
+        a.root_lo[k] = h.root_lo[k];
+        a.root_hi[k] = h.root_hi[k];
+    }
+    a.built = true;
+    return LMGPU_OK;
+}
+
+}  // extern "C"

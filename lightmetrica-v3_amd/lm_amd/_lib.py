@@ -35,7 +35,7 @@ EXPORTED = [
     'lmgpu_last_error', 'lmgpu_version', 'lmgpu_device_count',
     'lmgpu_accel_create', 'lmgpu_accel_destroy', 'lmgpu_accel_build', 'lmgpu_accel_info_get',
     'lmgpu_accel_intersect', 'lmgpu_accel_intersect_host', 'lmgpu_accel_occluded', 'lmgpu_accel_status',
-    'lmgpu_accel_intersect_counted',
+    'lmgpu_accel_intersect_counted', 'lmgpu_accel_serialize', 'lmgpu_accel_deserialize',
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
     'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
@@ -175,6 +175,8 @@ def lib():
         L.lmgpu_accel_occluded.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_accel_status.argtypes = [vp, vp]
         L.lmgpu_accel_intersect_counted.argtypes = [vp, vp, vp, i64, ctypes.POINTER(i64), ctypes.POINTER(i64), vp]
+        L.lmgpu_accel_serialize.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.lmgpu_accel_deserialize.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(BuildOpts)]
         L.lmgpu_scene_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ctypes.POINTER(BuildOpts)]
         L.lmgpu_scene_destroy.argtypes = [vp]
         L.lmgpu_scene_accel.argtypes = [vp, ctypes.POINTER(vp)]

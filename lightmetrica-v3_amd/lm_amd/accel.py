@@ -115,6 +115,28 @@ class Accel:
             _lib.call('lmgpu_accel_build', self._h, v.ctypes.data_as(ctypes.c_void_p), v.shape[0], 0,
                       ctypes.byref(opts), ctypes.c_void_p(_stream_ptr(stream)))
 
+    def serialize(self) -> bytes:
+        """The built BVH as bytes (lmgpu_accel_serialize)."""
+        n = ctypes.c_size_t(0)
+        _lib.call('lmgpu_accel_serialize', self._h, None, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value)
+        _lib.call('lmgpu_accel_serialize', self._h, ctypes.cast(buf, ctypes.c_void_p), ctypes.byref(n))
+        return buf.raw[:n.value]
+
+    def deserialize(self, data: bytes):
+        """Replace the BVH with a serialized one (validated before upload)."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        opts = _lib.BuildOpts(0, 0, self.refill, 0, 0)
+        _lib.call('lmgpu_accel_deserialize', self._h, ctypes.cast(buf, ctypes.c_void_p), len(data), ctypes.byref(opts))
+
+    def save(self, path: str):
+        with open(path, 'wb') as f:
+            f.write(self.serialize())
+
+    def load(self, path: str):
+        with open(path, 'rb') as f:
+            self.deserialize(f.read())
+
     def info(self) -> dict:
         i = _lib.AccelInfo()
         _lib.call('lmgpu_accel_info_get', self._h, ctypes.byref(i))

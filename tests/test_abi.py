@@ -64,3 +64,41 @@ def test_host_components_built():
         pytest.skip('liblmhost.so not built')
     ctypes.CDLL(lib)
     assert os.access(exe, os.X_OK)
+
+
+def _serial(width, nodes, tris, num_leaves=1, magic=b'LMGPUBVH', version=1):
+    """Serialized accel bytes: 128-byte header + nodes + triangles."""
+    import struct
+    nn, nt = len(nodes), len(tris)
+    h = magic + struct.pack('<IIII', version, width, 2, 1) + struct.pack('<qqq', nt, nn, num_leaves)
+    h += struct.pack('<d', 0.0) + struct.pack('<6f', 0, 0, 0, 1, 1, 1)
+    h += b'\0' * (128 - len(h))
+    body = b''
+    for child, count in nodes:   # one node: 6*W floats of bounds, W children, W counts
+        body += struct.pack(f'<{6 * width}f', *([0.0] * 3 * width + [1.0] * 3 * width))
+        body += struct.pack(f'<{width}i', *child) + struct.pack(f'<{width}i', *count)
+    for tid in tris:
+        body += struct.pack('<3f', 0, 0, 0) + struct.pack('<i', tid) + struct.pack('<8f', *([0.0] * 8))
+    return h + body
+
+
+def test_deserialize_rejects_malformed_bvh_on_the_host():
+    """lmgpu_accel_deserialize validates everything before any device work:
+    magic, version, sizes, child ranges, tree structure (no cycles / shared
+    children), leaf ranges and triangle ids."""
+    from lm_amd.accel import Accel
+    a = Accel(4)
+    good = _serial(4, [([0, 0, 0, 0], [1, -1, -1, -1])], [0])
+    bad = {
+        'magic': _serial(4, [([0, 0, 0, 0], [1, -1, -1, -1])], [0], magic=b'NOTABVH!'),
+        'version': _serial(4, [([0, 0, 0, 0], [1, -1, -1, -1])], [0], version=9),
+        'size': good[:-4],
+        'cycle': _serial(4, [([1, 0, 0, 0], [0, -1, -1, -1]), ([1, 0, 0, 0], [0, -1, -1, -1])], [0]),
+        'child range': _serial(4, [([7, 0, 0, 0], [0, -1, -1, -1])], [0]),
+        'leaf range': _serial(4, [([0, 0, 0, 0], [3, -1, -1, -1])], [0]),
+        'triangle id': _serial(4, [([0, 0, 0, 0], [1, -1, -1, -1])], [5]),
+    }
+    for name, data in bad.items():
+        with pytest.raises(_lib.LmError):
+            a.deserialize(data)
+        print(name, _lib.lib().lmgpu_last_error().decode())
