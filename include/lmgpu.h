@@ -242,6 +242,31 @@ typedef struct lmgpu_raycast_params {
 int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params, double* film, int film_on_device,
                          void* stream);
 
+/* renderer::lt (src/renderer/renderer_lt.cpp:22-170, Renderer_LT_NEE::render):
+ * light subpaths from the scene's lights (area: light_area.cpp:78-100,
+ * envconst: light_envconst.cpp:47-74) connected to the pinhole camera at every
+ * surface vertex, splatted into film[h][w][3] (FP64, y = 0 the bottom row;
+ * device films accumulate, host films are overwritten). `num_samples` light
+ * paths (scheduler::spi::sample), traced in passes of at most w*h; with
+ * render_time > 0, passes of w*h paths until that many seconds have passed.
+ * The reference's film value is film * (w*h) / stats->samples
+ * (renderer_lt.cpp:166); stats->spp counts the passes. Light paths of pass k
+ * (pass_offset + k) draw Philox counters (path, pass, vertex, block), so
+ * progressive calls continue with pass_offset += the previous spp. Paths that
+ * hit the emitter directly seen by the camera (two vertices) carry no
+ * contribution, as in the reference. Synchronous. */
+typedef struct lmgpu_lt_params {
+    int64_t num_samples;
+    int32_t max_verts;        /* max path vertices (camera and light included) */
+    int32_t pass_offset;
+    uint64_t seed;
+    double render_time;
+    int32_t reserved[8];
+} lmgpu_lt_params;
+
+int lmgpu_lt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
+                    lmgpu_pt_stats* stats, void* stream);
+
 /* ---- batched rendering ------------------------------------------------------
  * Many renderer::pt jobs (each its own scene, film size, spp and seed) in
  * one wavefront loop over a pool of path slots fed from a device work queue;

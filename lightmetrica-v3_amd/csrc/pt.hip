@@ -63,29 +63,6 @@ __global__ void k_reset_counts(int32_t* counts, int out_q) {
 }
 
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill) {
-    __shared__ int lds[kLds][kBlock];
-    const int n = q.counts[in_q];
-    const lmgpu_ray* rays = q.ext[in_q];
-    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
-        const float4* rp = reinterpret_cast<const float4*>(rays + id);
-        const float4 a = rp[0], b = rp[1];
-        o = mk(a.x, a.y, a.z);
-        d = mk(b.x, b.y, b.z);
-        tmin = a.w;
-        tmax = b.w;
-        return 0;
-    };
-    auto emit = [&](int64_t id, const TraceResult& r) {
-        reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
-    };
-    trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
-                                                           q.counts + 5, nullptr, refill);
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
-}
-
 // Unoccluded shadow rays add their contribution to the path radiance and,
 // when the vertex records guide training data, to the record's radiance
 // baseline (the NEE of a vertex is not part of its sampled direction's
@@ -806,10 +783,6 @@ int Scene::raycast(const lmgpu_raycast_params* p, double* film_in, bool on_devic
 using lmgpu::fail;
 
 extern "C" {
-
-struct lmgpu_scene {
-    lmgpu::Scene s;
-};
 
 int lmgpu_scene_create(lmgpu_scene** out, const lmgpu_scene_desc* desc, const lmgpu_build_opts* opts) {
     if (!out) return fail(LMGPU_ERR_INVALID_ARGUMENT, "out is NULL");

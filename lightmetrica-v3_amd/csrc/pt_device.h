@@ -419,4 +419,44 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
     P.L = L;
 }
 
+// Closest hit of the extension queue `in_q` (persistent traversal), hits
+// written as (prim or -1 / -2 on stack overflow, t, u, v); shared by the
+// path and light tracers.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill) {
+    __shared__ int lds[kLds][kBlock];
+    const int n = q.counts[in_q];
+    const lmgpu_ray* rays = q.ext[in_q];
+    auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
+        const float4* rp = reinterpret_cast<const float4*>(rays + id);
+        const float4 a = rp[0], b = rp[1];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+        return 0;
+    };
+    auto emit = [&](int64_t id, const TraceResult& r) {
+        reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
+    };
+    trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
+                                                           q.counts + 5, nullptr, refill);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
+}
+
+// Resident-occupancy grid of k_extend<W> on the current device.
+template <int W>
+inline int extend_grid() {
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, a = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<W>, kBlock, 0);
+        grid = (cus > 0 ? cus : 1) * (a > 0 ? a : 1);
+    }
+    return grid;
+}
+
 }  // namespace lmgpu

@@ -66,7 +66,13 @@ struct Scene {
     int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
                lmgpu_pt_stats* st, hipStream_t s);
     int raycast(const lmgpu_raycast_params* p, double* film, bool on_device, hipStream_t s);
+    // renderer::lt (lt.hip)
+    int lt_render(const lmgpu_lt_params* p, double* film, bool on_device, lmgpu_pt_stats* st, hipStream_t s);
 };
 
-
 }  // namespace lmgpu
+
+// The opaque C ABI handle of a scene.
+struct lmgpu_scene {
+    lmgpu::Scene s;
+};

@@ -329,6 +329,29 @@ private:
     std::vector<double> sum_, sumsq_;
 };
 
+// renderer::lt_mi355x (src/renderer/renderer_lt.cpp:22-170: light tracing with
+// NEE to the camera). Props: max_verts (int, required), scheduler ("sample":
+// num_samples light paths, or "time": render_time seconds), seed (int),
+// accel_width, accel_leaf_size. processed = light paths traced; the film is
+// the splat sum times w*h / processed (renderer_lt.cpp:166).
+class Renderer_LT_MI355X final : public Renderer {
+public:
+    Renderer_LT_MI355X(const Scene* scene, Film* film, const Props& props);
+    ~Renderer_LT_MI355X() override;
+    RenderResult render() override;
+
+private:
+    const Scene* scene_;
+    Film* film_;
+    int max_verts_;
+    long long num_samples_;
+    double render_time_;
+    uint64_t seed_;
+    int acc_width_, acc_leaf_;
+    lmgpu_scene* gs_ = nullptr;
+    uint64_t built_version_ = ~0ull;
+};
+
 // renderer::raycast_mi355x (src/renderer/renderer_raycast.cpp:48-97). Props:
 //   bg_color (Vec3), use_constant_color (bool), visualize_normal (bool), color (Vec3).
 class Renderer_Raycast_MI355X final : public Renderer {

@@ -186,6 +186,31 @@ class GpuScene:
                  'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s, 'guide_records': st.guide_records}
         return s, q, stats
 
+    def render_lt(self, num_samples: int = 0, max_verts: int = 8, seed: int = 0, pass_offset: int = 0,
+                  render_time: float = 0.0, film=None, stream=None):
+        """lmgpu_lt_render: splat sums of `num_samples` light paths (or of
+        w*h-path passes for `render_time` s). A torch CUDA tensor `film`
+        ([h,w,3] float64) is accumulated into on the device; otherwise a fresh
+        host array is returned. The film value is sum * w*h / samples."""
+        p = _lib.LTParams()
+        p.num_samples = int(num_samples)
+        p.max_verts = int(max_verts)
+        p.pass_offset = int(pass_offset)
+        p.seed = int(seed)
+        p.render_time = float(render_time)
+        st = _lib.PTStats()
+        if film is not None and hasattr(film, 'data_ptr'):
+            _lib.call('lmgpu_lt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()), 1,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+            s = film
+        else:
+            s = np.zeros((self.h, self.w, 3), dtype=np.float64)
+            _lib.call('lmgpu_lt_render', self._h, ctypes.byref(p), s.ctypes.data_as(ctypes.c_void_p), 0,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+        stats = {'samples': st.samples, 'passes': st.spp, 'extension_rays': st.extension_rays,
+                 'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s}
+        return s, stats
+
 
 class RendererRaycast:
     """renderer::raycast_mi355x: drop-in for renderer::raycast
@@ -255,4 +280,40 @@ class RendererPT:
         processed = st['spp']
         film.sum, film.sumsq, film.spp = s, q, processed
         film.data[:] = s / processed
+        return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}
+
+
+class RendererLT:
+    """renderer::lt_mi355x: drop-in for renderer::lt (light tracing with NEE to
+    the camera, src/renderer/renderer_lt.cpp:22-170) with the image-space
+    schedulers scheduler::spi::sample (`num_samples` light paths) and
+    scheduler::spi::time (`render_time` seconds, in passes of w*h paths)."""
+
+    def __init__(self, scene: GpuScene, output: Film, max_verts: int, seed: Optional[int] = None,
+                 scheduler: str = 'sample', num_samples: Optional[int] = None, render_time: Optional[float] = None):
+        if scheduler == 'sample' and not num_samples:
+            raise ValueError('scheduler "sample" needs num_samples')
+        if scheduler == 'time' and not render_time:
+            raise ValueError('scheduler "time" needs render_time')
+        if scheduler not in ('sample', 'time'):
+            raise ValueError(f'invalid scheduler {scheduler!r}')
+        if (output.w, output.h) != (scene.w, scene.h):
+            raise ValueError('film size differs from the scene film size')
+        self.scene = scene
+        self.film = output
+        self.max_verts = int(max_verts)
+        self.seed = int(seed) if seed is not None else int(np.random.SeedSequence().generate_state(1)[0])
+        self.scheduler = scheduler
+        self.num_samples = int(num_samples or 0)
+        self.render_time = float(render_time or 0.0)
+
+    def render(self) -> dict:
+        film = self.film
+        film.clear()
+        s, st = self.scene.render_lt(
+            num_samples=self.num_samples if self.scheduler == 'sample' else 0, max_verts=self.max_verts,
+            seed=self.seed, render_time=self.render_time if self.scheduler == 'time' else 0.0)
+        processed = st['samples']
+        # Film::rescale(w*h / processed) of the splatted sums (renderer_lt.cpp:166)
+        film.data[:] = s * (film.num_pixels() / processed)
         return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}

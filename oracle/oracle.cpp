@@ -22,6 +22,9 @@
 //   * path:: dispatch                     include/lm/path.h:193-775
 //   * renderer::pt                        src/renderer/renderer_pt.cpp:80-315
 //   * scheduler::spp::sample              src/scheduler.cpp:37-49
+//   * renderer::lt (light tracing, NEE to the camera)
+//                                         src/renderer/renderer_lt.cpp:56-170
+//   * scheduler::spi::sample              src/scheduler.cpp:114-139
 //
 // One deliberate difference: random numbers come from a counter-based
 // Philox4x32-10 stream keyed on (pixel, sample, vertex, block) instead of the
@@ -329,6 +332,14 @@ struct OrcPTParams {
     uint64_t seed;
     int32_t nthreads;
     int32_t sample_offset;   // first sample index (for progressive passes)
+};
+
+struct OrcLTParams {
+    int64_t num_samples;
+    int32_t max_verts;
+    int32_t pass_offset;
+    uint64_t seed;
+    int32_t nthreads;
 };
 
 struct OrcPTStats {
@@ -1072,6 +1083,165 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
     }
     if (stats) {
         stats->samples += npix * P->spp;
+        stats->extension_rays += n_ext;
+        stats->shadow_rays += n_shadow;
+        stats->path_vertices += n_vert;
+    }
+    return 0;
+}
+
+// ---- renderer::lt -------------------------------------------------------------
+
+// surface::shading_normal_correction with TransDir::LE (surface.h:424-438).
+static double shading_normal_correction_le(const Geom& g, V3 wi, V3 wo) {
+    const double wi_ng = dot(wi, g.gn), wo_ng = dot(wo, g.gn);
+    const double wi_ns = dot(wi, g.n), wo_ns = dot(wo, g.n);
+    if (wi_ng * wi_ns <= 0 || wo_ng * wo_ns <= 0) return 0;
+    return (wi_ns * wo_ng) / (wo_ns * wi_ng);
+}
+
+// Restates Renderer_LT_NEE::render (src/renderer/renderer_lt.cpp:56-170) with
+// scheduler::spi::sample: `num_samples` light paths in passes of w*h; path i
+// of pass k (= pass_offset + i / (w*h)) draws Philox counters
+// (i mod w*h, k, vertex, block): vertex 0 block 0 {light selection, light
+// triangle, position u0, u1}, block 1 {direction u0, u1}; vertex v >= 2
+// block 0 {RR, component, direction u0, u1}. Splats are accumulated unscaled
+// into film[h][w][3]; the reference's film is film * w*h / num_samples.
+//   path::sample_primary_ray(LE) -> light::area::sample_ray
+//       (light_area.cpp:78-100) or light::envconst::sample_ray
+//       (light_envconst.cpp:38-74), divided by the selection probability
+//   path::sample_direct(EL) -> camera::pinhole::sample_direct, raster_position
+//       (camera_pinhole.cpp:100-227)
+//   path::sample_direction(LE): material::glass refraction without eta^2
+//       (material_glass.cpp:85-87) and the LE shading-normal correction
+int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, double* film, OrcPTStats* stats) {
+    Scene S;
+    init_scene(S, scene, static_cast<Accel*>(accel));
+    if (S.num_lights == 0) return -1;
+    const int w = scene->w, h = scene->h;
+    const int64_t npix = int64_t(w) * h;
+    // scene bound for light::envconst (Light_EnvConst::set_scene_bound)
+    V3 bmin(Inf), bmax(-Inf);
+    for (int64_t k = 0; k < scene->num_tris * 3; k++) {
+        const V3 p = load3(scene->verts + 3 * k);
+        bmin = vmin(bmin, p);
+        bmax = vmax(bmax, p);
+    }
+    const V3 sc = (bmax + bmin) * .5;
+    const double sr = length(bmax - sc) * 1.01;
+    const Geom camG = make_degenerated(S.pos);
+    if (P->nthreads > 0) omp_set_num_threads(P->nthreads);
+    int64_t n_ext = 0, n_shadow = 0, n_vert = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : n_ext, n_shadow, n_vert)
+    for (int64_t i = 0; i < P->num_samples; i++) {
+        const uint32_t idx = uint32_t(i % npix);
+        const uint32_t pass = uint32_t(P->pass_offset + i / npix);
+        const Rand4 r0 = rand4(idx, pass, 0, 0, P->seed);
+        const Rand4 r1 = rand4(idx, pass, 0, 1, P->seed);
+        const int li = std::clamp(int(r0.u[0] * S.num_lights), 0, S.num_lights - 1);
+        const double psel = 1.0 / S.num_lights;
+        V3 o, d, thr;
+        if (li < int(S.lights.size())) {
+            const auto& AL = S.lights[li];
+            const int ti = AL.tris[dist_sample(AL.cdf, r0.u[1])];
+            const float* v = scene->verts + 9 * int64_t(ti);
+            const V3 a = load3(v), b = load3(v + 3), c = load3(v + 6);
+            const double s = safe_sqrt(r0.u[2]);
+            const double bu = 1 - s, bv = r0.u[3] * s;
+            o = a * (1 - bu - bv) + b * bu + c * bv;
+            const V3 gn = normalize(cross(b - a, c - a));
+            V3 u, vv;
+            onb(gn, u, vv);
+            const V3 l = sample_cosine_weighted(r1.u[0], r1.u[1]);
+            d = u * l.x + vv * l.y + gn * l.z;
+            thr = AL.Ke / (AL.invA * (1.0 / Pi)) / psel;
+        } else {
+            d = sample_uniform_sphere(r1.u[0], r1.u[1]);
+            const double r = safe_sqrt(r0.u[2]) * sr, t = 2 * Pi * r0.u[3];
+            V3 u, vv;
+            onb(d, u, vv);
+            o = sc + sr * (-d) + (u * (r * std::cos(t)) + vv * (r * std::sin(t)));
+            thr = S.env_Le / ((1.0 / (4 * Pi)) * (1.0 / (Pi * sr * sr))) / psel;
+        }
+        n_ext++;
+        Hit hit = scene_intersect(S, o, d, Eps, Inf);
+        if (!hit.valid || hit.env) continue;
+        n_vert++;
+        {
+            const Rand4 rn = rand4(idx, pass, 2, 0, P->seed);
+            int c;
+            double cw;
+            mat_sample_component(S.mats[hit.prim], hit.g, -d, rn.u[1], c, cw);
+            thr = thr * cw;
+            Geom sp = hit.g;
+            int sp_prim = hit.prim;
+            V3 wi = -d;
+            int comp = c;
+            for (int nv = 2; nv < P->max_verts; nv++) {
+                const Mat& m = S.mats[sp_prim];
+                // NEE edge to the camera
+                [&] {
+                    const V3 wo_c = normalize(sp.p - S.pos);   // camera -> point
+                    const double G = geometry_term(sp, camG);
+                    if (G == 0) return;
+                    const double ez = dot(wo_c, S.cw);
+                    if (ez >= 0) return;
+                    n_shadow++;
+                    if (!visible(S, sp, camG)) return;
+                    const double rx = -dot(wo_c, S.cu) / ez / S.tf / S.aspect * .5 + .5;
+                    const double ry = -dot(wo_c, S.cv) / ez / S.tf * .5 + .5;
+                    if (rx < 0 || rx > 1 || ry < 0 || ry > 1) return;
+                    const double ct = -ez;
+                    const double J = 1.0 / (ct * ct * ct) / (S.tf * S.tf * S.aspect * 4);
+                    const V3 wo = -wo_c;
+                    const V3 fs = mat_eval_delta(m, sp, wi, wo) * shading_normal_correction_le(sp, wi, wo);
+                    const V3 C = thr * fs * (J * G);
+                    const int px = std::clamp(int(rx * w), 0, w - 1);
+                    const int py = std::clamp(int(ry * h), 0, h - 1);
+                    double* f = film + 3 * (int64_t(py) * w + px);
+#pragma omp atomic
+                    f[0] += C.x;
+#pragma omp atomic
+                    f[1] += C.y;
+#pragma omp atomic
+                    f[2] += C.z;
+                }();
+                // a vertex beyond max_verts - 1 could not be connected any more
+                if (nv + 1 >= P->max_verts) break;
+                const Rand4 ra = rand4(idx, pass, uint32_t(nv), 0, P->seed);
+                V3 wo, sw;
+                if (!mat_sample_direction(m, sp, wi, comp, ra.u[2], ra.u[3], wo, sw)) break;
+                if (m.type == MAT_GLASS && comp == 1) {
+                    V3 wt; bool total; double eta;
+                    glass_Fr(m, sp, wi, wt, total, eta);
+                    sw = sw / (eta * eta);
+                }
+                sw = sw * shading_normal_correction_le(sp, wi, wo);
+                n_ext++;
+                const Hit nh = scene_intersect(S, sp.p, wo, Eps, Inf);
+                if (!nh.valid) break;
+                thr = thr * sw;
+                if (nh.env) break;
+                n_vert++;
+                const Rand4 rn2 = rand4(idx, pass, uint32_t(nv + 1), 0, P->seed);
+                if (nv > 5) {
+                    const double q = std::max(.2, 1 - comp_max(thr));
+                    if (rn2.u[0] < q) break;
+                    thr = thr / (1 - q);
+                }
+                int c2;
+                double cw2;
+                mat_sample_component(S.mats[nh.prim], nh.g, -wo, rn2.u[1], c2, cw2);
+                thr = thr * cw2;
+                wi = -wo;
+                sp = nh.g;
+                sp_prim = nh.prim;
+                comp = c2;
+            }
+        }
+    }
+    if (stats) {
+        stats->samples += P->num_samples;
         stats->extension_rays += n_ext;
         stats->shadow_rays += n_shadow;
         stats->path_vertices += n_vert;

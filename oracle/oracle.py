@@ -59,6 +59,16 @@ class OrcPTParams(ctypes.Structure):
     ]
 
 
+class OrcLTParams(ctypes.Structure):
+    _fields_ = [
+        ('num_samples', ctypes.c_int64),
+        ('max_verts', ctypes.c_int32),
+        ('pass_offset', ctypes.c_int32),
+        ('seed', ctypes.c_uint64),
+        ('nthreads', ctypes.c_int32),
+    ]
+
+
 class OrcPTStats(ctypes.Structure):
     _fields_ = [
         ('samples', ctypes.c_int64),
@@ -104,6 +114,9 @@ def lib():
         L.orc_pt_render.restype = ctypes.c_int
         L.orc_pt_render.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_void_p, ctypes.POINTER(OrcPTParams),
                                     _f64p, ctypes.c_void_p, ctypes.POINTER(OrcPTStats)]
+        L.orc_lt_render.restype = ctypes.c_int
+        L.orc_lt_render.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_void_p, ctypes.POINTER(OrcLTParams),
+                                    _f64p, ctypes.POINTER(OrcPTStats)]
         _lib = L
     return _lib
 
@@ -248,6 +261,26 @@ def pt_render(flat, spp: int, max_verts: int, sampling_mode: int = 2, rr_depth: 
     if fsq is not None:
         out['sumsq'] = fsq.reshape(flat.h, flat.w, 3)
     return out
+
+
+def lt_render(flat, num_samples: int, max_verts: int, seed: int = 0, pass_offset: int = 0, nthreads: int = 0,
+              accel: Accel = None):
+    """renderer::lt restated in FP64 (orc_lt_render). Returns dict with 'sum'
+    ([h,w,3] float64, unscaled splat sums), 'image' (= sum * w*h /
+    num_samples, the reference's film) and 'stats'."""
+    if accel is None:
+        accel = Accel(flat.verts, nthreads)
+    s, keep = make_orc_scene(flat)
+    P = OrcLTParams(int(num_samples), int(max_verts), int(pass_offset), int(seed), int(nthreads))
+    film = np.zeros(flat.w * flat.h * 3, dtype=np.float64)
+    st = OrcPTStats()
+    rc = lib().orc_lt_render(ctypes.byref(s), accel.h, ctypes.byref(P), film, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError(f'orc_lt_render failed ({rc})')
+    f = film.reshape(flat.h, flat.w, 3)
+    return {'sum': f, 'image': f * (flat.w * flat.h / num_samples),
+            'stats': {'samples': st.samples, 'extension_rays': st.extension_rays,
+                      'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices}}
 
 
 def _mat_params(color=(1, 1, 1), ax=0.0, ay=0.0, ni=1.0):

@@ -52,6 +52,7 @@ def test_structure_sizes_match_header():
     assert ctypes.sizeof(_lib.PTParams) == 4 * 4 + 8 + 4 + 4 + 8 + 32
     assert ctypes.sizeof(_lib.PTStats) == 4 * 8 + 8 + 4 * 8
     assert ctypes.sizeof(_lib.GuideOpts) == 4 * 12
+    assert ctypes.sizeof(_lib.LTParams) == 8 + 4 + 4 + 8 + 8 + 32
 
 
 def test_host_components_built():
