@@ -58,10 +58,11 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(flat, rays, n_rays):
-    """The oracle's FP64 restatement of accel::sahbvh and renderer::pt on all
-    host cores: incoherent (the headline unit) and coherent closest-hit
-    Mrays/s on the config-1 rays, the SAH build time, and renderer::pt
-    samples/s on config 0 (256x256, 16 spp, 8 vertices)."""
+    """The oracle's FP64 restatement of accel::sahbvh, renderer::pt and
+    renderer::lt on all host cores: incoherent (the headline unit) and
+    coherent closest-hit Mrays/s on the config-1 rays, the SAH build time,
+    renderer::pt samples/s and renderer::lt light paths/s on config 0
+    (256x256, 16 samples per pixel, 8 vertices)."""
     from oracle import oracle as orc
     import numpy as np
     from lm_amd import scenes
@@ -80,12 +81,17 @@ def cpu_baseline(flat, rays, n_rays):
     t0 = time.time()
     r = orc.pt_render(f0, spp=16, max_verts=8, seed=1, nthreads=cores, sumsq=False)
     pt_s = r['stats']['samples'] / (time.time() - t0)
+    t0 = time.time()
+    r = orc.lt_render(f0, 16 * f0.w * f0.h, 8, seed=1, nthreads=cores)
+    lt_s = r['stats']['samples'] / (time.time() - t0)
     return {'value': out['incoherent'], 'unit': 'Mrays/s', 'cores': cores, 'kind': 'port',
             'sample': f'FP64 accel::sahbvh restatement (oracle), OpenMP over {n_rays} of the 2,000,000 '
                       f'incoherent config-1 rays; SAH build of 1,320,720 tris {build_s:.1f} s excluded',
             'build_s': round(build_s, 2), 'coherent_mrays_s': round(out['coherent'], 2),
             'pt_c0_samples_s': round(pt_s, 1),
-            'pt_c0_sample': 'renderer::pt restatement (oracle), config 0 256x256 16 spp, max 8 vertices, MIS'}
+            'pt_c0_sample': 'renderer::pt restatement (oracle), config 0 256x256 16 spp, max 8 vertices, MIS',
+            'lt_c0_light_paths_s': round(lt_s, 1),
+            'lt_c0_sample': 'renderer::lt restatement (oracle), config 0, 16 x 256x256 light paths, max 8 vertices'}
 
 
 def main():
@@ -181,6 +187,17 @@ def main():
                              'extension_rays': st['extension_rays'], 'shadow_rays': st['shadow_rays'],
                              'ms_per_spp': round(dt / args.pt_spp * 1e3, 2)}
         log('pt', extra['pt_1080p'])
+        # renderer::lt on the same scene: passes of 1920x1080 light paths
+        sc.render_lt(num_samples=s2.w * s2.h, max_verts=16, seed=2, film=fs)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        _, st = sc.render_lt(num_samples=args.pt_spp * s2.w * s2.h, max_verts=16, seed=2, pass_offset=1, film=fs)
+        torch.cuda.synchronize()
+        dt = time.time() - t0
+        extra['lt_1080p'] = {'light_paths_s': round(st['samples'] / dt, 1), 'passes': st['passes'],
+                             'mrays_s': round((st['extension_rays'] + st['shadow_rays']) / dt / 1e6, 1),
+                             'ms_per_pass': round(dt / st['passes'] * 1e3, 2)}
+        log('lt', extra['lt_1080p'])
 
     if not args.no_c3:
         from lm_amd.batch import Batch
@@ -224,7 +241,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log('cpu baseline (oracle)')
-        cpu = cpu_baseline(flat, rays['incoherent'], min(args.cpu_rays, 2_000_000))
+        cpu = cpu_baseline(flat, rays, min(args.cpu_rays, 2_000_000))
 
     out = {
         'metric': 'Mrays/s (incoherent) & samples/s at 1080p; equal-time MSE guided vs plain PT',
