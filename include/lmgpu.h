@@ -154,11 +154,16 @@ int lmgpu_accel_deserialize(lmgpu_accel* accel, const void* buf, size_t size, co
 
 /* ---- scene + renderer::pt ------------------------------------------------------ */
 
-/* Material type codes: material::diffuse, ::glossy, ::mirror, ::glass. */
+/* Material type codes: material::diffuse, ::glossy, ::mirror, ::glass, and
+ * material::mixture_wavefrontobj (src/model/model_wavefrontobj.cpp:340-513: a
+ * diffuse Kd (+ map_Kd) and glossy Ks/ax/ay group picked per direction by
+ * max(Kd) / (max(Kd) + max(Ks)), plus an alpha-mask pass-through component
+ * when the diffuse texture has an alpha channel). */
 #define LMGPU_MAT_DIFFUSE 0
 #define LMGPU_MAT_GLOSSY 1
 #define LMGPU_MAT_MIRROR 2
 #define LMGPU_MAT_GLASS 3
+#define LMGPU_MAT_MIXTURE 4
 
 /* Flattened scene with identity transforms (world-space vertices). */
 typedef struct lmgpu_scene_desc {
@@ -168,7 +173,8 @@ typedef struct lmgpu_scene_desc {
     const int32_t* tri_prim;   /* [N] primitive (scene node) index */
     int32_t num_prims;
     const int32_t* mat_type;   /* [P] LMGPU_MAT_* */
-    const float* mat_param;    /* [P][8]: color rgb (Kd or Ks), ax, ay, Ni, -, - */
+    const float* mat_param;    /* [P][8]: color rgb (Kd of diffuse and mixture, Ks of glossy), ax, ay, Ni,
+                                  mixture: alpha mask enabled (0 = no_alpha_mask), - */
     const float* prim_Ke;      /* [P][3] light::area radiance (unused if not a light) */
     int32_t num_area_lights;
     const int32_t* light_prims;/* [L] primitives carrying light::area, in light order */
@@ -178,6 +184,15 @@ typedef struct lmgpu_scene_desc {
     float vfov;                /* degrees */
     float aspect;              /* width / height */
     int32_t w, h;              /* film size */
+    /* material::mixture_wavefrontobj and texture::bitmap (all optional) */
+    const float* mat_ks;       /* [P][3] Ks of LMGPU_MAT_MIXTURE materials (NULL when there is none) */
+    const int32_t* mat_tex;    /* [P] texture of the material's Kd (map_Kd; diffuse and mixture), -1 = none */
+    const float* uvs;          /* [N][6] texture coordinates t1,t2,t3 (Mesh::surface_point's t) */
+    int32_t num_textures;
+    const int32_t* tex_info;   /* [T][4] width, height, channels (3 or 4), offset into tex_data (floats) */
+    const float* tex_data;     /* texels as texture::bitmap holds them: row-major, row 0 = the first row
+                                  after its vertical flip, `channels` floats per texel */
+    int64_t tex_data_len;      /* floats in tex_data */
 } lmgpu_scene_desc;
 
 typedef struct lmgpu_scene lmgpu_scene;

@@ -348,7 +348,7 @@ int occupancy_grid(K kernel, int block) {
 
 // Host copy of one job (the caller's buffers are only valid during add_job).
 struct BatchJob {
-    std::vector<float> verts, normals, mat_param, prim_Ke;
+    std::vector<float> verts, normals, mat_param, prim_Ke, mat_ks;
     std::vector<int32_t> tri_prim, mat_type, light_prims;
     lmgpu_scene_desc desc{};
     int spp = 1, max_verts = 2, mode = 2, rr_depth = 5;
@@ -706,6 +706,12 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     if (job->sampling_mode < 0 || job->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
     if (d.num_area_lights < 0 || (d.num_area_lights > 0 && !d.light_prims))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad light list");
+    if (d.num_textures != 0 || d.uvs || d.tex_info || d.tex_data)
+        return fail(LMGPU_ERR_UNSUPPORTED, "textured scenes are not supported by the batched renderer");
+    if (d.mat_tex)
+        for (int i = 0; i < d.num_prims; i++)
+            if (d.mat_tex[i] >= 0)
+                return fail(LMGPU_ERR_UNSUPPORTED, "textured scenes are not supported by the batched renderer");
     lmgpu::BatchJob J;
     const int64_t N = d.num_tris;
     const int P = d.num_prims;
@@ -717,6 +723,7 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     if (d.prim_Ke) J.prim_Ke.assign(d.prim_Ke, d.prim_Ke + 3 * P);
     else J.prim_Ke.assign(3 * P, 0.f);
     if (d.num_area_lights > 0) J.light_prims.assign(d.light_prims, d.light_prims + d.num_area_lights);
+    if (d.mat_ks) J.mat_ks.assign(d.mat_ks, d.mat_ks + 3 * P);
     J.desc = d;
     J.spp = job->spp;
     J.max_verts = job->max_verts;
@@ -733,6 +740,8 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     S.desc.mat_param = S.mat_param.data();
     S.desc.prim_Ke = S.prim_Ke.data();
     S.desc.light_prims = S.light_prims.empty() ? nullptr : S.light_prims.data();
+    S.desc.mat_ks = S.mat_ks.empty() ? nullptr : S.mat_ks.data();
+    S.desc.mat_tex = nullptr;
     b->b.arena_width = 0;   // the arena must be rebuilt
     b->b.have_results = false;
     if (job_id) *job_id = int32_t(b->b.jobs.size() - 1);

@@ -164,10 +164,10 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                                              mk(st.n2.x, st.n2.y, st.n2.z) * b2);
                     const Geom g = make_on_surface(pos, nsh, normalize(cross(P1 - P0, P2 - P0)));
                     int light;
-                    const Mat m = load_mat(sc.mats, __float_as_int(st.p0.w), light);
+                    const Mat m = load_mat(sc, __float_as_int(st.p0.w), tri, b1, b2, light);
                     const f3 wi = mk(-rb.x, -rb.y, -rb.z);   // toward the light-side vertex
-                    float cw;
-                    const int comp = mat_sample_component(m, g, wi, r0.u[1], cw);
+                    float cw, ul;
+                    const int comp = mat_sample_component(m, g, wi, r0.u[1], cw, ul);
                     thr = thr * cw;
                     if (nv < c.max_verts) {
                         // camera connection; woc = direction from the camera to the point
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                             if (rx >= 0.f && rx <= 1.f && ry >= 0.f && ry <= 1.f && G > 0.f) {
                                 const float ct = -ez;
                                 const float J = 1.f / (ct * ct * ct * (4.f * cam.tf * cam.tf * cam.aspect));
-                                const f3 fs = mat_eval(m, g, wi, wo) * sn_correction_le(g, wi, wo);
+                                const f3 fs = mat_eval(m, g, wi, wo, comp) * sn_correction_le(g, wi, wo);
                                 const f3 Cc = thr * fs * (J * G);
                                 if (!is_zero(Cc)) {
                                     const int px = min(max(int(rx * float(sc.w)), 0), sc.w - 1);
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                     }
                     if (nv + 1 < c.max_verts) {
                         f3 wo2, sw;
-                        if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw)) {
+                        if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], ul, wo2, sw)) {
                             if (m.type == LMGPU_MAT_GLASS && comp == 1) {
                                 // refraction carries (1 - Fr) without eta^2 under TransDir::LE
                                 f3 wt;

@@ -305,8 +305,10 @@ __global__ __launch_bounds__(kBlock) void k_raycast(SceneDev sc, unsigned long l
                 C = mk(fabsf(nsh.x), fabsf(nsh.y), fabsf(nsh.z));
             } else {
                 int light;
-                const Mat m = load_mat(sc.mats, __float_as_int(st.p0.w), light);
-                const bool refl = m.type == LMGPU_MAT_DIFFUSE || m.type == LMGPU_MAT_GLOSSY;
+                const Mat m = load_mat(sc, __float_as_int(st.p0.w), int(r.prim), b1, b2, light);
+                // Material::reflectance: Kd (texture) of diffuse and mixture, Ks of glossy, 0 otherwise
+                const bool refl = m.type == LMGPU_MAT_DIFFUSE || m.type == LMGPU_MAT_GLOSSY ||
+                                  m.type == LMGPU_MAT_MIXTURE;
                 C = rc.use_color ? rc.color : (refl ? m.c : mk(0, 0, 0));
                 if (!rc.constant) C = C * (.2f + .8f * fabsf(dot(nsh, -dir_of(id))));
             }
@@ -381,7 +383,33 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
         if (d->tri_prim[t] < 0 || d->tri_prim[t] >= P)
             return fail(LMGPU_ERR_INVALID_ARGUMENT, "tri_prim out of range");
     for (int i = 0; i < P; i++)
-        if (d->mat_type[i] < 0 || d->mat_type[i] > 3) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad material type");
+        if (d->mat_type[i] < 0 || d->mat_type[i] > LMGPU_MAT_MIXTURE)
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad material type");
+    // textures (texture::bitmap): validated before anything is uploaded
+    const int T = d->num_textures;
+    if (T < 0 || (T > 0 && (!d->tex_info || !d->tex_data)))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad texture list");
+    int64_t texel_count = 0;
+    for (int t = 0; t < T; t++) {
+        const int32_t* ti = d->tex_info + 4 * t;
+        if (ti[0] <= 0 || ti[1] <= 0 || (ti[2] != 3 && ti[2] != 4) || ti[3] < 0 ||
+            int64_t(ti[3]) + int64_t(ti[0]) * ti[1] * ti[2] > d->tex_data_len)
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "texture " + std::to_string(t) + ": bad size or offset");
+        texel_count += int64_t(ti[0]) * ti[1];
+    }
+    bool textured = false;
+    if (d->mat_tex)
+        for (int i = 0; i < P; i++) {
+            const int tx = d->mat_tex[i];
+            if (tx < -1 || tx >= T) return fail(LMGPU_ERR_INVALID_ARGUMENT, "material texture out of range");
+            if (tx >= 0 && d->mat_type[i] != LMGPU_MAT_DIFFUSE && d->mat_type[i] != LMGPU_MAT_MIXTURE)
+                return fail(LMGPU_ERR_INVALID_ARGUMENT, "only diffuse and mixture materials take a texture");
+            textured = textured || tx >= 0;
+        }
+    if (textured && !d->uvs) return fail(LMGPU_ERR_INVALID_ARGUMENT, "textured materials need texture coordinates");
+    for (int i = 0; i < P; i++)
+        if (d->mat_type[i] == LMGPU_MAT_MIXTURE && !d->mat_ks)
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "mixture materials need mat_ks");
     LMGPU_TRY(accel.build(d->verts, N, false, opts, nullptr));
     refill_secondary = accel.refill;
     w = d->w;
@@ -448,9 +476,17 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
     std::vector<DevMat> hm(P);
     for (int i = 0; i < P; i++) {
         const float* mp = d->mat_param + 8 * i;
-        hm[i].a = make_float4(mp[0], mp[1], mp[2], float(d->mat_type[i]));
+        const int type = d->mat_type[i];
+        hm[i].a = make_float4(mp[0], mp[1], mp[2], float(type));
         hm[i].b = make_float4(mp[3], mp[4], mp[5], float(prim_light[i]));
-        if (d->mat_type[i] == LMGPU_MAT_GLOSSY && (mp[3] <= 0.f || mp[4] <= 0.f))
+        const float* ks = type == LMGPU_MAT_MIXTURE ? d->mat_ks + 3 * i : nullptr;
+        int32_t tx = d->mat_tex ? d->mat_tex[i] : -1;
+        // the mixture's alpha mask: a texture with an alpha channel, unless disabled
+        if (tx >= 0 && type == LMGPU_MAT_MIXTURE && d->tex_info[4 * tx + 2] == 4 && mp[6] != 0.f) tx |= kTexAlphaBit;
+        float txb;
+        std::memcpy(&txb, &tx, 4);
+        hm[i].c = make_float4(f_or(ks, 0), f_or(ks, 1), f_or(ks, 2), txb);
+        if ((type == LMGPU_MAT_GLOSSY || type == LMGPU_MAT_MIXTURE) && (mp[3] <= 0.f || mp[4] <= 0.f))
             return fail(LMGPU_ERR_INVALID_ARGUMENT, "glossy roughness must be positive");
         if (d->mat_type[i] == LMGPU_MAT_GLASS && !(mp[5] > 0.f))
             return fail(LMGPU_ERR_INVALID_ARGUMENT, "glass Ni must be positive");
@@ -465,6 +501,32 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
     LMGPU_HIP(hipMemcpy(light_cdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
     LMGPU_TRY(light_tri.alloc(ltri.size()));
     LMGPU_HIP(hipMemcpy(light_tri.p, ltri.data(), sizeof(int32_t) * ltri.size(), hipMemcpyHostToDevice));
+    // texture coordinates and texels (RGBA, A = 1 for RGB textures)
+    num_textures = T;
+    if (textured) {
+        std::vector<float2> huv(3 * size_t(N));
+        for (int64_t k = 0; k < 3 * N; k++) huv[k] = make_float2(d->uvs[2 * k], d->uvs[2 * k + 1]);
+        LMGPU_TRY(uvs.alloc(huv.size()));
+        LMGPU_HIP(hipMemcpy(uvs.p, huv.data(), sizeof(float2) * huv.size(), hipMemcpyHostToDevice));
+        std::vector<float4> htx(size_t(std::max<int64_t>(texel_count, 1)));
+        std::vector<int4> hti(T);
+        int64_t at = 0;
+        for (int t = 0; t < T; t++) {
+            const int32_t* ti = d->tex_info + 4 * t;
+            const float* src = d->tex_data + ti[3];
+            const int c = ti[2];
+            const int64_t n = int64_t(ti[0]) * ti[1];
+            for (int64_t k = 0; k < n; k++)
+                htx[at + k] = make_float4(src[c * k], src[c * k + 1], src[c * k + 2], c == 4 ? src[c * k + 3] : 1.f);
+            if (at > 0x7fffffff) return fail(LMGPU_ERR_INVALID_ARGUMENT, "textures too large");
+            hti[t] = make_int4(ti[0], ti[1], c, int(at));
+            at += n;
+        }
+        LMGPU_TRY(texels.alloc(htx.size()));
+        LMGPU_HIP(hipMemcpy(texels.p, htx.data(), sizeof(float4) * htx.size(), hipMemcpyHostToDevice));
+        LMGPU_TRY(tex_info.alloc(hti.size()));
+        LMGPU_HIP(hipMemcpy(tex_info.p, hti.data(), sizeof(int4) * hti.size(), hipMemcpyHostToDevice));
+    }
     // camera::pinhole construct (camera_pinhole.cpp:86-96)
     Camera cam;
     {
@@ -495,6 +557,9 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
     dev.lights = lights.p;
     dev.light_cdf = light_cdf.p;
     dev.light_tri = light_tri.p;
+    dev.uvs = uvs.p;
+    dev.texels = texels.p;
+    dev.tex_info = tex_info.p;
     dev.num_area_lights = L;
     dev.num_lights = L + (d->has_env ? 1 : 0);
     dev.has_env = d->has_env ? 1 : 0;

@@ -18,9 +18,12 @@ struct alignas(16) ShadeTri {
 };
 
 struct alignas(16) DevMat {
-    float4 a;   // rgb, type (as float)
+    float4 a;   // rgb (Kd; Ks of glossy), type (as float)
     float4 b;   // ax, ay, ni, light index (as float, -1 = none)
+    float4 c;   // mixture Ks rgb; w = int bits: texture of Kd (-1 = none), bit 30 = alpha mask
 };
+
+constexpr int kTexAlphaBit = 1 << 30;
 
 struct alignas(16) DevLight {
     float4 Ke_invA;   // Ke rgb, 1/area
@@ -37,6 +40,9 @@ struct SceneDev {
     const DevLight* lights;
     const float* light_cdf;    // per light: count+1 entries, c[0] = 0, normalised
     const int32_t* light_tri;  // per light: original triangle indices (face order)
+    const float2* uvs;         // [3N] texture coordinates per triangle corner (textured scenes only)
+    const float4* texels;      // all textures' texels, RGBA (A = 1 for RGB textures)
+    const int4* tex_info;      // per texture: width, height, channels, first texel
     int num_area_lights;
     int num_lights;            // area + env
     int has_env;
@@ -99,15 +105,37 @@ constexpr int kBlock = 256;
 constexpr int kLds = 24;
 constexpr int kPriv = 72;
 
-__device__ __forceinline__ Mat load_mat(const DevMat* mats, int prim, int& light) {
-    const DevMat d = mats[prim];
+// Material of primitive `prim` at barycentrics (b1, b2) of triangle `tri`.
+// A textured Kd is texture::bitmap::eval at the interpolated texture
+// coordinate (Mesh_Raw::surface_point, mesh_raw.cpp:87-97): wrapped to
+// [0,1), nearest texel (texture_bitmap.cpp:68-75); a mixture with an alpha
+// mask also takes the texel's alpha (texture_bitmap.cpp:82-89).
+__device__ __forceinline__ Mat load_mat(const SceneDev& sc, int prim, int tri, float b1, float b2, int& light) {
+    const DevMat d = sc.mats[prim];
     Mat m;
     m.type = int(d.a.w);
     m.c = mk(d.a.x, d.a.y, d.a.z);
     m.ax = d.b.x;
     m.ay = d.b.y;
     m.ni = d.b.z;
+    m.ks = mk(d.c.x, d.c.y, d.c.z);
+    m.alpha = 1.f;
     light = int(d.b.w);
+    const int tx = __float_as_int(d.c.w);
+    if (tx >= 0) {
+        const float2 t0 = sc.uvs[3 * int64_t(tri)], t1 = sc.uvs[3 * int64_t(tri) + 1], t2 = sc.uvs[3 * int64_t(tri) + 2];
+        const float b0 = 1.f - b1 - b2;
+        float u = t0.x * b0 + t1.x * b1 + t2.x * b2;
+        float v = t0.y * b0 + t1.y * b1 + t2.y * b2;
+        u -= floorf(u);
+        v -= floorf(v);
+        const int4 ti = sc.tex_info[tx & ~kTexAlphaBit];
+        const int x = min(max(int(u * float(ti.x)), 0), ti.x - 1);
+        const int y = min(max(int(v * float(ti.y)), 0), ti.y - 1);
+        const float4 tv = sc.texels[ti.w + int64_t(y) * ti.x + x];
+        m.c = mk(tv.x, tv.y, tv.z);
+        if (tx & kTexAlphaBit) m.alpha = tv.w;
+    }
     return m;
 }
 
@@ -236,7 +264,7 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
         const Geom g = make_on_surface(pos, nsh, gn);
         const int prim = __float_as_int(st.p0.w);
         int light;
-        const Mat m = load_mat(sc.mats, prim, light);
+        const Mat m = load_mat(sc, prim, tri, b1, b2, light);
         // emitter hit (renderer_pt.cpp:246-271)
         if (light >= 0 && direct_ok) {
             const f3 woL = -wo;
@@ -262,14 +290,15 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
         }
         const f3 wi = -wo;
         int comp = 0;
+        float ul = 0.f;   // lobe selection of a mixture
         if (alive) {
             float cw;
-            comp = mat_sample_component(m, g, wi, r0.u[1], cw);
+            comp = mat_sample_component(m, g, wi, r0.u[1], cw, ul);
             thr = thr * cw;
         }
         const int nv2 = nv + 1;
         if (alive && nv2 < c.max_verts) {
-            const bool spec = is_specular(m);
+            const bool spec = is_specular(m, comp);
             const bool nee = c.mode != LMGPU_SAMPLING_NAIVE && !spec;
             // guide mixture of this queue entry (guided vertices only)
             const float* gmix = (GUIDED && !spec) ? gmix_entry : nullptr;
@@ -331,11 +360,11 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
                     }
                 }
                 if (ok) {
-                    const f3 fs = mat_eval(m, g, wi, dirL) * sn_correction(g, wi, dirL);
+                    const f3 fs = mat_eval(m, g, wi, dirL, comp) * sn_correction(g, wi, dirL);
                     if (!is_zero(fs)) {
                         float mis = 1.f;
                         if (c.mode == LMGPU_SAMPLING_MIS) {
-                            float pdir = mat_pdf(m, g, wi, dirL);
+                            float pdir = mat_pdf(m, g, wi, dirL, comp);
                             if (GUIDED && gmix) {
                                 float pg = mixture_pdf(gmix, dirL);
                                 if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
@@ -366,22 +395,22 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
                     wo2 = mixture_sample(gmix, r2.u[1], r2.u[2], r2.u[3]);
                     sampled = true;
                 } else {
-                    sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
+                    sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], ul, wo2, sw);
                 }
                 sw = mk(0, 0, 0);
                 if (sampled) {
                     const float cs = fabsf(dot(wo2, g.n));
                     float pg = mixture_pdf(gmix, wo2);
                     if (!(pg >= 0.f && pg < 3e38f)) pg = 0.f;
-                    pmix = gd.alpha * pg + (1.f - gd.alpha) * mat_pdf(m, g, wi, wo2) * cs;
+                    pmix = gd.alpha * pg + (1.f - gd.alpha) * mat_pdf(m, g, wi, wo2, comp) * cs;
                     if (pmix > 0.f && cs > 0.f) {
-                        sw = mat_eval(m, g, wi, wo2) * (cs / pmix);
+                        sw = mat_eval(m, g, wi, wo2, comp) * (cs / pmix);
                         pdir = pmix / cs;
                     }
                 }
             } else {
-                sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], wo2, sw);
-                if (sampled) pdir = mat_pdf(m, g, wi, wo2);
+                sampled = mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], ul, wo2, sw);
+                if (sampled) pdir = mat_pdf(m, g, wi, wo2, comp);
             }
             if (sampled) {
                 sw = sw * sn_correction(g, wi, wo2);

@@ -30,6 +30,10 @@ struct Scene {
     DevBuf<DevLight> lights;
     DevBuf<float> light_cdf;
     DevBuf<int32_t> light_tri;
+    DevBuf<float2> uvs;      // textured scenes: texture coordinates per triangle corner
+    DevBuf<float4> texels;
+    DevBuf<int4> tex_info;
+    int num_textures = 0;
     SceneDev dev{};
     int w = 0, h = 0;
     // render workspace

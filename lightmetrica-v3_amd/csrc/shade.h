@@ -13,6 +13,7 @@
 //   material::glossy (aniso GGX)    src/material/material_glossy.cpp:43-113
 //   material::mirror                src/material/material_mirror.cpp:33-62
 //   material::glass (Schlick)       src/material/material_glass.cpp:85-286
+//   material::mixture_wavefrontobj  src/model/model_wavefrontobj.cpp:340-513
 //   light::area, light::envconst    src/light/light_area.cpp:40-187,
 //                                   src/light/light_envconst.cpp:100-141
 //   Scene::sample_light_selection   src/scene.cpp:374-387 (uniform)
@@ -99,8 +100,10 @@ __device__ __forceinline__ float balance_heuristic(float p1, float p2) {
 
 struct Mat {
     int type;       // LMGPU_MAT_*
-    f3 c;           // Kd or Ks
+    f3 c;           // Kd (diffuse, mixture: map_Kd's texel when textured) or Ks (glossy)
     float ax, ay, ni;
+    f3 ks;          // mixture: Ks of the glossy lobe
+    float alpha;    // mixture: alpha mask value at the hit (1 without a mask)
 };
 
 __device__ __forceinline__ float sq(float x) { return x * x; }
@@ -151,26 +154,71 @@ __device__ __forceinline__ float glass_Fr(const Mat& m, const Geom& g, f3 wi, f3
     return r2 + (1.f - r2) * (x * x * x * x * x);
 }
 
-__device__ __forceinline__ bool is_specular(const Mat& m) { return m.type >= LMGPU_MAT_MIRROR; }
+// Material::is_specular_component: mirror and glass always, the mixture's
+// alpha pass-through component (1).
+__device__ __forceinline__ bool is_specular(const Mat& m, int comp) {
+    return m.type == LMGPU_MAT_MIRROR || m.type == LMGPU_MAT_GLASS || (m.type == LMGPU_MAT_MIXTURE && comp != 0);
+}
 
-// Material::eval with eval_delta = true (renderer_pt.cpp:162).
-__device__ __forceinline__ f3 mat_eval(const Mat& m, const Geom& g, f3 wi, f3 wo) {
-    if (m.type == LMGPU_MAT_DIFFUSE) return opposite(g, wi, wo) ? mk(0, 0, 0) : m.c * (1.f / kPi);
+// The glossy lobe of a mixture as a material::glossy.
+__device__ __forceinline__ Mat mixture_glossy(const Mat& m) {
+    Mat gm = m;
+    gm.type = LMGPU_MAT_GLOSSY;
+    gm.c = m.ks;
+    return gm;
+}
+
+__device__ __forceinline__ f3 diffuse_eval(const Mat& m, const Geom& g, f3 wi, f3 wo) {
+    return opposite(g, wi, wo) ? mk(0, 0, 0) : m.c * (1.f / kPi);
+}
+
+// Probability of the diffuse lobe of a mixture (diffuse_selection_prob):
+// max(Kd) / (max(Kd) + max(Ks)), 1 when both are 0.
+__device__ __forceinline__ float mixture_diffuse_prob(const Mat& m) {
+    const float wd = maxc(m.c), wg = maxc(m.ks);
+    return (wd == 0.f && wg == 0.f) ? 1.f : wd / (wd + wg);
+}
+
+// Material::eval of component `comp` with eval_delta = true (renderer_pt.cpp:162).
+__device__ __forceinline__ f3 mat_eval(const Mat& m, const Geom& g, f3 wi, f3 wo, int comp) {
+    if (m.type == LMGPU_MAT_DIFFUSE) return diffuse_eval(m, g, wi, wo);
     if (m.type == LMGPU_MAT_GLOSSY) return glossy_eval(m, g, wi, wo);
+    if (m.type == LMGPU_MAT_MIXTURE && comp == 0)
+        return (diffuse_eval(m, g, wi, wo) + glossy_eval(mixture_glossy(m), g, wi, wo)) * m.alpha;
     return mk(0, 0, 0);
 }
 
-// Material::pdf_direction with eval_delta = true, projected solid angle.
-__device__ __forceinline__ float mat_pdf(const Mat& m, const Geom& g, f3 wi, f3 wo) {
+// Material::pdf_direction of component `comp` with eval_delta = true,
+// projected solid angle (the mixture's is the lobe-selection marginal).
+__device__ __forceinline__ float mat_pdf(const Mat& m, const Geom& g, f3 wi, f3 wo, int comp) {
     if (m.type == LMGPU_MAT_DIFFUSE) return opposite(g, wi, wo) ? 0.f : 1.f / kPi;
     if (m.type == LMGPU_MAT_GLOSSY) return glossy_pdf(m, g, wi, wo);
+    if (m.type == LMGPU_MAT_MIXTURE && comp == 0) {
+        const float w = mixture_diffuse_prob(m);
+        return w * (opposite(g, wi, wo) ? 0.f : 1.f / kPi) + (1.f - w) * glossy_pdf(mixture_glossy(m), g, wi, wo);
+    }
     return 0.f;
 }
 
 // Material::sample_component: glass picks reflection (0) / refraction (1)
-// with probability Fr / 1-Fr and weight 1/p; the others {0, 1}.
-__device__ __forceinline__ int mat_sample_component(const Mat& m, const Geom& g, f3 wi, float uc, float& w) {
+// with probability Fr / 1-Fr, the mixture its diffuse+glossy group (0) /
+// alpha pass-through (1) with probability alpha / 1-alpha, each with weight
+// 1/p; the others {0, 1}. `urest` is uc rescaled to [0,1) within the chosen
+// interval, independent of the choice (the mixture's lobe selection).
+__device__ __forceinline__ int mat_sample_component(const Mat& m, const Geom& g, f3 wi, float uc, float& w,
+                                                    float& urest) {
     w = 1.f;
+    urest = uc;
+    if (m.type == LMGPU_MAT_MIXTURE) {
+        if (uc < m.alpha) {
+            w = 1.f / m.alpha;
+            urest = uc / m.alpha;
+            return 0;
+        }
+        w = 1.f / (1.f - m.alpha);
+        urest = (uc - m.alpha) / (1.f - m.alpha);
+        return 1;
+    }
     if (m.type != LMGPU_MAT_GLASS) return 0;
     f3 wt;
     float eta;
@@ -180,9 +228,13 @@ __device__ __forceinline__ int mat_sample_component(const Mat& m, const Geom& g,
     return comp;
 }
 
-// Material::sample_direction (without the shading-normal correction).
+// Material::sample_direction (without the shading-normal correction). `ul`
+// selects the mixture's lobe.
 __device__ __forceinline__ bool mat_sample_direction(const Mat& m, const Geom& g, f3 wi, int comp, float u0, float u1,
-                                                     f3& wo, f3& weight) {
+                                                     float ul, f3& wo, f3& weight);
+
+__device__ __forceinline__ bool mat_sample_direction_basic(const Mat& m, const Geom& g, f3 wi, int comp, float u0,
+                                                           float u1, f3& wo, f3& weight) {
     if (m.type == LMGPU_MAT_DIFFUSE) {
         f3 n, u, v;
         twosided(g, wi, n, u, v);
@@ -223,6 +275,26 @@ __device__ __forceinline__ bool mat_sample_direction(const Mat& m, const Geom& g
         const float w = (1.f - Fr) * eta * eta;
         weight = mk(w, w, w);
     }
+    return true;
+}
+
+__device__ __forceinline__ bool mat_sample_direction(const Mat& m, const Geom& g, f3 wi, int comp, float u0, float u1,
+                                                     float ul, f3& wo, f3& weight) {
+    if (m.type != LMGPU_MAT_MIXTURE) return mat_sample_direction_basic(m, g, wi, comp, u0, u1, wo, weight);
+    if (comp != 0) {
+        // alpha pass-through; the reference returns weight 1 here
+        wo = -wi;
+        weight = mk(1, 1, 1);
+        return true;
+    }
+    // pick the diffuse or the glossy lobe, sample it, weight = f / marginal pdf
+    Mat lobe = m;
+    if (ul < mixture_diffuse_prob(m)) lobe.type = LMGPU_MAT_DIFFUSE;
+    else lobe = mixture_glossy(m);
+    f3 lw;
+    if (!mat_sample_direction_basic(lobe, g, wi, 0, u0, u1, wo, lw)) return false;
+    const float p = mat_pdf(m, g, wi, wo, 0);
+    weight = p > 0.f ? mat_eval(m, g, wi, wo, 0) / p : mk(0, 0, 0);
     return true;
 }
 

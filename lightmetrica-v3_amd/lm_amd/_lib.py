@@ -91,6 +91,13 @@ class SceneDesc(ctypes.Structure):
         ('aspect', ctypes.c_float),
         ('w', ctypes.c_int32),
         ('h', ctypes.c_int32),
+        ('mat_ks', ctypes.c_void_p),
+        ('mat_tex', ctypes.c_void_p),
+        ('uvs', ctypes.c_void_p),
+        ('num_textures', ctypes.c_int32),
+        ('tex_info', ctypes.c_void_p),
+        ('tex_data', ctypes.c_void_p),
+        ('tex_data_len', ctypes.c_int64),
     ]
 
 

@@ -8,15 +8,23 @@ texture and normal indices may be negative (relative). Materials follow the
 reference's mapping (model_wavefrontobj.cpp:155-215):
   illum 7          -> material::glass (Ni)
   illum 5          -> material::mirror
-  Ks == 0          -> material::diffuse (Kd)
-  otherwise        -> material::glossy (Ks, ax = r / as, ay = r * as with
-                      r = 2 / (2 + Ns), as = sqrt(1 - 0.9 an), both >= 1e-3)
-The reference's default for the last case is the Kd + Ks mixture
-(material::mixture_wavefrontobj), which the device materials do not have; its
-NO_MIXTURE_MATERIAL branch (diffuse when Ks == 0, else glossy) is what is
-restated here. Ke > 0 makes the group an area light (light::area). Texture
-maps (map_Kd) are not supported: a material that names one raises an error
-unless `ignore_textures` is set (then Kd is used).
+  (both black material::diffuse with skip_specular_mat)
+  otherwise        -> material::mixture_wavefrontobj: diffuse Kd (map_Kd)
+                      + glossy Ks with ax = r / as, ay = r * as,
+                      r = 2 / (2 + Ns), as = sqrt(1 - 0.9 an), both >= 1e-3;
+                      the map's alpha channel masks it unless
+                      skip_specular_mat
+With no_mixture (the reference's NO_MIXTURE_MATERIAL build switch) the last
+case is material::diffuse when Ks == 0, else material::glossy. Ke > 0 makes
+the group an area light (light::area).
+
+map_Kd loads as texture::bitmap does through stbi_loadf
+(texture_bitmap.cpp:50-66): 8-bit images become floats (c / 255)^2.2 on the
+colour channels and c / 255 on alpha, rows flipped vertically (flip = true);
+grey images are expanded to RGB (the reference reads 3 channels from every
+texel, which for 1- and 2-channel images reads neighbouring texels). Loading
+needs Pillow; a texture path that does not load is an error unless
+`ignore_textures` is set (then Kd is used).
 """
 from __future__ import annotations
 
@@ -79,33 +87,63 @@ def load_mtl(path: str) -> Dict[str, MtlParams]:
     return mats
 
 
-def material_of(m: MtlParams, ignore_textures: bool = False) -> scenes.Material:
-    """Reference mapping of MTL parameters to a device material."""
-    if m.map_Kd and not ignore_textures:
-        raise NotImplementedError(f'material {m.name!r}: texture maps (map_Kd) are not supported')
-    if m.illum == 7:
-        return scenes.Material(scenes.MAT_GLASS, ni=float(m.Ni))
-    if m.illum == 5:
+def load_texture(path: str, flip: bool = True) -> scenes.Texture:
+    """texture::bitmap from an image file, with stbi_loadf's conversion of
+    8-bit data (colour (c/255)^2.2, alpha c/255) and its vertical flip."""
+    try:
+        from PIL import Image
+    except ImportError as e:   # pragma: no cover - Pillow is part of the environment
+        raise NotImplementedError('loading texture images needs Pillow') from e
+    with Image.open(path) as im:
+        has_alpha = im.mode in ('RGBA', 'LA', 'PA') or (im.mode == 'P' and 'transparency' in im.info)
+        im = im.convert('RGBA' if has_alpha else 'RGB')
+        a = np.asarray(im, dtype=np.float64) / 255.0
+    a = a.copy()
+    a[..., :3] = a[..., :3] ** 2.2
+    if flip:
+        a = a[::-1]
+    return scenes.Texture(np.ascontiguousarray(a, dtype=np.float32))
+
+
+def material_of(m: MtlParams, ignore_textures: bool = False, texture: Optional[scenes.Texture] = None,
+                no_mixture: bool = False, skip_specular_mat: bool = False) -> scenes.Material:
+    """Reference mapping of MTL parameters to a device material
+    (model_wavefrontobj.cpp:155-215); `texture` is the loaded map_Kd."""
+    if m.map_Kd and texture is None and not ignore_textures:
+        raise NotImplementedError(f'material {m.name!r}: map_Kd {m.map_Kd!r} was not loaded')
+    if m.illum in (5, 7):
+        if skip_specular_mat:
+            return scenes.Material(scenes.MAT_DIFFUSE, (0.0, 0.0, 0.0))
+        if m.illum == 7:
+            return scenes.Material(scenes.MAT_GLASS, ni=float(m.Ni))
         return scenes.Material(scenes.MAT_MIRROR)
-    if max(m.Ks) == 0.0:
-        return scenes.Material(scenes.MAT_DIFFUSE, tuple(m.Kd))
     r = 2.0 / (2.0 + m.Ns)
     a_s = math.sqrt(max(0.0, 1.0 - m.an * 0.9))
-    return scenes.Material(scenes.MAT_GLOSSY, tuple(m.Ks), max(1e-3, r / a_s), max(1e-3, r * a_s))
+    ax, ay = max(1e-3, r / a_s), max(1e-3, r * a_s)
+    if no_mixture:
+        if max(m.Ks) == 0.0:
+            return scenes.Material(scenes.MAT_DIFFUSE, tuple(m.Kd), texture=texture)
+        return scenes.Material(scenes.MAT_GLOSSY, tuple(m.Ks), ax, ay)
+    return scenes.Material(scenes.MAT_MIXTURE, tuple(m.Kd), ax, ay, ks=tuple(m.Ks), texture=texture,
+                           alpha_mask=not skip_specular_mat)
 
 
 @dataclass
 class _Group:
     material: str
-    corners: List[Tuple[int, int]] = field(default_factory=list)   # (position index, normal index or -1)
+    # (position index, normal index or -1, texture coordinate index or -1)
+    corners: List[Tuple[int, int, int]] = field(default_factory=list)
 
 
-def load_obj(path: str, ignore_textures: bool = False) -> List[scenes.Primitive]:
-    """Primitives of an OBJ file (one per face group of one material)."""
+def load_obj(path: str, ignore_textures: bool = False, no_mixture: bool = False,
+             skip_specular_mat: bool = False) -> List[scenes.Primitive]:
+    """Primitives of an OBJ file (one per face group of one material). The
+    options are model::wavefrontobj's skip_specular_mat and the
+    NO_MIXTURE_MATERIAL switch; textures are shared per file name."""
     base = os.path.dirname(os.path.abspath(path))
     ps: List[Tuple[float, float, float]] = []
     ns: List[Tuple[float, float, float]] = []
-    nt = 0
+    ts: List[Tuple[float, float]] = []
     mats: Dict[str, MtlParams] = {}
     groups: List[_Group] = []
     cur_mtl = ''
@@ -126,7 +164,7 @@ def load_obj(path: str, ignore_textures: bool = False) -> List[scenes.Primitive]
             elif k == 'vn':
                 ns.append(_floats(a, 3))
             elif k == 'vt':
-                nt += 1
+                ts.append(_floats(a, 2))
             elif k == 'mtllib':
                 mats.update(load_mtl(os.path.join(base, ' '.join(a))))
             elif k == 'usemtl':
@@ -144,14 +182,19 @@ def load_obj(path: str, ignore_textures: bool = False) -> List[scenes.Primitive]
                 for v in a:
                     parts = v.split('/')
                     pi = index(parts[0], len(ps))
+                    # a texture index in a file without vt lines is ignored
+                    ti = index(parts[1], len(ts)) if len(parts) > 1 and parts[1] and ts else -1
                     ni = index(parts[2], len(ns)) if len(parts) > 2 and parts[2] else -1
-                    if not 0 <= pi < len(ps) or (ni != -1 and not 0 <= ni < len(ns)):
+                    if (not 0 <= pi < len(ps) or (ni != -1 and not 0 <= ni < len(ns))
+                            or (ti != -1 and not 0 <= ti < len(ts))):
                         raise ValueError(f'{path}: face index out of range')
-                    corner.append((pi, ni))
+                    corner.append((pi, ni, ti))
                 for j in range(1, len(corner) - 1):   # fan
                     cur.corners += [corner[0], corner[j], corner[j + 1]]
     P = np.asarray(ps, dtype=np.float64).reshape(-1, 3)
     N = np.asarray(ns, dtype=np.float64).reshape(-1, 3)
+    TC = np.asarray(ts, dtype=np.float64).reshape(-1, 2)
+    textures: Dict[str, scenes.Texture] = {}
     prims: List[scenes.Primitive] = []
     for g in groups:
         if not g.corners:
@@ -167,14 +210,32 @@ def load_obj(path: str, ignore_textures: bool = False) -> List[scenes.Primitive]
             vn = N[c[has, 1]]
             vn = vn / np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-30)
             nrm[has] = vn
+        uv = np.zeros((c.shape[0], 2))
+        has_t = c[:, 2] >= 0
+        if has_t.any():
+            uv[has_t] = TC[c[has_t, 2]]
         mesh = scenes.Mesh(tri.reshape(-1, 3).astype(np.float32), nrm.astype(np.float32),
-                           np.arange(tri.shape[0] * 3, dtype=np.int32).reshape(-1, 3))
+                           np.arange(tri.shape[0] * 3, dtype=np.int32).reshape(-1, 3), uv.astype(np.float32))
         m = mats.get(g.material, MtlParams(g.material or 'default', Kd=(0.8, 0.8, 0.8)))
+        tex = None
+        if m.map_Kd and m.illum not in (5, 7):
+            # texture_<stem>: one texture per file name (model_wavefrontobj.cpp:133-153)
+            key = os.path.splitext(os.path.basename(m.map_Kd))[0]
+            if key not in textures:
+                try:
+                    textures[key] = load_texture(os.path.join(base, m.map_Kd))
+                except (OSError, NotImplementedError):
+                    if not ignore_textures:
+                        raise
+            tex = textures.get(key)
         Ke = tuple(m.Ke) if max(m.Ke) > 0 else None
-        prims.append(scenes.Primitive(mesh, material_of(m, ignore_textures), Ke))
+        prims.append(scenes.Primitive(mesh, material_of(m, ignore_textures, tex, no_mixture, skip_specular_mat),
+                                      Ke))
     return prims
 
 
 def scene_from_obj(path: str, camera: scenes.Camera, width: int, height: int,
-                   env_Le: Optional[tuple] = None, ignore_textures: bool = False) -> scenes.Scene:
-    return scenes.Scene(load_obj(path, ignore_textures), camera, env_Le, width, height)
+                   env_Le: Optional[tuple] = None, ignore_textures: bool = False, no_mixture: bool = False,
+                   skip_specular_mat: bool = False) -> scenes.Scene:
+    return scenes.Scene(load_obj(path, ignore_textures, no_mixture, skip_specular_mat), camera, env_Le,
+                        width, height)

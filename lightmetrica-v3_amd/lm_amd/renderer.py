@@ -104,6 +104,21 @@ def scene_desc(flat):
     d.aspect = float(flat.camera.aspect)
     d.w = int(flat.w)
     d.h = int(flat.h)
+    nprim = d.num_prims
+    mat_ks = flat.mat_ks if flat.mat_ks is not None else np.zeros((nprim, 3), dtype=np.float32)
+    mat_tex = flat.mat_tex if flat.mat_tex is not None else np.full(nprim, -1, dtype=np.int32)
+    k['mat_ks'] = np.ascontiguousarray(mat_ks, dtype=np.float32)
+    k['mat_tex'] = np.ascontiguousarray(mat_tex, dtype=np.int32)
+    d.mat_ks = k['mat_ks'].ctypes.data
+    d.mat_tex = k['mat_tex'].ctypes.data
+    if flat.textures:
+        k['uvs'] = np.ascontiguousarray(flat.uvs, dtype=np.float32)
+        k['tex_info'], k['tex_data'] = flat.texture_tables()
+        d.uvs = k['uvs'].ctypes.data
+        d.num_textures = len(flat.textures)
+        d.tex_info = k['tex_info'].ctypes.data
+        d.tex_data = k['tex_data'].ctypes.data
+        d.tex_data_len = int(k['tex_data'].size)
     return keep, d
 
 

@@ -36,6 +36,7 @@ MAT_DIFFUSE = 0   # material::diffuse  (Kd)
 MAT_GLOSSY = 1    # material::glossy   (Ks, ax, ay)
 MAT_MIRROR = 2    # material::mirror
 MAT_GLASS = 3     # material::glass    (Ni)
+MAT_MIXTURE = 4   # material::mixture_wavefrontobj (Kd [+ map_Kd] diffuse + Ks/ax/ay glossy, alpha mask)
 
 
 def subseed(config: int, seed: int = MASTER_SEED) -> int:
@@ -52,6 +53,7 @@ class Mesh:
     ps: np.ndarray   # [V,3] float32 positions
     ns: np.ndarray   # [V,3] float32 vertex normals
     fs: np.ndarray   # [F,3] int32 vertex indices (counter-clockwise = front)
+    ts: Optional[np.ndarray] = None   # [V,2] float32 texture coordinates (Mesh::surface_point's t), or None
 
     @property
     def num_triangles(self) -> int:
@@ -64,6 +66,12 @@ class Mesh:
     def corner_normals(self) -> np.ndarray:
         """[F,9] float32 vertex normals (n1,n2,n3) per face."""
         return self.ns[self.fs].reshape(-1, 9).astype(np.float32, copy=False)
+
+    def corner_uvs(self) -> np.ndarray:
+        """[F,6] float32 texture coordinates (t1,t2,t3) per face (zeros without ts)."""
+        if self.ts is None:
+            return np.zeros((self.num_triangles, 6), dtype=np.float32)
+        return self.ts[self.fs].reshape(-1, 6).astype(np.float32, copy=False)
 
 
 def _normalize(v: np.ndarray) -> np.ndarray:
@@ -268,13 +276,24 @@ def cube(size: float, angle: float, center) -> Mesh:
 # Scene
 # ----------------------------------------------------------------------------
 
+@dataclasses.dataclass(eq=False)
+class Texture:
+    """texture::bitmap: texels [h, w, c] (c = 3 or 4) as the reference holds
+    them after loading (linear floats, row 0 = the first row after its
+    vertical flip); looked up nearest-texel with wrapping."""
+    data: np.ndarray
+
+
 @dataclasses.dataclass
 class Material:
     type: int
-    color: tuple = (1.0, 1.0, 1.0)   # Kd (diffuse) or Ks (glossy)
+    color: tuple = (1.0, 1.0, 1.0)   # Kd (diffuse, mixture) or Ks (glossy)
     ax: float = 0.0
     ay: float = 0.0
     ni: float = 1.0
+    ks: tuple = (0.0, 0.0, 0.0)      # mixture: Ks of the glossy lobe
+    texture: Optional[Texture] = None   # map_Kd of diffuse / mixture (takes priority over color)
+    alpha_mask: bool = True          # mixture: use the texture's alpha channel as a mask
 
 
 @dataclasses.dataclass
@@ -302,22 +321,33 @@ class Scene:
     h: int = 256
 
     def flatten(self) -> "FlatScene":
-        tris, norms, prim, face = [], [], [], []
+        tris, norms, prim, face, uvs = [], [], [], [], []
         for pi, p in enumerate(self.primitives):
             tris.append(p.mesh.triangles())
             norms.append(p.mesh.corner_normals())
+            uvs.append(p.mesh.corner_uvs())
             nf = p.mesh.num_triangles
             prim.append(np.full(nf, pi, dtype=np.int32))
             face.append(np.arange(nf, dtype=np.int32))
         nprim = len(self.primitives)
         mat_type = np.array([p.material.type for p in self.primitives], dtype=np.int32)
         mat_param = np.zeros((nprim, 8), dtype=np.float32)
+        mat_ks = np.zeros((nprim, 3), dtype=np.float32)
+        mat_tex = np.full(nprim, -1, dtype=np.int32)
+        textures, tex_index = [], {}
         for i, p in enumerate(self.primitives):
             m = p.material
             mat_param[i, 0:3] = m.color
             mat_param[i, 3] = m.ax
             mat_param[i, 4] = m.ay
             mat_param[i, 5] = m.ni
+            mat_param[i, 6] = 1.0 if m.alpha_mask else 0.0
+            mat_ks[i] = m.ks
+            if m.texture is not None:
+                if id(m.texture) not in tex_index:
+                    tex_index[id(m.texture)] = len(textures)
+                    textures.append(np.ascontiguousarray(m.texture.data, dtype=np.float32))
+                mat_tex[i] = tex_index[id(m.texture)]
         prim_Ke = np.zeros((nprim, 3), dtype=np.float32)
         light_prims = []
         for i, p in enumerate(self.primitives):
@@ -333,7 +363,10 @@ class Scene:
             light_prims=np.array(light_prims, dtype=np.int32),
             env_Le=np.array(self.env_Le if self.env_Le is not None else (0, 0, 0), dtype=np.float32),
             has_env=self.env_Le is not None,
-            camera=self.camera, w=self.w, h=self.h)
+            camera=self.camera, w=self.w, h=self.h,
+            mat_ks=mat_ks, mat_tex=mat_tex,
+            uvs=np.ascontiguousarray(np.concatenate(uvs), dtype=np.float32) if textures else None,
+            textures=textures)
 
 
 @dataclasses.dataclass
@@ -343,7 +376,7 @@ class FlatScene:
     tri_prim: np.ndarray    # [N] int32 primitive (scene node) index
     tri_face: np.ndarray    # [N] int32 face index within the primitive's mesh
     mat_type: np.ndarray    # [P] int32
-    mat_param: np.ndarray   # [P,8] float32: color rgb, ax, ay, ni, pad, pad
+    mat_param: np.ndarray   # [P,8] float32: color rgb, ax, ay, ni, alpha mask, pad
     prim_Ke: np.ndarray     # [P,3] float32
     light_prims: np.ndarray # [L] int32 primitives carrying light::area, in order
     env_Le: np.ndarray      # [3] float32
@@ -351,6 +384,24 @@ class FlatScene:
     camera: Camera
     w: int
     h: int
+    mat_ks: Optional[np.ndarray] = None    # [P,3] float32 mixture Ks
+    mat_tex: Optional[np.ndarray] = None   # [P] int32 texture of Kd, -1 none
+    uvs: Optional[np.ndarray] = None       # [N,6] float32 texture coordinates (textured scenes)
+    textures: list = dataclasses.field(default_factory=list)   # [h,w,c] float32 texel arrays
+
+    def texture_tables(self):
+        """(tex_info [T,4] int32: width, height, channels, offset; tex_data
+        float32) of the scene's textures."""
+        info = np.zeros((len(self.textures), 4), dtype=np.int32)
+        off = 0
+        for t, a in enumerate(self.textures):
+            if a.ndim != 3 or a.shape[2] not in (3, 4):
+                raise ValueError('a texture must be [h, w, 3] or [h, w, 4]')
+            info[t] = (a.shape[1], a.shape[0], a.shape[2], off)
+            off += a.size
+        data = (np.concatenate([a.reshape(-1) for a in self.textures]).astype(np.float32)
+                if self.textures else np.zeros(1, dtype=np.float32))
+        return info, data
 
     @property
     def num_triangles(self) -> int:

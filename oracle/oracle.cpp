@@ -16,6 +16,8 @@
 //   * material::{diffuse,glossy,mirror,glass}
 //       src/material/material_diffuse.cpp:60-95, material_glossy.cpp:43-113,
 //       material_mirror.cpp:33-62, material_glass.cpp:85-286
+//   * material::mixture_wavefrontobj      src/model/model_wavefrontobj.cpp:340-513
+//   * texture::bitmap lookup              src/texture/texture_bitmap.cpp:68-89
 //   * light::area, light::envconst        src/light/light_area.cpp:40-187,
 //                                         src/light/light_envconst.cpp:38-141
 //   * uniform light selection             src/scene.cpp:374-387
@@ -300,7 +302,7 @@ inline Rand4 rand4(uint32_t pixel, uint32_t sample, uint32_t vertex, uint32_t bl
 // Scene (flattened, identity transforms; see lm_amd/scenes.py)
 // ---------------------------------------------------------------------------
 
-enum { MAT_DIFFUSE = 0, MAT_GLOSSY = 1, MAT_MIRROR = 2, MAT_GLASS = 3 };
+enum { MAT_DIFFUSE = 0, MAT_GLOSSY = 1, MAT_MIRROR = 2, MAT_GLASS = 3, MAT_MIXTURE = 4 };
 
 }  // namespace
 
@@ -322,6 +324,14 @@ struct OrcScene {
     float cam_pos[3], cam_center[3], cam_up[3];
     float vfov, aspect;
     int32_t w, h;
+    // material::mixture_wavefrontobj and texture::bitmap (optional)
+    const float* mat_ks;     // [P][3] or NULL
+    const int32_t* mat_tex;  // [P] texture of Kd, -1 none (or NULL)
+    const float* uvs;        // [N][6]
+    int32_t num_textures;
+    const int32_t* tex_info; // [T][4] width, height, channels, offset
+    const float* tex_data;
+    int64_t tex_data_len;
 };
 
 struct OrcPTParams {
@@ -443,8 +453,10 @@ inline double balance_heuristic(double p1, double p2) {
 
 struct Mat {
     int type;
-    V3 c;
+    V3 c;            // Kd (diffuse, mixture) or Ks (glossy)
     double ax, ay, ni;
+    V3 ks;           // mixture: Ks
+    double alpha;    // mixture: alpha mask value (1 without a mask)
 };
 
 // material::glossy helpers (material_glossy.cpp:43-58)
@@ -496,28 +508,67 @@ inline double glass_Fr(const Mat& m, const Geom& g, V3 wi, V3& wt, bool& total, 
     return total ? 1.0 : glass_fresnel(m, wi, wt, g);
 }
 
-inline bool is_specular(const Mat& m) { return m.type == MAT_MIRROR || m.type == MAT_GLASS; }
+// Material::is_specular_component: mirror, glass, and the mixture's alpha
+// pass-through component (model_wavefrontobj.cpp:509-511).
+inline bool is_specular(const Mat& m, int comp) {
+    return m.type == MAT_MIRROR || m.type == MAT_GLASS || (m.type == MAT_MIXTURE && comp != 0);
+}
 
-// Material::eval with eval_delta = true (the only use in renderer::pt besides
-// sample_direction's internal calls, which use eval_delta = false).
-inline V3 mat_eval_delta(const Mat& m, const Geom& g, V3 wi, V3 wo) {
+// The mixture's underlying material::diffuse and material::glossy
+// (model_wavefrontobj.cpp:386-400).
+inline Mat mixture_diffuse(const Mat& m) { Mat d = m; d.type = MAT_DIFFUSE; return d; }
+inline Mat mixture_glossy(const Mat& m) { Mat d = m; d.type = MAT_GLOSSY; d.c = m.ks; return d; }
+// diffuse_selection_prob (model_wavefrontobj.cpp:426-434) from the
+// reflectances Kd and Ks.
+inline double mixture_w(const Mat& m) {
+    const double wd = comp_max(m.c), wg = comp_max(m.ks);
+    if (wd == 0 && wg == 0) return 1;
+    return wd / (wd + wg);
+}
+
+// Material::eval of component `comp` with eval_delta = true.
+inline V3 mat_eval_delta(const Mat& m, const Geom& g, V3 wi, V3 wo, int comp) {
     switch (m.type) {
         case MAT_DIFFUSE: return opposite(g, wi, wo) ? V3(0) : m.c / Pi;
         case MAT_GLOSSY: return glossy_eval(m, g, wi, wo);
+        case MAT_MIXTURE:
+            // model_wavefrontobj.cpp:495-506
+            if (comp != 0) return V3(0);
+            return (mat_eval_delta(mixture_diffuse(m), g, wi, wo, 0) + glossy_eval(mixture_glossy(m), g, wi, wo)) *
+                   m.alpha;
         default: return V3(0);   // mirror / glass: eval_delta -> 0
     }
 }
-inline double mat_pdf_delta(const Mat& m, const Geom& g, V3 wi, V3 wo) {
+inline double mat_pdf_delta(const Mat& m, const Geom& g, V3 wi, V3 wo, int comp) {
     switch (m.type) {
         case MAT_DIFFUSE: return opposite(g, wi, wo) ? 0.0 : 1.0 / Pi;
         case MAT_GLOSSY: return glossy_pdf(m, g, wi, wo);
+        case MAT_MIXTURE: {
+            // model_wavefrontobj.cpp:482-493
+            if (comp != 0) return 0;
+            const double w = mixture_w(m);
+            return w * mat_pdf_delta(mixture_diffuse(m), g, wi, wo, 0) +
+                   (1 - w) * glossy_pdf(mixture_glossy(m), g, wi, wo);
+        }
         default: return 0.0;
     }
 }
 
-// Material::sample_component (glass: Fresnel-proportional, others: {0,1}).
-inline void mat_sample_component(const Mat& m, const Geom& g, V3 wi, double uc, int& comp, double& w) {
-    comp = 0; w = 1;
+// Material::sample_component (glass: Fresnel-proportional; mixture: alpha,
+// model_wavefrontobj.cpp:412-419; others: {0,1}). `urest` is uc rescaled
+// within the chosen interval (the renderer's random number for the mixture's
+// lobe selection).
+inline void mat_sample_component(const Mat& m, const Geom& g, V3 wi, double uc, int& comp, double& w,
+                                 double& urest) {
+    comp = 0; w = 1; urest = uc;
+    if (m.type == MAT_MIXTURE) {
+        if (uc < m.alpha) {
+            comp = 0; w = 1 / m.alpha; urest = uc / m.alpha;
+        } else {
+            comp = 1; w = 1 / (1 - m.alpha); urest = (uc - m.alpha) / (1 - m.alpha);
+        }
+        return;
+    }
     if (m.type != MAT_GLASS) return;
     V3 wt; bool total; double eta;
     const double Fr = glass_Fr(m, g, wi, wt, total, eta);
@@ -527,10 +578,26 @@ inline void mat_sample_component(const Mat& m, const Geom& g, V3 wi, double uc, 
 }
 
 // Material::sample_direction (returns false on nullopt); weight excludes the
-// shading-normal correction, which path::sample_direction applies.
-inline bool mat_sample_direction(const Mat& m, const Geom& g, V3 wi, int comp, double u0, double u1,
+// shading-normal correction, which path::sample_direction applies. `ul`
+// picks the mixture's lobe.
+inline bool mat_sample_direction(const Mat& m, const Geom& g, V3 wi, int comp, double u0, double u1, double ul,
                                  V3& wo, V3& weight) {
     switch (m.type) {
+        case MAT_MIXTURE: {
+            // model_wavefrontobj.cpp:436-474
+            if (comp != 0) {
+                wo = -wi;
+                weight = V3(1);
+                return true;
+            }
+            const Mat lobe = ul < mixture_w(m) ? mixture_diffuse(m) : mixture_glossy(m);
+            V3 lw;
+            if (!mat_sample_direction(lobe, g, wi, 0, u0, u1, 0, wo, lw)) return false;
+            const V3 f = mat_eval_delta(m, g, wi, wo, 0);
+            const double p = mat_pdf_delta(m, g, wi, wo, 0);
+            weight = f / p;
+            return true;
+        }
         case MAT_DIFFUSE: {
             V3 n, u, v;
             twosided(g, wi, n, u, v);
@@ -600,7 +667,8 @@ void init_scene(Scene& S, const OrcScene* s, const Accel* accel) {
     S.mats.resize(s->num_prims);
     for (int i = 0; i < s->num_prims; i++) {
         const float* p = s->mat_param + 8 * i;
-        S.mats[i] = Mat{s->mat_type[i], load3(p), double(p[3]), double(p[4]), double(p[5])};
+        const V3 ks = (s->mat_type[i] == MAT_MIXTURE && s->mat_ks) ? load3(s->mat_ks + 3 * i) : V3(0);
+        S.mats[i] = Mat{s->mat_type[i], load3(p), double(p[3]), double(p[4]), double(p[5]), ks, 1.0};
     }
     S.prim_light.assign(s->num_prims, -1);
     for (int l = 0; l < s->num_area_lights; l++) {
@@ -646,7 +714,35 @@ struct Hit {
     bool env = false;
     int prim = -1;
     Geom g;
+    Mat mat{};   // the primitive's material at the hit (textures applied)
 };
+
+// Material of primitive `prim` at hit (u, v) of triangle `tri`: a Kd texture
+// (material::diffuse / mixture mapKd) is texture::bitmap::eval at
+// Mesh_Raw::surface_point's t (mesh_raw.cpp:87-97, texture_bitmap.cpp:68-75);
+// the mixture's alpha mask is eval_alpha (texture_bitmap.cpp:82-89,
+// model_wavefrontobj.cpp:376-379,403-410).
+Mat material_at(const Scene& S, int prim, int64_t tri, double u, double v) {
+    Mat m = S.mats[prim];
+    const OrcScene* s = S.s;
+    const int tx = s->mat_tex ? s->mat_tex[prim] : -1;
+    if (tx < 0) return m;
+    const float* t = s->uvs + 6 * tri;
+    const double w0 = 1 - u - v;
+    double tu = t[0] * w0 + t[2] * u + t[4] * v;
+    double tv = t[1] * w0 + t[3] * u + t[5] * v;
+    tu -= std::floor(tu);
+    tv -= std::floor(tv);
+    const int32_t* ti = s->tex_info + 4 * tx;
+    const int W = ti[0], H = ti[1], C = ti[2];
+    const int x = std::clamp(int(tu * W), 0, W - 1);
+    const int y = std::clamp(int(tv * H), 0, H - 1);
+    const float* px = s->tex_data + ti[3] + int64_t(C) * (int64_t(W) * y + x);
+    m.c = load3(px);
+    const bool mask = m.type == MAT_MIXTURE && C == 4 && s->mat_param[8 * prim + 6] != 0.f;
+    if (mask) m.alpha = px[3];
+    return m;
+}
 
 // Scene_::intersect (src/scene.cpp:319-345) with Mesh_Raw::surface_point.
 Hit scene_intersect(const Scene& S, V3 o, V3 d, double tmin, double tmax) {
@@ -670,6 +766,7 @@ Hit scene_intersect(const Scene& S, V3 o, V3 d, double tmin, double tmax) {
     h.valid = true;
     h.prim = S.s->tri_prim[r.prim];
     h.g = make_on_surface(p, n, gn);
+    h.mat = material_at(S, h.prim, r.prim, r.u, r.v);
     return h;
 }
 
@@ -958,14 +1055,15 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
             // camera endpoint (path::sample_position -> Camera_Pinhole::sample_position)
             Geom sp = make_degenerated(S.pos);
             bool sp_camera = true;
-            int sp_prim = -1;
+            Mat sp_mat{};
             int comp = 0;
+            double ul = 0;   // the mixture's lobe selection at sp
             V3 thr(1);
             V3 wi(0);
             for (int nv = 1; nv < P->max_verts; nv++) {
                 const Rand4 ra = rand4(uint32_t(pix), sample, uint32_t(nv - 1), 0, P->seed);
-                const Mat* m = sp_camera ? nullptr : &S.mats[sp_prim];
-                const bool spec = m ? is_specular(*m) : false;
+                const Mat* m = sp_camera ? nullptr : &sp_mat;
+                const bool spec = m ? is_specular(*m, comp) : false;
                 const bool samplable_by_nee = mode != 0 && nv > 1 && !spec;
                 if (samplable_by_nee) {
                     const Rand4 rb = rand4(uint32_t(pix), sample, uint32_t(nv - 1), 1, P->seed);
@@ -1010,12 +1108,12 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                         n_shadow++;
                         if (visible(S, sp, gL)) {
                             const V3 wo = -wL;
-                            const V3 fs = mat_eval_delta(*m, sp, wi, wo) * shading_normal_correction(sp, wi, wo);
+                            const V3 fs = mat_eval_delta(*m, sp, wi, wo, comp) * shading_normal_correction(sp, wi, wo);
                             if (!is_zero(fs)) {
                                 double mis_w = 1;
                                 if (mode == 2) {
                                     const double p_light = pdf_direct_light(S, sp, gL, li, wL);
-                                    const double p_bsdf = mat_pdf_delta(*m, sp, wi, wo);
+                                    const double p_bsdf = mat_pdf_delta(*m, sp, wi, wo, comp);
                                     mis_w = balance_heuristic(p_light, p_bsdf);
                                 }
                                 L = L + thr * fs * weight * mis_w;
@@ -1031,7 +1129,7 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     wo = primary_dir(S, ux, uy);
                     sw = V3(1);
                 } else {
-                    if (!mat_sample_direction(*m, sp, wi, comp, ra.u[2], ra.u[3], wo, sw)) break;
+                    if (!mat_sample_direction(*m, sp, wi, comp, ra.u[2], ra.u[3], ul, wo, sw)) break;
                     sw = sw * shading_normal_correction(sp, wi, wo);
                 }
                 n_ext++;
@@ -1048,7 +1146,7 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     else fs = dot(woL, hit.g.n) <= 0 ? V3(0) : S.lights[hit_light].Ke;
                     double mis_w = 1;
                     if (mode != 0 && samplable_by_nee) {
-                        const double pdf_bsdf = mat_pdf_delta(*m, sp, wi, wo);
+                        const double pdf_bsdf = mat_pdf_delta(*m, sp, wi, wo, comp);
                         const double pdf_light = pdf_direct_light(S, sp, hit.g, hit_light, woL);
                         mis_w = balance_heuristic(pdf_bsdf, pdf_light);
                     }
@@ -1065,12 +1163,12 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                 }
                 int c;
                 double cw;
-                mat_sample_component(S.mats[hit.prim], hit.g, -wo, rn.u[1], c, cw);
+                mat_sample_component(hit.mat, hit.g, -wo, rn.u[1], c, cw, ul);
                 thr = thr * cw;
                 wi = -wo;
                 sp = hit.g;
                 sp_camera = false;
-                sp_prim = hit.prim;
+                sp_mat = hit.mat;
                 comp = c;
             }
             sx += L.x; sy += L.y; sz += L.z;
@@ -1170,15 +1268,15 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
         {
             const Rand4 rn = rand4(idx, pass, 2, 0, P->seed);
             int c;
-            double cw;
-            mat_sample_component(S.mats[hit.prim], hit.g, -d, rn.u[1], c, cw);
+            double cw, ul;
+            mat_sample_component(hit.mat, hit.g, -d, rn.u[1], c, cw, ul);
             thr = thr * cw;
             Geom sp = hit.g;
-            int sp_prim = hit.prim;
+            Mat sp_mat = hit.mat;
             V3 wi = -d;
             int comp = c;
             for (int nv = 2; nv < P->max_verts; nv++) {
-                const Mat& m = S.mats[sp_prim];
+                const Mat& m = sp_mat;
                 // NEE edge to the camera
                 [&] {
                     const V3 wo_c = normalize(sp.p - S.pos);   // camera -> point
@@ -1194,7 +1292,7 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
                     const double ct = -ez;
                     const double J = 1.0 / (ct * ct * ct) / (S.tf * S.tf * S.aspect * 4);
                     const V3 wo = -wo_c;
-                    const V3 fs = mat_eval_delta(m, sp, wi, wo) * shading_normal_correction_le(sp, wi, wo);
+                    const V3 fs = mat_eval_delta(m, sp, wi, wo, comp) * shading_normal_correction_le(sp, wi, wo);
                     const V3 C = thr * fs * (J * G);
                     const int px = std::clamp(int(rx * w), 0, w - 1);
                     const int py = std::clamp(int(ry * h), 0, h - 1);
@@ -1210,7 +1308,7 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
                 if (nv + 1 >= P->max_verts) break;
                 const Rand4 ra = rand4(idx, pass, uint32_t(nv), 0, P->seed);
                 V3 wo, sw;
-                if (!mat_sample_direction(m, sp, wi, comp, ra.u[2], ra.u[3], wo, sw)) break;
+                if (!mat_sample_direction(m, sp, wi, comp, ra.u[2], ra.u[3], ul, wo, sw)) break;
                 if (m.type == MAT_GLASS && comp == 1) {
                     V3 wt; bool total; double eta;
                     glass_Fr(m, sp, wi, wt, total, eta);
@@ -1231,11 +1329,11 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
                 }
                 int c2;
                 double cw2;
-                mat_sample_component(S.mats[nh.prim], nh.g, -wo, rn2.u[1], c2, cw2);
+                mat_sample_component(nh.mat, nh.g, -wo, rn2.u[1], c2, cw2, ul);
                 thr = thr * cw2;
                 wi = -wo;
                 sp = nh.g;
-                sp_prim = nh.prim;
+                sp_mat = nh.mat;
                 comp = c2;
             }
         }
@@ -1256,14 +1354,16 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
 // sn-correction. Outputs f[3n], pdf[n].
 void orc_mat_eval(int type, const double* params, const double* nrm, int64_t n, const double* wi,
                   const double* wo, double* f, double* pdf) {
-    Mat m{type, V3(params[0], params[1], params[2]), params[3], params[4], params[5]};
+    // params[12]: color rgb, ax, ay, ni, mixture Ks rgb, mixture alpha, -, -
+    Mat m{type, V3(params[0], params[1], params[2]), params[3], params[4], params[5],
+          V3(params[6], params[7], params[8]), params[9]};
     const V3 N(nrm[0], nrm[1], nrm[2]);
     const Geom g = make_on_surface(V3(0), N, N);
     for (int64_t i = 0; i < n; i++) {
         const V3 a(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), b(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
-        const V3 e = mat_eval_delta(m, g, a, b);
+        const V3 e = mat_eval_delta(m, g, a, b, 0);
         f[3 * i] = e.x; f[3 * i + 1] = e.y; f[3 * i + 2] = e.z;
-        pdf[i] = mat_pdf_delta(m, g, a, b);
+        pdf[i] = mat_pdf_delta(m, g, a, b, 0);
     }
 }
 
@@ -1272,16 +1372,17 @@ void orc_mat_eval(int type, const double* params, const double* nrm, int64_t n, 
 // correction), comp[n], ok[n].
 void orc_mat_sample(int type, const double* params, const double* nrm, int64_t n, const double* wi,
                     const double* u, double* wo, double* weight, int32_t* comp, int32_t* ok) {
-    Mat m{type, V3(params[0], params[1], params[2]), params[3], params[4], params[5]};
+    Mat m{type, V3(params[0], params[1], params[2]), params[3], params[4], params[5],
+          V3(params[6], params[7], params[8]), params[9]};
     const V3 N(nrm[0], nrm[1], nrm[2]);
     const Geom g = make_on_surface(V3(0), N, N);
     for (int64_t i = 0; i < n; i++) {
         const V3 a(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
         int c;
-        double cw;
-        mat_sample_component(m, g, a, u[4 * i], c, cw);
+        double cw, ul;
+        mat_sample_component(m, g, a, u[4 * i], c, cw, ul);
         V3 w, sw;
-        const bool good = mat_sample_direction(m, g, a, c, u[4 * i + 1], u[4 * i + 2], w, sw);
+        const bool good = mat_sample_direction(m, g, a, c, u[4 * i + 1], u[4 * i + 2], ul, w, sw);
         ok[i] = good ? 1 : 0;
         comp[i] = c;
         if (good) sw = sw * shading_normal_correction(g, a, w) * cw;

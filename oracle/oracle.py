@@ -44,6 +44,13 @@ class OrcScene(ctypes.Structure):
         ('aspect', ctypes.c_float),
         ('w', ctypes.c_int32),
         ('h', ctypes.c_int32),
+        ('mat_ks', ctypes.c_void_p),
+        ('mat_tex', ctypes.c_void_p),
+        ('uvs', ctypes.c_void_p),
+        ('num_textures', ctypes.c_int32),
+        ('tex_info', ctypes.c_void_p),
+        ('tex_data', ctypes.c_void_p),
+        ('tex_data_len', ctypes.c_int64),
     ]
 
 
@@ -235,6 +242,21 @@ def make_orc_scene(flat) -> tuple:
     s.aspect = float(flat.camera.aspect)
     s.w = int(flat.w)
     s.h = int(flat.h)
+    nprim = s.num_prims
+    mat_ks = getattr(flat, 'mat_ks', None)
+    mat_tex = getattr(flat, 'mat_tex', None)
+    keep['mat_ks'] = np.ascontiguousarray(mat_ks if mat_ks is not None else np.zeros((nprim, 3)), dtype=np.float32)
+    keep['mat_tex'] = np.ascontiguousarray(mat_tex if mat_tex is not None else np.full(nprim, -1), dtype=np.int32)
+    s.mat_ks = keep['mat_ks'].ctypes.data
+    s.mat_tex = keep['mat_tex'].ctypes.data
+    if getattr(flat, 'textures', None):
+        keep['uvs'] = np.ascontiguousarray(flat.uvs, dtype=np.float32)
+        keep['tex_info'], keep['tex_data'] = flat.texture_tables()
+        s.uvs = keep['uvs'].ctypes.data
+        s.num_textures = len(flat.textures)
+        s.tex_info = keep['tex_info'].ctypes.data
+        s.tex_data = keep['tex_data'].ctypes.data
+        s.tex_data_len = int(keep['tex_data'].size)
     return s, keep
 
 
@@ -283,8 +305,10 @@ def lt_render(flat, num_samples: int, max_verts: int, seed: int = 0, pass_offset
                       'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices}}
 
 
-def _mat_params(color=(1, 1, 1), ax=0.0, ay=0.0, ni=1.0):
-    return np.array([color[0], color[1], color[2], ax, ay, ni, 0, 0], dtype=np.float64)
+def _mat_params(color=(1, 1, 1), ax=0.0, ay=0.0, ni=1.0, ks=(0, 0, 0), alpha=1.0):
+    """Probe parameters: color rgb (Kd / Ks), ax, ay, ni, mixture Ks rgb, mixture alpha."""
+    return np.array([color[0], color[1], color[2], ax, ay, ni, ks[0], ks[1], ks[2], alpha, 0, 0],
+                    dtype=np.float64)
 
 
 def mat_eval(mtype, params, normal, wi, wo):

@@ -97,3 +97,84 @@ def batch_mean_z(values, expected):
     v = np.asarray(values, dtype=np.float64)
     se = v.std(ddof=1) / math.sqrt(len(v))
     return (v.mean() - expected) / se
+
+
+def checker_texture(n=4, alpha=False, seed=0):
+    """n x n texels of seeded colours in [0.1, 0.9]; with `alpha`, a fourth
+    channel of alternating 0 / 1 (a binary mask)."""
+    rng = np.random.default_rng(seed)
+    rgb = rng.uniform(0.1, 0.9, size=(n, n, 3))
+    if not alpha:
+        return scenes.Texture(rgb.astype(np.float32))
+    a = ((np.arange(n)[:, None] + np.arange(n)[None, :]) % 2).astype(np.float64)
+    return scenes.Texture(np.concatenate([rgb, a[..., None]], -1).astype(np.float32))
+
+
+def textured_quad(tex, mtype=scenes.MAT_MIXTURE, uv_scale=1.0, z=0.0):
+    """Unit square [-1,1]^2 at depth z facing +z, texture coordinates
+    (x+1)/2 * uv_scale, (y+1)/2 * uv_scale."""
+    ps = np.array([[-1, -1, z], [1, -1, z], [1, 1, z], [-1, 1, z]], dtype=np.float32)
+    ns = np.tile(np.array([[0, 0, 1]], dtype=np.float32), (4, 1))
+    fs = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    ts = ((ps[:, :2] + 1) / 2 * uv_scale).astype(np.float32)
+    mat = scenes.Material(mtype, (1.0, 1.0, 1.0), 0.3, 0.3, ks=(0.0, 0.0, 0.0), texture=tex)
+    return scenes.Primitive(scenes.Mesh(ps, ns, fs, ts), mat)
+
+
+def masked_quad_scene(w=32, h=32, uv_scale=1.0):
+    """A textured mixture quad (Ks = 0, binary alpha mask) filling the view,
+    under a constant environment Le = 1: where alpha = 1 the pixel is the
+    texel's Kd (a one-sided diffuse plane under a constant environment
+    reflects rho Le), where alpha = 0 the environment behind (1)."""
+    tex = checker_texture(4, alpha=True, seed=5)
+    cam = scenes.Camera((0, 0, 2.0), (0, 0, 0), (0, 1, 0), 40.0, w / h)
+    return scenes.Scene([textured_quad(tex, uv_scale=uv_scale)], cam, (1.0, 1.0, 1.0), w, h), tex
+
+
+def mixture_room_scene(w=24, h=24):
+    """Box room with mixture walls (Kd + Ks), a textured mixture floor panel,
+    a glossy-heavy mixture cube and a ceiling emitter."""
+    room = scenes.box_room(open_front=True)
+    tex = checker_texture(8, seed=9)
+    floor = textured_quad(tex, uv_scale=2.0)
+    # lay the panel on the floor (y = -1 + 1e-3), facing +y
+    fl = floor.mesh
+    ps = np.stack([fl.ps[:, 0] * 0.8, np.full(4, -1 + 1e-3, dtype=np.float32), -fl.ps[:, 1] * 0.8], -1)
+    floor = scenes.Primitive(scenes.Mesh(ps.astype(np.float32), np.tile(np.array([[0, 1, 0]], np.float32), (4, 1)),
+                                         fl.fs, fl.ts),
+                             scenes.Material(scenes.MAT_MIXTURE, (1, 1, 1), 0.2, 0.2, ks=(0.2, 0.2, 0.2), texture=tex))
+    cube = scenes.cube(0.6, 0.5, (0.35, -0.7, -0.2))
+    y = 1.0 - 1e-3
+    lamp = scenes.flat_mesh(scenes.quad((-.3, y, -.3), (.3, y, -.3), (.3, y, .3), (-.3, y, .3), (0, -1, 0)))
+    prims = [scenes.Primitive(room, scenes.Material(scenes.MAT_MIXTURE, (0.5, 0.45, 0.4), 0.3, 0.3,
+                                                    ks=(0.15, 0.15, 0.15))),
+             floor,
+             scenes.Primitive(cube, scenes.Material(scenes.MAT_MIXTURE, (0.1, 0.2, 0.3), 0.08, 0.15,
+                                                    ks=(0.6, 0.5, 0.4))),
+             scenes.Primitive(lamp, scenes.Material(scenes.MAT_DIFFUSE, (0.5, 0.5, 0.5)), Ke=(12.0, 12.0, 12.0))]
+    cam = scenes.Camera((0, 0, 3.5), (0, -0.2, 0), (0, 1, 0), 38.0, w / h)
+    return scenes.Scene(prims, cam, None, w, h)
+
+
+def masked_quad_expected(flat, tex, uv_scale=1.0):
+    """Per-pixel expected radiance of masked_quad_scene and a mask of the
+    pixels whose whole footprint maps to one texel."""
+    w, h = flat.w, flat.h
+    tf = math.tan(math.radians(flat.camera.vfov) / 2)
+    asp = flat.camera.aspect
+    n = tex.data.shape[0]
+    rx = np.arange(w + 1) / w
+    ry = np.arange(h + 1) / h
+    # camera at (0, 0, 2) looking down -z: the hit on z = 0 of raster (rx, ry)
+    X = 2 * asp * tf * (2 * rx - 1)
+    Y = 2 * tf * (2 * ry - 1)
+
+    def texel(c):
+        t = (c + 1) / 2 * uv_scale
+        return np.clip(((t - np.floor(t)) * n).astype(int), 0, n - 1)
+
+    tx, ty = texel(X), texel(Y)
+    same = (tx[:-1] == tx[1:])[None, :] & (ty[:-1] == ty[1:])[:, None]
+    T = tex.data[ty[:-1]][:, tx[:-1]]          # [h, w, 4]
+    exp = np.where(T[..., 3:4] > 0.5, T[..., :3], 1.0)
+    return exp, same

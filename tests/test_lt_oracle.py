@@ -10,7 +10,7 @@ from lm_amd import scenes
 from oracle import oracle as orc
 
 from lt_common import (batch_mean_z, block_z, env_cube_scene, furnace_expected, furnace_scene, glass_lamp_scene,
-                       smooth_sphere_scene)
+                       masked_quad_expected, masked_quad_scene, mixture_room_scene, smooth_sphere_scene)
 
 
 def test_lt_furnace_closed_form():
@@ -57,9 +57,15 @@ def _identity(flat, spp, batches, per_batch, seed=1, M=8):
     return z, nb, zt, np.mean(lts, 0).mean(), (pt['image'] - pt2['image']).mean()
 
 
-@pytest.mark.parametrize('name', ['config0', 'glass_lamp', 'smooth_sphere'])
+SCENES = {'config0': scenes.config0, 'glass_lamp': glass_lamp_scene, 'smooth_sphere': smooth_sphere_scene,
+          'mixture_room': mixture_room_scene}
+
+
+@pytest.mark.parametrize('name', list(SCENES))
 def test_lt_equals_pt_minus_direct(name):
-    make = {'config0': scenes.config0, 'glass_lamp': glass_lamp_scene, 'smooth_sphere': smooth_sphere_scene}[name]
+    """mixture_room: material::mixture_wavefrontobj walls and cube and a
+    texture::bitmap floor, in both estimators."""
+    make = SCENES[name]
     flat = make().flatten()
     flat.w = flat.h = 24
     flat.camera.aspect = 1.0
@@ -71,3 +77,22 @@ def test_lt_equals_pt_minus_direct(name):
     assert np.abs(z).max() < 5.0
     # 36 blocks x 3 correlated channels; z of a 32-batch variance estimate
     assert np.mean(z ** 2) < 2.0
+
+
+def test_pt_masked_texture_quad_closed_form():
+    """PT oracle on a textured mixture quad with a binary alpha mask under a
+    constant environment: texel Kd where alpha = 1, the environment (exactly)
+    where alpha = 0, on every pixel whose footprint lies in one texel."""
+    s, tex = masked_quad_scene()
+    flat = s.flatten()
+    spp = 64
+    r = orc.pt_render(flat, spp, 8, seed=3)
+    exp, same = masked_quad_expected(flat, tex)
+    img = r['image']
+    se = np.sqrt(np.maximum(r['sumsq'] / spp - img ** 2, 0) / spp)
+    hole = same & (exp == 1.0).all(-1)
+    assert hole.sum() > 100 and (same & ~hole).sum() > 100
+    assert np.all(img[hole] == 1.0)
+    z = (img - exp)[same & ~hole] / se[same & ~hole]
+    print(f'masked quad: {same.sum()} pixels, max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}')
+    assert np.abs(z).max() < 5 and np.mean(z ** 2) < 1.5

@@ -74,29 +74,71 @@ def test_parse_groups_and_triangulation(tmp_path):
 
 
 def test_material_mapping_follows_the_reference(tmp_path):
-    prims = objloader.load_obj(write(tmp_path))
-    white, shiny, glass, mirror, lamp = (p.material for p in prims)
-    assert white.type == scenes.MAT_DIFFUSE and white.color == (0.5, 0.5, 0.5)
+    """Default build: every non-specular MTL material is
+    material::mixture_wavefrontobj (Kd, Ks, ax, ay); NO_MIXTURE_MATERIAL:
+    diffuse when Ks == 0, else glossy; skip_specular_mat: black diffuse for
+    illum 5 / 7 and no alpha mask."""
     r = 2.0 / (2.0 + 98.0)
     a_s = math.sqrt(1.0 - 0.5 * 0.9)
-    assert shiny.type == scenes.MAT_GLOSSY and shiny.color == (0.7, 0.6, 0.5)
-    assert shiny.ax == pytest.approx(r / a_s) and shiny.ay == pytest.approx(r * a_s)
+    prims = objloader.load_obj(write(tmp_path))
+    white, shiny, glass, mirror, lamp = (p.material for p in prims)
+    assert white.type == scenes.MAT_MIXTURE and white.color == (0.5, 0.5, 0.5) and white.ks == (0.0, 0.0, 0.0)
+    assert shiny.type == scenes.MAT_MIXTURE and shiny.color == (0.1, 0.1, 0.1) and shiny.ks == (0.7, 0.6, 0.5)
+    assert shiny.ax == pytest.approx(r / a_s) and shiny.ay == pytest.approx(r * a_s) and shiny.alpha_mask
     assert glass.type == scenes.MAT_GLASS and glass.ni == pytest.approx(1.45)
     assert mirror.type == scenes.MAT_MIRROR
     assert prims[4].Ke == (4.0, 5.0, 6.0) and prims[0].Ke is None
+    white, shiny, glass, mirror, lamp = (p.material for p in objloader.load_obj(write(tmp_path), no_mixture=True))
+    assert white.type == scenes.MAT_DIFFUSE and white.color == (0.5, 0.5, 0.5)
+    assert shiny.type == scenes.MAT_GLOSSY and shiny.color == (0.7, 0.6, 0.5)
+    assert shiny.ax == pytest.approx(r / a_s) and shiny.ay == pytest.approx(r * a_s)
+    white, shiny, glass, mirror, lamp = (p.material for p in
+                                         objloader.load_obj(write(tmp_path), skip_specular_mat=True))
+    assert glass.type == scenes.MAT_DIFFUSE and glass.color == (0.0, 0.0, 0.0) and mirror.color == (0.0, 0.0, 0.0)
+    assert shiny.type == scenes.MAT_MIXTURE and not shiny.alpha_mask
     # roughness floor of 1e-3 for very high Ns
     m = objloader.material_of(objloader.MtlParams('x', Ks=(1, 1, 1), Ns=1e9))
     assert m.ax == pytest.approx(1e-3) and m.ay == pytest.approx(1e-3)
 
 
-def test_textures_and_bad_indices_are_errors(tmp_path):
+def test_missing_textures_and_bad_indices_are_errors(tmp_path):
     p = write(tmp_path, OBJ + "usemtl tex\nf 1 2 3\n")
-    with pytest.raises(NotImplementedError):
-        objloader.load_obj(p)
+    with pytest.raises(OSError):
+        objloader.load_obj(p)   # wood.png does not exist
     prims = objloader.load_obj(p, ignore_textures=True)
-    assert prims[-1].material.color == (0.3, 0.3, 0.3)
+    assert prims[-1].material.color == (0.3, 0.3, 0.3) and prims[-1].material.texture is None
     with pytest.raises(ValueError):
         objloader.load_obj(write(tmp_path, "v 0 0 0\nv 1 0 0\nf 1 2 5\n"))
+    with pytest.raises(ValueError):
+        objloader.load_obj(write(tmp_path, "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nf 1/1 2/2 3/1\n"))
+
+
+def test_texture_loading_follows_stbi_loadf(tmp_path):
+    """map_Kd through texture::bitmap's stbi_loadf: colour (c/255)^2.2, alpha
+    c/255, rows flipped; texture coordinates from vt per corner; one texture
+    per file name."""
+    from PIL import Image
+    img = np.zeros((2, 3, 4), dtype=np.uint8)
+    img[0, 0] = (255, 0, 0, 255)      # top-left in the file
+    img[1, 2] = (10, 128, 200, 64)    # bottom-right in the file
+    Image.fromarray(img, 'RGBA').save(tmp_path / 'leaf.png')
+    mtl = "newmtl leaf\nKd 1 1 1\nKs 0 0 0\nmap_Kd leaf.png\nnewmtl leaf2\nKd 1 1 1\nmap_Kd leaf.png\n"
+    obj = ("mtllib scene.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0.25 0.75\n"
+           "usemtl leaf\nf 1/1 2/2 3/3\nusemtl leaf2\nf 1/3 2/2 3/1\n")
+    prims = objloader.load_obj(write(tmp_path, obj, mtl))
+    t = prims[0].material.texture
+    assert t is not None and t is prims[1].material.texture
+    a = t.data
+    assert a.shape == (2, 3, 4)
+    # flipped: file row 1 is texture row 0
+    assert np.allclose(a[0, 2, :3], (np.array([10, 128, 200]) / 255.0) ** 2.2, rtol=1e-6)
+    assert a[0, 2, 3] == pytest.approx(64 / 255.0)
+    assert np.allclose(a[1, 0], (1, 0, 0, 1))
+    assert np.allclose(prims[0].mesh.corner_uvs()[0], [0, 0, 1, 0, 0.25, 0.75])
+    assert np.allclose(prims[1].mesh.corner_uvs()[0], [0.25, 0.75, 1, 0, 0, 0])
+    flat = scenes.Scene(prims, scenes.Camera((0, 0, 3), (0, 0, 0), (0, 1, 0), 40, 1), (1, 1, 1), 8, 8).flatten()
+    info, data = flat.texture_tables()
+    assert info.tolist() == [[3, 2, 4, 0]] and data.size == 24 and flat.mat_tex.tolist() == [0, 0]
 
 
 BOX_OBJ = """

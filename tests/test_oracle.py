@@ -181,6 +181,7 @@ def _hemi_quadrature(n_theta=256, n_phi=512):
     (scenes.MAT_DIFFUSE, orc._mat_params((0.7, 0.5, 0.3))),
     (scenes.MAT_GLOSSY, orc._mat_params((0.9, 0.9, 0.9), 0.3, 0.3)),
     (scenes.MAT_GLOSSY, orc._mat_params((0.8, 0.8, 0.8), 0.1, 0.4)),
+    (scenes.MAT_MIXTURE, orc._mat_params((0.5, 0.4, 0.3), 0.2, 0.3, ks=(0.3, 0.35, 0.4))),
 ])
 def test_pdf_integrates_to_one_and_energy(mtype, params):
     """Projected-solid-angle pdf integrates to 1 (diffuse exactly, GGX up to
@@ -203,6 +204,7 @@ def test_pdf_integrates_to_one_and_energy(mtype, params):
 @pytest.mark.parametrize('mtype,params', [
     (scenes.MAT_DIFFUSE, orc._mat_params((0.7, 0.5, 0.3))),
     (scenes.MAT_GLOSSY, orc._mat_params((0.9, 0.6, 0.3), 0.25, 0.25)),
+    (scenes.MAT_MIXTURE, orc._mat_params((0.5, 0.4, 0.3), 0.2, 0.3, ks=(0.3, 0.35, 0.4))),
 ])
 def test_reciprocity(mtype, params):
     rng = np.random.default_rng(1)
@@ -216,6 +218,7 @@ def test_reciprocity(mtype, params):
 @pytest.mark.parametrize('mtype,params', [
     (scenes.MAT_DIFFUSE, orc._mat_params((0.7, 0.5, 0.3))),
     (scenes.MAT_GLOSSY, orc._mat_params((0.9, 0.6, 0.3), 0.25, 0.25)),
+    (scenes.MAT_MIXTURE, orc._mat_params((0.5, 0.4, 0.3), 0.2, 0.3, ks=(0.3, 0.35, 0.4))),
 ])
 def test_sample_weight_equals_f_over_pdf_and_chi2(mtype, params):
     """Sampled directions: weight = f / pdf (projected measure), and the
@@ -248,6 +251,48 @@ def test_sample_weight_equals_f_over_pdf_and_chi2(mtype, params):
     chi2 = np.sum((hist[m] - expected[m]) ** 2 / expected[m])
     from scipy.stats import chi2 as chi2d
     assert chi2d.sf(chi2, m.sum() - 1) > 1e-4, chi2
+
+
+def test_mixture_is_the_sum_of_its_lobes():
+    """material::mixture_wavefrontobj (model_wavefrontobj.cpp:340-513): f =
+    f_diffuse(Kd) + f_glossy(Ks), pdf = w p_diffuse + (1-w) p_glossy with
+    w = max(Kd) / (max(Kd) + max(Ks)), both evaluated by the separately
+    pinned diffuse and glossy materials."""
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(400, 3)); a[:, 2] = np.abs(a[:, 2]); a /= np.linalg.norm(a, axis=1, keepdims=True)
+    b = rng.normal(size=(400, 3)); b[:, 2] = np.abs(b[:, 2]); b /= np.linalg.norm(b, axis=1, keepdims=True)
+    kd, ks, ax, ay = (0.5, 0.4, 0.3), (0.2, 0.6, 0.1), 0.15, 0.35
+    fm, pm = orc.mat_eval(scenes.MAT_MIXTURE, orc._mat_params(kd, ax, ay, ks=ks), N, a, b)
+    fd, pd = orc.mat_eval(scenes.MAT_DIFFUSE, orc._mat_params(kd), N, a, b)
+    fg, pg = orc.mat_eval(scenes.MAT_GLOSSY, orc._mat_params(ks, ax, ay), N, a, b)
+    w = 0.5 / (0.5 + 0.6)
+    assert np.allclose(fm, fd + fg, rtol=1e-12, atol=1e-14)
+    assert np.allclose(pm, w * pd + (1 - w) * pg, rtol=1e-12, atol=1e-14)
+    # black Kd and Ks: the diffuse lobe is chosen (w = 1), f = 0
+    f0, p0 = orc.mat_eval(scenes.MAT_MIXTURE, orc._mat_params((0, 0, 0), ax, ay), N, a, b)
+    assert np.all(f0 == 0) and np.allclose(p0, 1 / np.pi)
+
+
+def test_mixture_alpha_mask_components():
+    """Alpha mask: component 0 (diffuse+glossy) with probability alpha and
+    weight 1/alpha, component 1 (pass-through, specular) otherwise with
+    weight 1/(1-alpha) and direction -wi; the reference's pass-through
+    direction weight is 1 (model_wavefrontobj.cpp:466-473), so comp 0's
+    total weight is f_sum / pdf and comp 1's is 1/(1-alpha)."""
+    rng = np.random.default_rng(4)
+    n, alpha = 50000, 0.3
+    kd, ks = (0.5, 0.4, 0.3), (0.2, 0.2, 0.2)
+    p = orc._mat_params(kd, 0.2, 0.2, ks=ks, alpha=alpha)
+    wi = np.repeat(np.array([[0.3, 0.2, math.sqrt(1 - 0.13)]]), n, 0)
+    wo, w, comp, ok = orc.mat_sample(scenes.MAT_MIXTURE, p, N, wi, rng.random((n, 4)))
+    frac = (comp == 1).mean()
+    assert abs(frac - (1 - alpha)) < 5 * math.sqrt(alpha * (1 - alpha) / n)
+    c1 = comp == 1
+    assert np.allclose(wo[c1], -wi[c1]) and np.allclose(w[c1], 1 / (1 - alpha))
+    c0 = (comp == 0) & ok
+    f, pdf = orc.mat_eval(scenes.MAT_MIXTURE, p, N, wi[c0], wo[c0])   # includes the alpha factor
+    good = pdf > 1e-6
+    assert np.allclose(w[c0][good], f[good] / alpha / pdf[good][:, None], rtol=1e-6)
 
 
 def test_mirror_and_glass():
