@@ -276,11 +276,25 @@ typedef struct lmgpu_lt_params {
     int32_t pass_offset;
     uint64_t seed;
     double render_time;
-    int32_t reserved[8];
+    int32_t min_verts;        /* renderer::bdpt: skip strategies of fewer vertices (0 = 2); unused by LT */
+    int32_t reserved[7];
 } lmgpu_lt_params;
 
 int lmgpu_lt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
                     lmgpu_pt_stats* stats, void* stream);
+
+/* renderer::bdpt (src/renderer/renderer_bdpt.cpp:339-411, include/lm/bidir.h):
+ * per sample an eye subpath (pinhole, raster uniform over the image) and a
+ * light subpath (area lights) of at most max_verts vertices each, without
+ * Russian roulette; every connection strategy (s, t), 2 <= s + t <= max_verts,
+ * weighted by the power heuristic over all strategies of its full path
+ * (Path::eval_mis_weight) and splatted at its raster position. Same film and
+ * scheduler contract as lmgpu_lt_render (film value = film * (w*h) /
+ * stats->samples; stats->spp counts the passes, of at most 2^18 samples).
+ * Scenes with an environment light are rejected (LMGPU_ERR_UNSUPPORTED).
+ * max_verts in [2, 32]. Synchronous. */
+int lmgpu_bdpt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
+                      lmgpu_pt_stats* stats, void* stream);
 
 /* ---- batched rendering ------------------------------------------------------
  * Many renderer::pt jobs (each its own scene, film size, spp and seed) in

@@ -306,6 +306,518 @@ void lt_pass(Scene& S, const LTConsts& c, double* film, int refill, hipStream_t 
     }
 }
 
+// ---------------------------------------------------------------------------
+// renderer::bdpt (src/renderer/renderer_bdpt.cpp:339-411, include/lm/bidir.h)
+//
+// Per sample a light subpath y_0..y_{nL-1} (y_0 on an area light) and an eye
+// subpath z_0..z_{nE-1} (z_0 the pinhole, its raster uniform over the image)
+// of at most max_verts vertices each, no Russian roulette; every strategy
+// (s, t), 2 <= s + t <= max_verts, t >= 1, weighted by the power heuristic
+// over all strategies of its full path (Path::eval_mis_weight). The weights
+// are accumulated along the subpaths instead of re-walking the path per
+// strategy: with q_j = pE(y_j) / pL(y_j) (area pdfs of sampling y_j from the
+// eye side over the light side) and u_k = pL(z_k) / pE(z_k),
+//   1/w = 1 + q_{s-1}^2 (samp(s-1) + q_{s-2}^2 T(s-2)) + u_{t-1}^2 (samp'(t-1) + u_{t-2}^2 V(t-2)),
+//   T(j) = samp(j) + q_{j-1}^2 T(j-1), T(0) = 1,  V(k) = samp'(k) + u_{k-1}^2 V(k-1), V(0) = 0,
+// where samp(j) / samp'(k) say whether the strategy connecting the two
+// vertices around that edge is samplable (both non-specular;
+// Path::is_samplable_bidir) and the connection-dependent pdfs are evaluated
+// at the connection. Component pdfs are symmetric and cancel in every ratio.
+// Light vertices are stored (LVert) for the eye walk; the eye walk carries
+// its accumulator in the path state.
+
+struct alignas(16) LVert {
+    float4 a;   // position, primitive bits (light endpoint: -1 - light index)
+    float4 b;   // shading normal, triangle bits
+    float4 c;   // geometric normal, barycentric b1
+    float4 d;   // direction toward the previous vertex, barycentric b2
+    float4 e;   // subpath weight (1 / p_comp of this vertex included), w = flags bits
+    float4 f;   // pL, pL of the previous vertex, G(previous, this), T(j - 1)
+};
+
+constexpr int kFlagSpec = 1 << 8;        // this vertex's component is specular
+constexpr int kFlagSpecPrev = 1 << 9;    // the previous vertex's component is specular
+
+struct BDConsts {
+    int max_verts;
+    int min_verts;    // strategies with fewer vertices are skipped (Renderer_Path_Base, renderer_bdpt.cpp:49)
+    int npaths;
+    uint32_t pass;
+    uint32_t seed_lo, seed_hi;
+    int lstride;      // LVert slots per path (max_verts - 1)
+};
+
+__device__ __forceinline__ uint64_t bd_seed(const BDConsts& c) { return (uint64_t(c.seed_hi) << 32) | c.seed_lo; }
+
+// camera::pinhole: raster position of the direction `woc` (camera -> point)
+// and its importance J (camera_pinhole.cpp:102-109, 207-226).
+__device__ __forceinline__ bool cam_raster(const Camera& cam, f3 woc, float& rx, float& ry) {
+    const float ez = dot(woc, cam.w);
+    if (!(ez < 0.f)) return false;
+    rx = -dot(woc, cam.u) / ez / cam.tf / cam.aspect * .5f + .5f;
+    ry = -dot(woc, cam.v) / ez / cam.tf * .5f + .5f;
+    return rx >= 0.f && rx <= 1.f && ry >= 0.f && ry <= 1.f;
+}
+__device__ __forceinline__ float cam_J(const Camera& cam, f3 woc) {
+    const float ct = -dot(woc, cam.w);
+    return 1.f / (ct * ct * ct * (4.f * cam.tf * cam.tf * cam.aspect));
+}
+
+__device__ __forceinline__ Geom geom_of(const LVert& v) {
+    return make_on_surface(mk(v.a.x, v.a.y, v.a.z), mk(v.b.x, v.b.y, v.b.z), mk(v.c.x, v.c.y, v.c.z));
+}
+
+// Light subpath start: light::area::sample_ray (light_area.cpp:78-100) with
+// the uniform light selection; y_0 is stored, the ray goes to the queue.
+__global__ __launch_bounds__(kBlock) void k_bd_lgen(SceneDev sc, PathState ps, Queues q, BDConsts c,
+                                                    LVert* __restrict__ lv, int32_t* __restrict__ lcount) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.npaths) return;
+    const uint64_t seed = bd_seed(c);
+    const rand4 r0 = rng(uint32_t(i), c.pass, 0u, 0u, seed);
+    const rand4 r1 = rng(uint32_t(i), c.pass, 0u, 1u, seed);
+    const int li = min(max(int(r0.u[0] * sc.num_area_lights), 0), sc.num_area_lights - 1);
+    const DevLight dl = sc.lights[li];
+    const float* cdf = sc.light_cdf + dl.cdf_off;
+    int lo = 0, hi = dl.count + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= r0.u[1]) lo = mid + 1;
+        else hi = mid;
+    }
+    const int k = min(max(lo - 1, 0), dl.count - 1);
+    const ShadeTri ls = sc.stri[sc.light_tri[dl.cdf_off - li + k]];
+    const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z), C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
+    const float s = sqrtf(fmaxf(0.f, r0.u[2]));
+    const float bu = 1.f - s, bv = r0.u[3] * s;
+    const f3 org = A * (1.f - bu - bv) + B * bu + C * bv;
+    const f3 nL = normalize(cross(B - A, C - A));
+    f3 u, v;
+    onb(nL, u, v);
+    const f3 dl_ = sample_cosine_weighted(r1.u[0], r1.u[1]);
+    const f3 d = u * dl_.x + v * dl_.y + nL * dl_.z;
+    // pL(y_0) = p_sel / area (path::pdf_position of the light endpoint)
+    const float pL0 = dl.Ke_invA.w / float(sc.num_lights);
+    const float a0 = 1.f / pL0;
+    LVert y;
+    y.a = make_float4(org.x, org.y, org.z, __int_as_float(-1 - li));
+    y.b = make_float4(nL.x, nL.y, nL.z, __int_as_float(-1));
+    y.c = make_float4(nL.x, nL.y, nL.z, 0.f);
+    y.d = make_float4(0.f, 0.f, 0.f, 0.f);
+    y.e = make_float4(a0, a0, a0, __int_as_float(0));
+    y.f = make_float4(pL0, 0.f, 0.f, 0.f);
+    lv[int64_t(i) * c.lstride] = y;
+    lcount[i] = 1;
+    // weight toward y_1: alpha_0 Le / pdf_dir, pdf_dir = 1 / pi (projected solid angle)
+    const f3 w = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) * (a0 * kPi);
+    ps.thr[i] = make_float4(w.x, w.y, w.z, pL0);
+    ps.L[i] = make_float4(org.x, org.y, org.z, 1.f / kPi);
+    ps.prev[i] = make_float4(fabsf(dot(nL, d)), 1.f, 0.f, 0.f);   // |cos| at y_0, T(0) = 1
+    ps.info[i] = make_int4(1, 0, 0, 0);                          // index of the next vertex, flags
+    lmgpu_ray ray;
+    ray.org[0] = org.x; ray.org[1] = org.y; ray.org[2] = org.z;
+    ray.tmin = kEps;
+    ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+    ray.tmax = kInf;
+    q.ext[0][i] = ray;
+    q.ext_path[0][i] = i;
+    if (i == 0) {
+        q.counts[0] = c.npaths;
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npaths);
+    }
+    // strategy (s, t) = (1, 1): y_0 seen by the pinhole; the only other
+    // strategy of this two-vertex path is the eye ray hitting the light (s = 0)
+    const Camera& cam = sc.cam;
+    const f3 dc = org - cam.pos;
+    const float dc2 = dot(dc, dc);
+    const float dist = sqrtf(dc2);
+    const f3 woc = dc / dist;
+    float rx, ry;
+    if (c.min_verts <= 2 && cam_raster(cam, woc, rx, ry) && dot(-woc, nL) > 0.f) {
+        const float G = fabsf(dot(nL, woc)) / dc2;
+        const float J = cam_J(cam, woc);
+        const float q0 = J * G / pL0;
+        const f3 Cw = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) * (a0 * G * J / (1.f + q0 * q0));
+        const int px = min(max(int(rx * float(sc.w)), 0), sc.w - 1);
+        const int py = min(max(int(ry * float(sc.h)), 0), sc.h - 1);
+        const int ss = atomicAdd(q.counts + 2, 1);
+        lmgpu_ray sr;
+        sr.org[0] = org.x; sr.org[1] = org.y; sr.org[2] = org.z;
+        sr.tmin = kEps;
+        sr.dir[0] = -woc.x; sr.dir[1] = -woc.y; sr.dir[2] = -woc.z;
+        sr.tmax = dist * (1.f - kEps);
+        q.sh[ss] = sr;
+        q.sh_contrib[ss] = make_float4(Cw.x, Cw.y, Cw.z, __int_as_float(py * sc.w + px));
+    }
+}
+
+// A light subpath vertex y_j (j >= 1): store it, connect it to the camera
+// (strategy t = 1) and sample the next direction (path::sample_direction, LE).
+__global__ __launch_bounds__(kBlock) void k_bd_lshade(SceneDev sc, PathState ps, Queues q, BDConsts c, int in_q,
+                                                      LVert* __restrict__ lv, int32_t* __restrict__ lcount) {
+    const int n = q.counts[in_q];
+    const int out_q = in_q ^ 1;
+    const uint64_t seed = bd_seed(c);
+    const int stride = gridDim.x * kBlock;
+    const int n_iter = (n + stride - 1) / stride;
+    for (int it = 0; it < n_iter; it++) {
+        const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
+        bool emit_ext = false, emit_sh = false;
+        lmgpu_ray ext_ray, sh_ray;
+        float4 sh_c = make_float4(0.f, 0.f, 0.f, 0.f);
+        int p = 0;
+        if (i < n) {
+            p = q.ext_path[in_q][i];
+            const float4 rb = reinterpret_cast<const float4*>(q.ext[in_q] + i)[1];
+            const float4 h = reinterpret_cast<const float4*>(q.hit)[i];
+            const int tri = __float_as_int(h.x);
+            if (tri == -2) atomicOr(q.counts + 5, 1);
+            if (tri >= 0) {
+                const float4 thr4 = ps.thr[p], prv = ps.L[p], acc = ps.prev[p];
+                const int4 info = ps.info[p];
+                const int j = info.x;
+                const bool spec_prev = info.y & kFlagSpec;
+                const ShadeTri st = sc.stri[tri];
+                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
+                const f3 P0 = mk(st.p0.x, st.p0.y, st.p0.z), P1 = mk(st.p1.x, st.p1.y, st.p1.z),
+                         P2 = mk(st.p2.x, st.p2.y, st.p2.z);
+                const f3 pos = P0 * b0 + P1 * b1 + P2 * b2;
+                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+                const f3 gn = normalize(cross(P1 - P0, P2 - P0));
+                const Geom g = make_on_surface(pos, nsh, gn);
+                const int prim = __float_as_int(st.p0.w);
+                int light;
+                const Mat m = load_mat(sc, prim, tri, b1, b2, light);
+                const f3 wdir = mk(rb.x, rb.y, rb.z);
+                const f3 wi = -wdir;
+                const f3 pprev = mk(prv.x, prv.y, prv.z);
+                const f3 dv = pos - pprev;
+                const float d2 = dot(dv, dv);
+                const float Gin = acc.x * fabsf(dot(nsh, wdir)) / d2;   // G(y_{j-1}, y_j)
+                const float pL = prv.w * Gin;                           // area pdf from the light side
+                const rand4 r0 = rng(uint32_t(p), c.pass, uint32_t(j), 0u, seed);
+                float cw, ul;
+                const int comp = mat_sample_component(m, g, wi, r0.u[1], cw, ul);
+                const f3 alpha = mk(thr4.x, thr4.y, thr4.z) * cw;
+                const bool spec = is_specular(m, comp);
+                const float Tprev = acc.y;   // T(j - 1)
+                LVert y;
+                y.a = make_float4(pos.x, pos.y, pos.z, __int_as_float(prim));
+                y.b = make_float4(nsh.x, nsh.y, nsh.z, __int_as_float(tri));
+                y.c = make_float4(gn.x, gn.y, gn.z, b1);
+                y.d = make_float4(wi.x, wi.y, wi.z, b2);
+                y.e = make_float4(alpha.x, alpha.y, alpha.z,
+                                  __int_as_float(comp | (spec ? kFlagSpec : 0) | (spec_prev ? kFlagSpecPrev : 0)));
+                y.f = make_float4(pL, thr4.w, Gin, Tprev);
+                lv[int64_t(p) * c.lstride + j] = y;
+                lcount[p] = j + 1;
+                // strategy (s, t) = (j + 1, 1): connect y_j to the pinhole (path::sample_direct, EL)
+                if (!spec && !is_zero(alpha) && j + 2 >= c.min_verts) {
+                    const Camera& cam = sc.cam;
+                    const f3 dc = pos - cam.pos;
+                    const float dc2 = dot(dc, dc);
+                    const float dist = sqrtf(dc2);
+                    const f3 woc = dc / dist;
+                    float rx, ry;
+                    if (cam_raster(cam, woc, rx, ry)) {
+                        const f3 wo = -woc;
+                        const float G = fabsf(dot(nsh, wo)) / dc2;
+                        const float J = cam_J(cam, woc);
+                        const f3 fs = mat_eval(m, g, wi, wo, comp) * sn_correction_le(g, wi, wo);
+                        const f3 Cc = alpha * fs * (J * G);
+                        if (!is_zero(Cc) && G > 0.f) {
+                            // MIS: q_j with pE(y_j) = J G; q_{j-1} through y_j's BSDF toward the camera
+                            const float qj = J * G / pL;
+                            const float pEprev = mat_pdf_full(m, g, wo, wi, comp) * Gin;
+                            const float qp = pEprev / thr4.w;
+                            const float sampj = spec_prev ? 0.f : 1.f;
+                            const float sumL = qj * qj * (sampj + qp * qp * Tprev);
+                            const float wgt = 1.f / (1.f + sumL);
+                            // a grazing edge can leave a zero pdf: no weight, no NaN
+                            emit_sh = wgt >= 0.f && wgt <= 1.f;
+                            const int px = min(max(int(rx * float(sc.w)), 0), sc.w - 1);
+                            const int py = min(max(int(ry * float(sc.h)), 0), sc.h - 1);
+                            sh_ray.org[0] = pos.x; sh_ray.org[1] = pos.y; sh_ray.org[2] = pos.z;
+                            sh_ray.tmin = kEps;
+                            sh_ray.dir[0] = wo.x; sh_ray.dir[1] = wo.y; sh_ray.dir[2] = wo.z;
+                            sh_ray.tmax = dist * (1.f - kEps);
+                            const f3 Cw = Cc * wgt;
+                            sh_c = make_float4(Cw.x, Cw.y, Cw.z, __int_as_float(py * sc.w + px));
+                        }
+                    }
+                }
+                // next light vertex y_{j+1} (usable while j + 2 <= max_verts - 1)
+                if (j + 2 <= c.max_verts - 1 && !is_zero(alpha)) {
+                    f3 wo2, sw;
+                    if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], ul, wo2, sw)) {
+                        const f3 f = mat_eval_full(m, g, wi, wo2, comp, true) * sn_correction_le(g, wi, wo2);
+                        const float pd = mat_pdf_full(m, g, wi, wo2, comp);
+                        const f3 a2 = pd > 0.f ? alpha * f / pd : mk(0, 0, 0);
+                        if (!is_zero(a2)) {
+                            // T(j) = samp(j) + q_{j-1}^2 T(j-1), q_{j-1} through y_j toward y_{j+1}
+                            const float pEprev = mat_pdf_full(m, g, wo2, wi, comp) * Gin;
+                            const float qp = pEprev / thr4.w;
+                            const float Tj = ((spec_prev || spec) ? 0.f : 1.f) + qp * qp * Tprev;
+                            ps.thr[p] = make_float4(a2.x, a2.y, a2.z, pL);
+                            ps.L[p] = make_float4(pos.x, pos.y, pos.z, pd);
+                            ps.prev[p] = make_float4(fabsf(dot(nsh, wo2)), Tj, 0.f, 0.f);
+                            ps.info[p] = make_int4(j + 1, spec ? kFlagSpec : 0, 0, 0);
+                            emit_ext = true;
+                            ext_ray.org[0] = pos.x; ext_ray.org[1] = pos.y; ext_ray.org[2] = pos.z;
+                            ext_ray.tmin = kEps;
+                            ext_ray.dir[0] = wo2.x; ext_ray.dir[1] = wo2.y; ext_ray.dir[2] = wo2.z;
+                            ext_ray.tmax = kInf;
+                        }
+                    }
+                }
+            }
+        }
+        const int se = wave_append(q.counts + out_q, emit_ext);
+        if (emit_ext) {
+            q.ext[out_q][se] = ext_ray;
+            q.ext_path[out_q][se] = p;
+        }
+        const int ss = wave_append(q.counts + 2, emit_sh);
+        if (emit_sh) {
+            q.sh[ss] = sh_ray;
+            q.sh_contrib[ss] = sh_c;
+        }
+    }
+}
+
+// Eye subpath start: camera::pinhole::sample_ray (camera_pinhole.cpp:134-140),
+// a raster position uniform over the image from (sample, pass, 0, 2)[0..1].
+__global__ __launch_bounds__(kBlock) void k_bd_egen(SceneDev sc, PathState ps, Queues q, BDConsts c) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.npaths) return;
+    const rand4 r = rng(uint32_t(i), c.pass, 0u, 2u, bd_seed(c));
+    const f3 d = primary_dir(sc.cam, r.u[0], r.u[1]);
+    const int px = min(max(int(r.u[0] * float(sc.w)), 0), sc.w - 1);
+    const int py = min(max(int(r.u[1] * float(sc.h)), 0), sc.h - 1);
+    // z_0: pE = 1 (pdf_position), the sampled direction's pdf is J; V(0) = 0
+    ps.thr[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+    ps.L[i] = make_float4(sc.cam.pos.x, sc.cam.pos.y, sc.cam.pos.z, cam_J(sc.cam, d));
+    ps.prev[i] = make_float4(1.f, 0.f, 0.f, 0.f);
+    ps.info[i] = make_int4(1, 0, py * sc.w + px, 0);
+    lmgpu_ray ray;
+    ray.org[0] = sc.cam.pos.x; ray.org[1] = sc.cam.pos.y; ray.org[2] = sc.cam.pos.z;
+    ray.tmin = kEps;
+    ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+    ray.tmax = kInf;
+    q.ext[0][i] = ray;
+    q.ext_path[0][i] = i;
+    if (i == 0) q.counts[0] = c.npaths;
+}
+
+// An eye subpath vertex z_{t-1} (t >= 2): the emitter strategy (s = 0),
+// the connections to the sample's light vertices (s >= 1) queued as shadow
+// rays with their weighted contributions, and the next direction.
+__global__ __launch_bounds__(kBlock) void k_bd_eshade(SceneDev sc, PathState ps, Queues q, BDConsts c, int in_q,
+                                                      const LVert* __restrict__ lv, const int32_t* __restrict__ lcount,
+                                                      double* __restrict__ film) {
+    const int n = q.counts[in_q];
+    const int out_q = in_q ^ 1;
+    const uint64_t seed = bd_seed(c);
+    const int stride = gridDim.x * kBlock;
+    const int n_iter = (n + stride - 1) / stride;
+    for (int it = 0; it < n_iter; it++) {
+        const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
+        bool emit_ext = false;
+        lmgpu_ray ext_ray;
+        int p = 0;
+        bool active = false;
+        // vertex state shared by the connection loop below
+        Geom g;
+        Mat m;
+        f3 wi = mk(0, 0, 0), alphaE = mk(0, 0, 0);
+        float Gin = 0.f, pEcur = 0.f, pEprev = 1.f, Vprev = 0.f;
+        int comp = 0, t = 0, nl = 0, pixel = 0;
+        bool spec_prev = false, spec = true;
+        if (i < n) {
+            p = q.ext_path[in_q][i];
+            const float4 rb = reinterpret_cast<const float4*>(q.ext[in_q] + i)[1];
+            const float4 h = reinterpret_cast<const float4*>(q.hit)[i];
+            const int tri = __float_as_int(h.x);
+            if (tri == -2) atomicOr(q.counts + 5, 1);
+            if (tri >= 0) {
+                const float4 thr4 = ps.thr[p], prv = ps.L[p], acc = ps.prev[p];
+                const int4 info = ps.info[p];
+                t = info.x + 1;   // z_{t-1} is this vertex
+                spec_prev = info.y & kFlagSpec;
+                pixel = info.z;
+                const ShadeTri st = sc.stri[tri];
+                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
+                const f3 P0 = mk(st.p0.x, st.p0.y, st.p0.z), P1 = mk(st.p1.x, st.p1.y, st.p1.z),
+                         P2 = mk(st.p2.x, st.p2.y, st.p2.z);
+                const f3 pos = P0 * b0 + P1 * b1 + P2 * b2;
+                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+                g = make_on_surface(pos, nsh, normalize(cross(P1 - P0, P2 - P0)));
+                int light;
+                m = load_mat(sc, __float_as_int(st.p0.w), tri, b1, b2, light);
+                const f3 wdir = mk(rb.x, rb.y, rb.z);
+                wi = -wdir;
+                const f3 dv = pos - mk(prv.x, prv.y, prv.z);
+                Gin = acc.x * fabsf(dot(nsh, wdir)) / dot(dv, dv);   // G(z_{t-2}, z_{t-1})
+                pEcur = prv.w * Gin;
+                pEprev = thr4.w;
+                Vprev = acc.y;
+                const f3 thr = mk(thr4.x, thr4.y, thr4.z);
+                // s = 0: the eye subpath hits a light (z_{t-1} becomes the light endpoint)
+                if (light >= 0 && dot(wi, nsh) > 0.f && t >= c.min_verts) {
+                    const DevLight dl = sc.lights[light];
+                    const f3 Le = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z);
+                    const float u1 = (dl.Ke_invA.w / float(sc.num_lights)) / pEcur;   // pL(z_{t-1}) / pE
+                    const float u2 = (1.f / kPi) * Gin / pEprev;                      // pL(z_{t-2}) / pE
+                    const float sumE = u1 * u1 * ((spec_prev ? 0.f : 1.f) + u2 * u2 * Vprev);
+                    const float wgt = 1.f / (1.f + sumE);
+                    const f3 C = thr * Le * wgt;
+                    if (!is_zero(C) && wgt >= 0.f && wgt <= 1.f) {
+                        double* f = film + 3 * int64_t(pixel);
+                        atomicAdd(f + 0, double(C.x));
+                        atomicAdd(f + 1, double(C.y));
+                        atomicAdd(f + 2, double(C.z));
+                    }
+                }
+                const rand4 r0 = rng(uint32_t(p), c.pass, uint32_t(t - 1), 2u, seed);
+                float cw, ul;
+                comp = mat_sample_component(m, g, wi, r0.u[1], cw, ul);
+                alphaE = thr * cw;
+                spec = is_specular(m, comp);
+                nl = lcount[p];
+                active = !is_zero(alphaE);
+                // next eye vertex z_t (while t + 1 <= max_verts)
+                if (active && t + 1 <= c.max_verts) {
+                    f3 wo2, sw;
+                    if (mat_sample_direction(m, g, wi, comp, r0.u[2], r0.u[3], ul, wo2, sw)) {
+                        const f3 f = mat_eval_full(m, g, wi, wo2, comp, false) * sn_correction(g, wi, wo2);
+                        const float pd = mat_pdf_full(m, g, wi, wo2, comp);
+                        const f3 a2 = pd > 0.f ? alphaE * f / pd : mk(0, 0, 0);
+                        if (!is_zero(a2)) {
+                            // V(t-1) = samp'(t-1) + u_{t-2}^2 V(t-2), u_{t-2} through z_{t-1} toward z_t
+                            const float u2 = mat_pdf_full(m, g, wo2, wi, comp) * Gin / pEprev;
+                            const float Vt = ((spec_prev || spec) ? 0.f : 1.f) + u2 * u2 * Vprev;
+                            ps.thr[p] = make_float4(a2.x, a2.y, a2.z, pEcur);
+                            ps.L[p] = make_float4(pos.x, pos.y, pos.z, pd);
+                            ps.prev[p] = make_float4(fabsf(dot(nsh, wo2)), Vt, 0.f, 0.f);
+                            ps.info[p] = make_int4(t, spec ? kFlagSpec : 0, pixel, 0);
+                            emit_ext = true;
+                            ext_ray.org[0] = pos.x; ext_ray.org[1] = pos.y; ext_ray.org[2] = pos.z;
+                            ext_ray.tmin = kEps;
+                            ext_ray.dir[0] = wo2.x; ext_ray.dir[1] = wo2.y; ext_ray.dir[2] = wo2.z;
+                            ext_ray.tmax = kInf;
+                        }
+                    }
+                }
+            }
+        }
+        const int se = wave_append(q.counts + out_q, emit_ext);
+        if (emit_ext) {
+            q.ext[out_q][se] = ext_ray;
+            q.ext_path[out_q][se] = p;
+        }
+        // connections to light vertices y_j, s = j + 1 <= max_verts - t (every lane runs
+        // the same loop so the wave-level appends see the whole wave)
+        const bool can = active && !spec;
+        for (int j = 0; j < c.lstride; j++) {
+            bool emit_sh = false;
+            lmgpu_ray sh_ray;
+            float4 sh_c;
+            if (can && j < nl && j + 1 + t <= c.max_verts && j + 1 + t >= c.min_verts) {
+                const LVert y = lv[int64_t(p) * c.lstride + j];
+                const int yflags = __float_as_int(y.e.w);
+                if (!(yflags & kFlagSpec)) {
+                    const f3 ypos = mk(y.a.x, y.a.y, y.a.z);
+                    const f3 yn = mk(y.b.x, y.b.y, y.b.z);
+                    const f3 dv = ypos - g.p;
+                    const float d2 = dot(dv, dv);
+                    const float dist = sqrtf(d2);
+                    const f3 dir = dv / dist;   // z -> y
+                    const float G = fabsf(dot(g.n, dir)) * fabsf(dot(yn, dir)) / d2;
+                    const f3 fsE = mat_eval(m, g, wi, dir, comp) * sn_correction(g, wi, dir);
+                    f3 fsL;
+                    float pLz;   // pL(z_{t-1}): y's pdf toward z, to area
+                    float pEy_prev = 0.f;   // pE(y_{j-1}) through y toward z
+                    if (j == 0) {
+                        const int li = -1 - __float_as_int(y.a.w);
+                        const float4 ke = sc.lights[li].Ke_invA;
+                        const bool front = dot(-dir, yn) > 0.f;
+                        fsL = front ? mk(ke.x, ke.y, ke.z) : mk(0, 0, 0);
+                        pLz = front ? (1.f / kPi) * G : 0.f;
+                    } else {
+                        int ylight;
+                        const Mat ym = load_mat(sc, __float_as_int(y.a.w), __float_as_int(y.b.w), y.c.w, y.d.w, ylight);
+                        const Geom yg = geom_of(y);
+                        const f3 ywi = mk(y.d.x, y.d.y, y.d.z);
+                        const int ycomp = yflags & 0xff;
+                        fsL = mat_eval(ym, yg, ywi, -dir, ycomp) * sn_correction_le(yg, ywi, -dir);
+                        pLz = mat_pdf_full(ym, yg, ywi, -dir, ycomp) * G;
+                        pEy_prev = mat_pdf_full(ym, yg, -dir, ywi, ycomp) * y.f.z;
+                    }
+                    const f3 C = mk(y.e.x, y.e.y, y.e.z) * fsL * G * fsE * alphaE;
+                    if (!is_zero(C) && G > 0.f) {
+                        // light side: q_j, q_{j-1}, T(j-1)
+                        const float qj = mat_pdf_full(m, g, wi, dir, comp) * G / y.f.x;
+                        float inner = 1.f;   // samp(0) for j = 0
+                        if (j > 0) {
+                            const float qp = pEy_prev / y.f.y;
+                            inner = ((yflags & kFlagSpecPrev) ? 0.f : 1.f) + qp * qp * y.f.w;
+                        }
+                        const float sumL = qj * qj * inner;
+                        // eye side: u_{t-1}, u_{t-2}, V(t-2)
+                        const float u1 = pLz / pEcur;
+                        const float u2 = mat_pdf_full(m, g, dir, wi, comp) * Gin / pEprev;
+                        const float sumE = u1 * u1 * ((spec_prev ? 0.f : 1.f) + u2 * u2 * Vprev);
+                        const float wgt = 1.f / (1.f + sumL + sumE);
+                        const f3 Cw = C * wgt;
+                        emit_sh = wgt >= 0.f && wgt <= 1.f;
+                        // shadow ray from the light vertex, as Scene::visible(vL, vE)
+                        sh_ray.org[0] = ypos.x; sh_ray.org[1] = ypos.y; sh_ray.org[2] = ypos.z;
+                        sh_ray.tmin = kEps;
+                        sh_ray.dir[0] = -dir.x; sh_ray.dir[1] = -dir.y; sh_ray.dir[2] = -dir.z;
+                        sh_ray.tmax = dist * (1.f - kEps);
+                        sh_c = make_float4(Cw.x, Cw.y, Cw.z, __int_as_float(pixel));
+                    }
+                }
+            }
+            const int ss = wave_append(q.counts + 2, emit_sh);
+            if (emit_sh) {
+                q.sh[ss] = sh_ray;
+                q.sh_contrib[ss] = sh_c;
+            }
+        }
+    }
+}
+
+template <int W>
+void bd_pass(Scene& S, const BDConsts& c, double* film, int refill, LVert* lv, hipStream_t s) {
+    PathState ps{S.thr.p, S.L.p, S.prev.p, S.info.p};
+    Queues q{{S.ext0.p, S.ext1.p}, S.hit.p, {S.path0.p, S.path1.p}, S.bsh.p, S.bshc.p, S.counts.p};
+    const unsigned gb = unsigned((c.npaths + kBlock - 1) / kBlock);
+    const int sg = std::max(1, std::min((c.npaths + kBlock - 1) / kBlock, 2048));
+    const int tg = extend_grid<W>();
+    const int cg = lt_connect_grid<W>();
+    hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, 1);
+    hipLaunchKernelGGL(k_bd_lgen, dim3(gb), dim3(kBlock), 0, s, S.dev, ps, q, c, lv, S.blcount.p);
+    hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+    for (int b = 0; b < c.max_verts - 2; b++) {   // y_1 .. y_{max_verts - 2}
+        const int in_q = b & 1;
+        hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
+        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill);
+        hipLaunchKernelGGL(k_bd_lshade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, lv, S.blcount.p);
+        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+    }
+    hipLaunchKernelGGL(k_bd_egen, dim3(gb), dim3(kBlock), 0, s, S.dev, ps, q, c);
+    for (int b = 0; b < c.max_verts - 1; b++) {   // z_1 .. z_{max_verts - 1}
+        const int in_q = b & 1;
+        hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
+        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill);
+        hipLaunchKernelGGL(k_bd_eshade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, lv, S.blcount.p, film);
+        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+    }
+}
+
 }  // namespace
 
 int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, lmgpu_pt_stats* st, hipStream_t s) {
@@ -372,7 +884,89 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
     return LMGPU_OK;
 }
 
+int Scene::bdpt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, lmgpu_pt_stats* st, hipStream_t s) {
+    if (!p) return fail(LMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!film_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film is NULL");
+    if (p->max_verts < 2 || p->max_verts > 32) return fail(LMGPU_ERR_INVALID_ARGUMENT, "max_verts must be in [2, 32]");
+    if (!(p->render_time > 0) && p->num_samples <= 0)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "num_samples must be positive");
+    if (p->pass_offset < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "pass_offset must be >= 0");
+    if (dev.has_env)
+        return fail(LMGPU_ERR_UNSUPPORTED, "bidirectional path tracing with an environment light is not supported");
+    if (dev.num_area_lights == 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bidirectional path tracing needs an area light");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int npix = w * h;
+    LMGPU_TRY(ensure_workspace(npix));
+    // samples per pass: bounded so that the stored light vertices and the
+    // connection queue stay modest (P (max_verts - 1) entries each)
+    const int P = std::min(npix, 1 << 18);
+    const int lstride = p->max_verts - 1;
+    const size_t nv = size_t(P) * lstride;
+    if (blverts.n < nv * sizeof(LVert) / sizeof(float4)) LMGPU_TRY(blverts.alloc(nv * sizeof(LVert) / sizeof(float4)));
+    if (bsh.n < nv) {
+        LMGPU_TRY(bsh.alloc(nv));
+        LMGPU_TRY(bshc.alloc(nv));
+    }
+    if (blcount.n < size_t(P)) LMGPU_TRY(blcount.alloc(P));
+    double* film = film_in;
+    if (!on_device) {
+        if (fsum.n != size_t(npix) * 3) LMGPU_TRY(fsum.alloc(size_t(npix) * 3));
+        film = fsum.p;
+        LMGPU_HIP(hipMemsetAsync(film, 0, sizeof(double) * 3 * size_t(npix), s));
+    }
+    LMGPU_HIP(hipMemsetAsync(counts.p, 0, 16 * sizeof(int32_t), s));
+    BDConsts c{};
+    c.max_verts = p->max_verts;
+    c.min_verts = p->min_verts > 0 ? p->min_verts : 2;
+    c.seed_lo = uint32_t(p->seed);
+    c.seed_hi = uint32_t(p->seed >> 32);
+    c.lstride = lstride;
+    LVert* lv = reinterpret_cast<LVert*>(blverts.p);
+    int64_t processed = 0, passes = 0;
+    uint32_t pass = uint32_t(p->pass_offset);
+    const int refill = accel.refill;
+    while (true) {
+        const int64_t left = p->render_time > 0 ? int64_t(P) : p->num_samples - processed;
+        if (left <= 0) break;
+        c.npaths = int(std::min<int64_t>(left, P));
+        c.pass = pass++;
+        if (accel.width == 4) bd_pass<4>(*this, c, film, refill, lv, s);
+        else if (accel.width == 8) bd_pass<8>(*this, c, film, refill, lv, s);
+        else bd_pass<2>(*this, c, film, refill, lv, s);
+        LMGPU_HIP(hipGetLastError());
+        processed += c.npaths;
+        passes++;
+        if (p->render_time > 0) {
+            LMGPU_HIP(hipStreamSynchronize(s));
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= p->render_time) break;
+        }
+    }
+    int32_t cnt[16];
+    LMGPU_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+    LMGPU_HIP(hipStreamSynchronize(s));
+    if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during BDPT");
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        int64_t v[3];
+        std::memcpy(v, cnt + 8, sizeof(v));
+        st->samples = processed;
+        st->spp = passes;
+        st->extension_rays = v[1];
+        st->shadow_rays = v[2];
+        st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return LMGPU_OK;
+}
+
 }  // namespace lmgpu
+
+extern "C" int lmgpu_bdpt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
+                                 lmgpu_pt_stats* stats, void* stream) {
+    if (!scene) return lmgpu::fail(LMGPU_ERR_INVALID_ARGUMENT, "scene is NULL");
+    lmgpu::clear_error();
+    return scene->s.bdpt_render(params, film, film_on_device != 0, stats, static_cast<hipStream_t>(stream));
+}
 
 extern "C" int lmgpu_lt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
                                lmgpu_pt_stats* stats, void* stream) {

@@ -70,8 +70,13 @@ struct Scene {
     int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
                lmgpu_pt_stats* st, hipStream_t s);
     int raycast(const lmgpu_raycast_params* p, double* film, bool on_device, hipStream_t s);
-    // renderer::lt (lt.hip)
+    // renderer::lt and renderer::bdpt (lt.hip)
+    DevBuf<float4> blverts;            // BDPT light vertices, [samples per pass][max_verts - 1]
+    DevBuf<lmgpu_ray> bsh;             // BDPT connection queue
+    DevBuf<float4> bshc;
+    DevBuf<int32_t> blcount;           // BDPT light subpath lengths
     int lt_render(const lmgpu_lt_params* p, double* film, bool on_device, lmgpu_pt_stats* st, hipStream_t s);
+    int bdpt_render(const lmgpu_lt_params* p, double* film, bool on_device, lmgpu_pt_stats* st, hipStream_t s);
 };
 
 }  // namespace lmgpu

@@ -298,6 +298,29 @@ __device__ __forceinline__ bool mat_sample_direction(const Mat& m, const Geom& g
     return true;
 }
 
+// Material::eval with eval_delta = false (path-space estimators): specular
+// components evaluate to their sampled weight (mirror 1, glass Fr or
+// (1 - Fr) with eta^2 for radiance transport only, material_glass.cpp:85-87,
+// 231-286), the mixture's pass-through to 1 - alpha.
+__device__ __forceinline__ f3 mat_eval_full(const Mat& m, const Geom& g, f3 wi, f3 wo, int comp, bool le) {
+    if (m.type == LMGPU_MAT_MIRROR) return mk(1, 1, 1);
+    if (m.type == LMGPU_MAT_GLASS) {
+        f3 wt;
+        float eta;
+        const float Fr = glass_Fr(m, g, wi, wt, eta);
+        const float w = comp == 0 ? Fr : (1.f - Fr) * (le ? 1.f : eta * eta);
+        return mk(w, w, w);
+    }
+    if (m.type == LMGPU_MAT_MIXTURE && comp != 0) return mk(1.f - m.alpha, 1.f - m.alpha, 1.f - m.alpha);
+    return mat_eval(m, g, wi, wo, comp);
+}
+
+// Material::pdf_direction with eval_delta = false: 1 for specular components.
+__device__ __forceinline__ float mat_pdf_full(const Mat& m, const Geom& g, f3 wi, f3 wo, int comp) {
+    if (is_specular(m, comp)) return 1.f;
+    return mat_pdf(m, g, wi, wo, comp);
+}
+
 // ---- camera ------------------------------------------------------------------
 
 struct Camera {

@@ -39,7 +39,7 @@ EXPORTED = [
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
     'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
-    'lmgpu_raycast_render', 'lmgpu_lt_render',
+    'lmgpu_raycast_render', 'lmgpu_lt_render', 'lmgpu_bdpt_render',
     'lmgpu_batch_create', 'lmgpu_batch_destroy', 'lmgpu_batch_add_job', 'lmgpu_batch_run', 'lmgpu_batch_job_film',
 ]
 
@@ -125,7 +125,8 @@ class RaycastParams(ctypes.Structure):
 
 class LTParams(ctypes.Structure):
     _fields_ = [('num_samples', ctypes.c_int64), ('max_verts', ctypes.c_int32), ('pass_offset', ctypes.c_int32),
-                ('seed', ctypes.c_uint64), ('render_time', ctypes.c_double), ('reserved', ctypes.c_int32 * 8)]
+                ('seed', ctypes.c_uint64), ('render_time', ctypes.c_double), ('min_verts', ctypes.c_int32),
+                ('reserved', ctypes.c_int32 * 7)]
 
 
 class JobDesc(ctypes.Structure):
@@ -203,6 +204,7 @@ def lib():
         L.lmgpu_guide_stats.argtypes = [vp, ctypes.POINTER(i64), f32p]
         L.lmgpu_raycast_render.argtypes = [vp, ctypes.POINTER(RaycastParams), vp, ctypes.c_int, vp]
         L.lmgpu_lt_render.argtypes = [vp, ctypes.POINTER(LTParams), vp, ctypes.c_int, ctypes.POINTER(PTStats), vp]
+        L.lmgpu_bdpt_render.argtypes = [vp, ctypes.POINTER(LTParams), vp, ctypes.c_int, ctypes.POINTER(PTStats), vp]
         L.lmgpu_batch_create.argtypes = [ctypes.POINTER(vp)]
         L.lmgpu_batch_destroy.argtypes = [vp]
         L.lmgpu_batch_add_job.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int32)]

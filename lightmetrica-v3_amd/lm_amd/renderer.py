@@ -226,6 +226,30 @@ class GpuScene:
                  'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s}
         return s, stats
 
+    def render_bdpt(self, num_samples: int = 0, max_verts: int = 8, seed: int = 0, pass_offset: int = 0,
+                    render_time: float = 0.0, min_verts: int = 2, film=None, stream=None):
+        """lmgpu_bdpt_render: splat sums of `num_samples` bidirectional
+        samples (passes of min(w*h, 2^18)); same film contract as render_lt."""
+        p = _lib.LTParams()
+        p.num_samples = int(num_samples)
+        p.max_verts = int(max_verts)
+        p.pass_offset = int(pass_offset)
+        p.seed = int(seed)
+        p.render_time = float(render_time)
+        p.min_verts = int(min_verts)
+        st = _lib.PTStats()
+        if film is not None and hasattr(film, 'data_ptr'):
+            _lib.call('lmgpu_bdpt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()), 1,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+            s = film
+        else:
+            s = np.zeros((self.h, self.w, 3), dtype=np.float64)
+            _lib.call('lmgpu_bdpt_render', self._h, ctypes.byref(p), s.ctypes.data_as(ctypes.c_void_p), 0,
+                      ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
+        stats = {'samples': st.samples, 'passes': st.spp, 'extension_rays': st.extension_rays,
+                 'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s}
+        return s, stats
+
 
 class RendererRaycast:
     """renderer::raycast_mi355x: drop-in for renderer::raycast
@@ -331,4 +355,42 @@ class RendererLT:
         processed = st['samples']
         # Film::rescale(w*h / processed) of the splatted sums (renderer_lt.cpp:166)
         film.data[:] = s * (film.num_pixels() / processed)
+        return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}
+
+
+class RendererBDPT:
+    """renderer::bdpt_mi355x: drop-in for renderer::bdpt (bidirectional path
+    tracing with power-heuristic MIS, src/renderer/renderer_bdpt.cpp:339-411)
+    with the keys of Renderer_Path_Base::construct (:43-55): max_verts,
+    min_verts (2), seed, scheduler (sample: num_samples | time: render_time)."""
+
+    def __init__(self, scene: GpuScene, output: Film, max_verts: int, seed: Optional[int] = None,
+                 scheduler: str = 'sample', num_samples: Optional[int] = None, render_time: Optional[float] = None,
+                 min_verts: int = 2):
+        if scheduler == 'sample' and not num_samples:
+            raise ValueError('scheduler "sample" needs num_samples')
+        if scheduler == 'time' and not render_time:
+            raise ValueError('scheduler "time" needs render_time')
+        if scheduler not in ('sample', 'time'):
+            raise ValueError(f'invalid scheduler {scheduler!r}')
+        if (output.w, output.h) != (scene.w, scene.h):
+            raise ValueError('film size differs from the scene film size')
+        self.scene = scene
+        self.film = output
+        self.max_verts = int(max_verts)
+        self.min_verts = int(min_verts)
+        self.seed = int(seed) if seed is not None else int(np.random.SeedSequence().generate_state(1)[0])
+        self.scheduler = scheduler
+        self.num_samples = int(num_samples or 0)
+        self.render_time = float(render_time or 0.0)
+
+    def render(self) -> dict:
+        film = self.film
+        film.clear()
+        s, st = self.scene.render_bdpt(
+            num_samples=self.num_samples if self.scheduler == 'sample' else 0, max_verts=self.max_verts,
+            seed=self.seed, render_time=self.render_time if self.scheduler == 'time' else 0.0,
+            min_verts=self.min_verts)
+        processed = st['samples']
+        film.data[:] = s * (film.num_pixels() / processed)   # renderer_bdpt.cpp:398-399
         return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}

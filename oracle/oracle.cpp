@@ -350,6 +350,7 @@ struct OrcLTParams {
     int32_t pass_offset;
     uint64_t seed;
     int32_t nthreads;
+    int32_t min_verts;   // renderer::bdpt (0 = 2)
 };
 
 struct OrcPTStats {
@@ -1335,6 +1336,309 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
                 sp = nh.g;
                 sp_mat = nh.mat;
                 comp = c2;
+            }
+        }
+    }
+    if (stats) {
+        stats->samples += P->num_samples;
+        stats->extension_rays += n_ext;
+        stats->shadow_rays += n_shadow;
+        stats->path_vertices += n_vert;
+    }
+    return 0;
+}
+
+// ---- renderer::bdpt -------------------------------------------------------------
+
+// A vertex of a subpath or a full path (Vert, include/lm/bidir.h:23-39).
+struct BVert {
+    enum Kind { Camera, Light, Surface } kind = Surface;
+    Geom g;
+    Mat mat{};
+    int comp = 0;
+    double ul = 0;    // the component's random number rescaled (a mixture's lobe selection)
+    int light = -1;   // area light of the primitive (-1: none)
+};
+
+// Material::eval with eval_delta = false of component `comp`, with the
+// transport direction's refraction scaling (material_glass.cpp:85-87) and
+// the mixture's alpha pass-through (model_wavefrontobj.cpp:495-506).
+V3 mat_eval_full(const Mat& m, const Geom& g, V3 wi, V3 wo, int comp, bool le) {
+    switch (m.type) {
+        case MAT_MIRROR: return V3(1);
+        case MAT_GLASS: {
+            V3 wt; bool total; double eta;
+            const double Fr = glass_Fr(m, g, wi, wt, total, eta);
+            if (comp == 0) return V3(Fr);
+            return V3((1 - Fr) * (le ? 1.0 : eta * eta));
+        }
+        case MAT_MIXTURE:
+            if (comp != 0) return V3(1 - m.alpha);
+            return mat_eval_delta(m, g, wi, wo, 0);
+        default: return mat_eval_delta(m, g, wi, wo, comp);
+    }
+}
+double mat_pdf_full(const Mat& m, const Geom& g, V3 wi, V3 wo, int comp) {
+    if (m.type == MAT_MIRROR || m.type == MAT_GLASS || (m.type == MAT_MIXTURE && comp != 0)) return 1;
+    return mat_pdf_delta(m, g, wi, wo, comp);
+}
+
+struct BDPT {
+    const Scene& S;
+    int w, h;
+    // camera::pinhole raster position and importance (camera_pinhole.cpp:102-109, 207-226)
+    bool raster(V3 wo, double& rx, double& ry) const {
+        const double ez = dot(wo, S.cw);
+        if (ez >= 0) return false;
+        rx = -dot(wo, S.cu) / ez / S.tf / S.aspect * .5 + .5;
+        ry = -dot(wo, S.cv) / ez / S.tf * .5 + .5;
+        return !(rx < 0 || rx > 1 || ry < 0 || ry > 1);
+    }
+    double J(V3 wo) const {
+        const double c = -dot(wo, S.cw);
+        return 1.0 / (c * c * c) / (S.tf * S.tf * S.aspect * 4);
+    }
+    // path::eval_contrb_direction (path.h:770-772) at a vertex
+    V3 eval(const BVert& v, const V3* wi, V3 wo, bool le, bool eval_delta) const {
+        if (v.kind == BVert::Camera) {
+            double rx, ry;
+            return raster(wo, rx, ry) ? V3(J(wo)) : V3(0);
+        }
+        if (v.kind == BVert::Light) return dot(wo, v.g.n) <= 0 ? V3(0) : S.lights[v.light].Ke;
+        const V3 f = eval_delta ? mat_eval_delta(v.mat, v.g, *wi, wo, v.comp)
+                                : mat_eval_full(v.mat, v.g, *wi, wo, v.comp, le);
+        const double sn = le ? shading_normal_correction_le(v.g, *wi, wo) : shading_normal_correction(v.g, *wi, wo);
+        return f * sn;
+    }
+    // path::pdf_direction (projected solid angle, eval_delta = false)
+    double pdf_dir(const BVert& v, const V3* wi, V3 wo) const {
+        if (v.kind == BVert::Camera) {
+            double rx, ry;
+            return raster(wo, rx, ry) ? J(wo) : 0.0;
+        }
+        if (v.kind == BVert::Light) return dot(wo, v.g.n) <= 0 ? 0.0 : 1.0 / Pi;
+        return mat_pdf_full(v.mat, v.g, *wi, wo, v.comp);
+    }
+    // path::pdf_component
+    double pdf_comp(const BVert& v, V3 wi) const {
+        if (v.kind != BVert::Surface) return 1;
+        const Mat& m = v.mat;
+        if (m.type == MAT_MIXTURE) return v.comp == 0 ? m.alpha : 1 - m.alpha;
+        if (m.type != MAT_GLASS) return 1;
+        V3 wt; bool total; double eta;
+        const double Fr = glass_Fr(m, v.g, wi, wt, total, eta);
+        return v.comp == 0 ? Fr : 1 - Fr;
+    }
+    bool specular(const BVert& v) const { return v.kind == BVert::Surface && is_specular(v.mat, v.comp); }
+    // path::pdf_position of an endpoint: camera 1, light p_sel / area
+    double pdf_pos(const BVert& v) const {
+        return v.kind == BVert::Camera ? 1.0 : S.lights[v.light].invA / S.num_lights;
+    }
+    static V3 dir(const BVert& a, const BVert& b) { return normalize(b.g.p - a.g.p); }
+
+    // Path::eval_subpath_sampling_weight (bidir.h:239-282) of the first l
+    // vertices of `p` seen from its endpoint `from0` (0 = light side).
+    V3 alpha(const std::vector<BVert>& p, int l, bool light_side) const {
+        if (l == 0) return V3(1);
+        const int n = int(p.size());
+        auto at = [&](int i) -> const BVert* {
+            if (i < 0 || i >= n) return nullptr;
+            return &p[light_side ? i : n - 1 - i];
+        };
+        V3 a = V3(1.0 / (pdf_pos(*at(0)) * 1.0));
+        for (int i = 0; i < l - 1; i++) {
+            const BVert* v = at(i);
+            const BVert* vp = at(i - 1);
+            const BVert* vn = at(i + 1);
+            const V3 wo = dir(*v, *vn);
+            V3 wi(0);
+            if (vp) wi = dir(*v, *vp);
+            const V3 f = eval(*v, vp ? &wi : nullptr, wo, light_side, false);
+            if (is_zero(f)) return V3(0);
+            const double pd = pdf_dir(*v, vp ? &wi : nullptr, wo);
+            a = a * f / pd / pdf_comp(*vn, -wo);
+        }
+        return a;
+    }
+    // Path::eval_connection_term (bidir.h:293-318)
+    V3 connection(const std::vector<BVert>& p, int s) const {
+        const int n = int(p.size()), t = n - s;
+        if (s == 0) return eval(p[0], nullptr, dir(p[0], p[1]), true, true);
+        const BVert& vL = p[s - 1];
+        const BVert& vE = p[s];
+        V3 wiL(0), wiE(0);
+        if (s >= 2) wiL = dir(vL, p[s - 2]);
+        if (t >= 2) wiE = dir(vE, p[s + 1]);
+        const V3 fL = eval(vL, s >= 2 ? &wiL : nullptr, dir(vL, vE), true, true);
+        const V3 fE = eval(vE, t >= 2 ? &wiE : nullptr, dir(vE, vL), false, true);
+        return fL * geometry_term(vL.g, vE.g) * fE;
+    }
+    // Path::is_samplable_bidir (bidir.h:203-228) for a pinhole camera and area lights
+    bool samplable(const std::vector<BVert>& p, int s) const {
+        const int n = int(p.size()), t = n - s;
+        if (s == 0) return !specular(p[0]);
+        if (t == 0) return false;   // the pinhole is degenerate
+        return !specular(p[s - 1]) && !specular(p[s]);
+    }
+    // Path::pdf_bidir (bidir.h:392-437)
+    double pdf(const std::vector<BVert>& p, int s) const {
+        if (!samplable(p, s)) return 0;
+        const int n = int(p.size()), t = n - s;
+        auto sub = [&](int l, bool light_side) -> double {
+            if (l == 0) return 1;
+            auto at = [&](int i) -> const BVert* {
+                if (i < 0 || i >= n) return nullptr;
+                return &p[light_side ? i : n - 1 - i];
+            };
+            double q = pdf_pos(*at(0));
+            for (int i = 0; i < l - 1; i++) {
+                const BVert* v = at(i);
+                const BVert* vp = at(i - 1);
+                const BVert* vn = at(i + 1);
+                const V3 wo = dir(*v, *vn);
+                V3 wi(0);
+                if (vp) wi = dir(*v, *vp);
+                const double pc = pdf_comp(*vn, -wo);
+                q *= pc * pdf_dir(*v, vp ? &wi : nullptr, wo) * geometry_term(v->g, vn->g);
+            }
+            return q;
+        };
+        return sub(s, true) * sub(t, false);
+    }
+    // Path::eval_mis_weight (bidir.h:448-466): power heuristic
+    double mis(const std::vector<BVert>& p, int s) const {
+        const double ps = pdf(p, s);
+        double inv = 0;
+        for (int s2 = 0; s2 <= int(p.size()); s2++) {
+            const double pi = pdf(p, s2);
+            if (pi == 0) continue;
+            const double r = pi / ps;
+            inv += r * r;
+        }
+        return 1 / inv;
+    }
+};
+
+// Restates Renderer_BDPT::render (src/renderer/renderer_bdpt.cpp:339-411) with
+// scheduler::spi::sample (in passes of min(w*h, 2^18) samples) for area
+// lights and the pinhole camera: per sample
+// an eye subpath and a light subpath of at most max_verts vertices each
+// (path::sample_subpath, bidir.h:489-548: no Russian roulette), every
+// (s, t) with 2 <= s + t <= max_verts connected (path::connect_subpaths,
+// bidir.h:566-626), weighted by the power heuristic over all strategies of
+// the full path, and splatted at its raster position. Sample i of pass k
+// draws Philox counters (i mod w*h, k, vertex, block): the light subpath
+// uses blocks 0 (vertex 0: light selection, triangle, position; vertex
+// v >= 1: -, component, direction u0, u1) and 1 (vertex 0: direction), the
+// eye subpath block 2 (vertex 0: raster position; vertex v >= 1 as the
+// light's block 0).
+int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, double* film, OrcPTStats* stats) {
+    Scene S;
+    init_scene(S, scene, static_cast<Accel*>(accel));
+    if (S.lights.empty() || S.has_env) return -1;
+    const int w = scene->w, h = scene->h;
+    const int64_t npix = int64_t(w) * h;
+    const BDPT B{S, w, h};
+    const int M = P->max_verts;
+    const int min_verts = P->min_verts > 0 ? P->min_verts : 2;
+    // samples per pass, as the GPU renderer: min(w*h, 2^18)
+    const int64_t per_pass = std::min<int64_t>(npix, int64_t(1) << 18);
+    if (P->nthreads > 0) omp_set_num_threads(P->nthreads);
+    int64_t n_ext = 0, n_shadow = 0, n_vert = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : n_ext, n_shadow, n_vert)
+    for (int64_t i = 0; i < P->num_samples; i++) {
+        const uint32_t idx = uint32_t(i % per_pass);
+        const uint32_t pass = uint32_t(P->pass_offset + i / per_pass);
+        // subpath random walk (sample_subpath_from_endpoint, bidir.h:489-535)
+        auto walk = [&](std::vector<BVert>& p, V3 o, V3 d, uint32_t blk) {
+            while (int(p.size()) < M) {
+                if (p.size() > 1) {
+                    // path::sample_direction at the last vertex: (vertex, block)[2..3]
+                    const BVert& v = p.back();
+                    const V3 wi = normalize(p[p.size() - 2].g.p - v.g.p);
+                    const Rand4 r = rand4(idx, pass, uint32_t(p.size() - 1), blk, P->seed);
+                    V3 sw;
+                    if (!mat_sample_direction(v.mat, v.g, wi, v.comp, r.u[2], r.u[3], v.ul, d, sw)) break;
+                    o = v.g.p;
+                }
+                n_ext++;
+                const Hit hit = scene_intersect(S, o, d, Eps, Inf);
+                if (!hit.valid || hit.env) break;
+                n_vert++;
+                const Rand4 rc = rand4(idx, pass, uint32_t(p.size()), blk, P->seed);
+                BVert v;
+                v.kind = BVert::Surface;
+                v.g = hit.g;
+                v.mat = hit.mat;
+                v.light = S.prim_light[hit.prim];
+                double cw;
+                mat_sample_component(v.mat, v.g, -d, rc.u[1], v.comp, cw, v.ul);
+                p.push_back(v);
+            }
+        };
+        // light subpath: light::area::sample_ray (light_area.cpp:78-100)
+        std::vector<BVert> L, E;
+        {
+            const Rand4 r0 = rand4(idx, pass, 0, 0, P->seed);
+            const Rand4 r1 = rand4(idx, pass, 0, 1, P->seed);
+            const int li = std::clamp(int(r0.u[0] * S.num_lights), 0, S.num_lights - 1);
+            const auto& AL = S.lights[li];
+            const int ti = AL.tris[dist_sample(AL.cdf, r0.u[1])];
+            const float* vv = scene->verts + 9 * int64_t(ti);
+            const V3 a = load3(vv), b = load3(vv + 3), c = load3(vv + 6);
+            const double s = safe_sqrt(r0.u[2]);
+            const double bu = 1 - s, bv = r0.u[3] * s;
+            const V3 gn = normalize(cross(b - a, c - a));
+            BVert v0;
+            v0.kind = BVert::Light;
+            v0.g = make_on_surface(a * (1 - bu - bv) + b * bu + c * bv, gn, gn);
+            v0.light = li;
+            V3 u, vb;
+            onb(gn, u, vb);
+            const V3 l = sample_cosine_weighted(r1.u[0], r1.u[1]);
+            L.push_back(v0);
+            if (M > 1) walk(L, v0.g.p, u * l.x + vb * l.y + gn * l.z, 0);
+        }
+        // eye subpath: camera::pinhole::sample_ray (camera_pinhole.cpp:134-140)
+        {
+            const Rand4 r = rand4(idx, pass, 0, 2, P->seed);
+            BVert c;
+            c.kind = BVert::Camera;
+            c.g = make_degenerated(S.pos);
+            E.push_back(c);
+            walk(E, S.pos, primary_dir(S, r.u[0], r.u[1]), 2);
+        }
+        const int nL = int(L.size()), nE = int(E.size());
+        for (int s = 0; s <= nL; s++) {
+            for (int t = 1; t <= nE; t++) {
+                const int k = s + t;
+                if (k < std::max(2, min_verts) || k > M) continue;
+                std::vector<BVert> p;
+                if (s == 0) {
+                    if (E[t - 1].kind != BVert::Surface || E[t - 1].light < 0) continue;
+                    for (int j = t - 1; j >= 0; j--) p.push_back(E[j]);
+                    p[0].kind = BVert::Light;
+                } else {
+                    n_shadow++;
+                    if (!visible(S, L[s - 1].g, E[t - 1].g)) continue;
+                    for (int j = 0; j < s; j++) p.push_back(L[j]);
+                    for (int j = t - 1; j >= 0; j--) p.push_back(E[j]);
+                }
+                const V3 C = B.alpha(p, s, true) * B.connection(p, s) * B.alpha(p, t, false);
+                if (is_zero(C)) continue;
+                const double wgt = B.mis(p, s);
+                double rx, ry;
+                if (!B.raster(BDPT::dir(p[k - 1], p[k - 2]), rx, ry)) continue;
+                const int px = std::clamp(int(rx * w), 0, w - 1);
+                const int py = std::clamp(int(ry * h), 0, h - 1);
+                const V3 Cw = C * wgt;
+                double* f = film + 3 * (int64_t(py) * w + px);
+#pragma omp atomic
+                f[0] += Cw.x;
+#pragma omp atomic
+                f[1] += Cw.y;
+#pragma omp atomic
+                f[2] += Cw.z;
             }
         }
     }

@@ -73,6 +73,7 @@ class OrcLTParams(ctypes.Structure):
         ('pass_offset', ctypes.c_int32),
         ('seed', ctypes.c_uint64),
         ('nthreads', ctypes.c_int32),
+        ('min_verts', ctypes.c_int32),
     ]
 
 
@@ -121,6 +122,9 @@ def lib():
         L.orc_pt_render.restype = ctypes.c_int
         L.orc_pt_render.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_void_p, ctypes.POINTER(OrcPTParams),
                                     _f64p, ctypes.c_void_p, ctypes.POINTER(OrcPTStats)]
+        L.orc_bdpt_render.restype = ctypes.c_int
+        L.orc_bdpt_render.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_void_p, ctypes.POINTER(OrcLTParams),
+                                      _f64p, ctypes.POINTER(OrcPTStats)]
         L.orc_lt_render.restype = ctypes.c_int
         L.orc_lt_render.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_void_p, ctypes.POINTER(OrcLTParams),
                                     _f64p, ctypes.POINTER(OrcPTStats)]
@@ -299,6 +303,26 @@ def lt_render(flat, num_samples: int, max_verts: int, seed: int = 0, pass_offset
     rc = lib().orc_lt_render(ctypes.byref(s), accel.h, ctypes.byref(P), film, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f'orc_lt_render failed ({rc})')
+    f = film.reshape(flat.h, flat.w, 3)
+    return {'sum': f, 'image': f * (flat.w * flat.h / num_samples),
+            'stats': {'samples': st.samples, 'extension_rays': st.extension_rays,
+                      'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices}}
+
+
+def bdpt_render(flat, num_samples: int, max_verts: int, seed: int = 0, pass_offset: int = 0, nthreads: int = 0,
+                accel: Accel = None, min_verts: int = 2):
+    """renderer::bdpt restated in FP64 (orc_bdpt_render; area lights only).
+    Returns dict with 'sum' ([h,w,3] unscaled splat sums), 'image' (= sum *
+    w*h / num_samples, the reference's film) and 'stats'."""
+    if accel is None:
+        accel = Accel(flat.verts, nthreads)
+    s, keep = make_orc_scene(flat)
+    P = OrcLTParams(int(num_samples), int(max_verts), int(pass_offset), int(seed), int(nthreads), int(min_verts))
+    film = np.zeros(flat.w * flat.h * 3, dtype=np.float64)
+    st = OrcPTStats()
+    rc = lib().orc_bdpt_render(ctypes.byref(s), accel.h, ctypes.byref(P), film, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError(f'orc_bdpt_render failed ({rc})')
     f = film.reshape(flat.h, flat.w, 3)
     return {'sum': f, 'image': f * (flat.w * flat.h / num_samples),
             'stats': {'samples': st.samples, 'extension_rays': st.extension_rays,
