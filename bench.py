@@ -59,10 +59,10 @@ def traffic_from_profiles():
 
 def cpu_baseline(flat, rays, n_rays):
     """The oracle's FP64 restatement of accel::sahbvh, renderer::pt and
-    renderer::lt on all host cores: incoherent (the headline unit) and
-    coherent closest-hit Mrays/s on the config-1 rays, the SAH build time,
-    renderer::pt samples/s and renderer::lt light paths/s on config 0
-    (256x256, 16 samples per pixel, 8 vertices)."""
+    renderer::lt and renderer::bdpt on all host cores: incoherent (the
+    headline unit) and coherent closest-hit Mrays/s on the config-1 rays, the
+    SAH build time, renderer::pt samples/s, renderer::lt light paths/s and
+    renderer::bdpt samples/s on config 0 (256x256, 8 vertices)."""
     from oracle import oracle as orc
     import numpy as np
     from lm_amd import scenes
@@ -84,6 +84,9 @@ def cpu_baseline(flat, rays, n_rays):
     t0 = time.time()
     r = orc.lt_render(f0, 16 * f0.w * f0.h, 8, seed=1, nthreads=cores)
     lt_s = r['stats']['samples'] / (time.time() - t0)
+    t0 = time.time()
+    r = orc.bdpt_render(f0, 4 * f0.w * f0.h, 8, seed=1, nthreads=cores)
+    bdpt_s = r['stats']['samples'] / (time.time() - t0)
     return {'value': out['incoherent'], 'unit': 'Mrays/s', 'cores': cores, 'kind': 'port',
             'sample': f'FP64 accel::sahbvh restatement (oracle), OpenMP over {n_rays} of the 2,000,000 '
                       f'incoherent config-1 rays; SAH build of 1,320,720 tris {build_s:.1f} s excluded',
@@ -91,6 +94,9 @@ def cpu_baseline(flat, rays, n_rays):
             'pt_c0_samples_s': round(pt_s, 1),
             'pt_c0_sample': 'renderer::pt restatement (oracle), config 0 256x256 16 spp, max 8 vertices, MIS',
             'lt_c0_light_paths_s': round(lt_s, 1),
+            'bdpt_c0_samples_s': round(bdpt_s, 1),
+            'bdpt_c0_sample': 'renderer::bdpt restatement (oracle, explicit MIS), config 0, 4 x 256x256 samples, '
+                              'max 8 vertices',
             'lt_c0_sample': 'renderer::lt restatement (oracle), config 0, 16 x 256x256 light paths, max 8 vertices'}
 
 
@@ -198,6 +204,23 @@ def main():
                              'mrays_s': round((st['extension_rays'] + st['shadow_rays']) / dt / 1e6, 1),
                              'ms_per_pass': round(dt / st['passes'] * 1e3, 2)}
         log('lt', extra['lt_1080p'])
+        # renderer::bdpt on config 0 (area light only) at 1920x1080, max 8 vertices
+        f0 = scenes.config0().flatten()
+        f0.w, f0.h = 1920, 1080
+        f0.camera.aspect = 1920 / 1080
+        sb = GpuScene(f0, width=args.width, leaf_size=args.leaf)
+        fb = torch.zeros((f0.h, f0.w, 3), dtype=torch.float64, device=dev)
+        sb.render_bdpt(num_samples=1 << 18, max_verts=8, seed=3, film=fb)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        _, st = sb.render_bdpt(num_samples=8 << 18, max_verts=8, seed=3, pass_offset=1, film=fb)
+        torch.cuda.synchronize()
+        dt = time.time() - t0
+        extra['bdpt_c0_1080p'] = {'samples_s': round(st['samples'] / dt, 1), 'samples': st['samples'],
+                                  'mrays_s': round((st['extension_rays'] + st['shadow_rays']) / dt / 1e6, 1),
+                                  'max_verts': 8}
+        log('bdpt', extra['bdpt_c0_1080p'])
+        del sb, fb
 
     if not args.no_c3:
         from lm_amd.batch import Batch

@@ -72,22 +72,8 @@ __global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState p
     f3 org, d, w;
     if (li < sc.num_area_lights) {
         const DevLight dl = sc.lights[li];
-        // Dist::sample: upper_bound on the light's area CDF
-        const float* cdf = sc.light_cdf + dl.cdf_off;
-        int lo = 0, hi = dl.count + 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (cdf[mid] <= r0.u[1]) lo = mid + 1;
-            else hi = mid;
-        }
-        const int k = min(max(lo - 1, 0), dl.count - 1);
-        const ShadeTri ls = sc.stri[sc.light_tri[dl.cdf_off - li + k]];
-        const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z),
-                 C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
-        const float s = sqrtf(fmaxf(0.f, r0.u[2]));
-        const float bu = 1.f - s, bv = r0.u[3] * s;
-        org = A * (1.f - bu - bv) + B * bu + C * bv;
-        const f3 nL = normalize(cross(B - A, C - A));
+        f3 nL;
+        sample_light_point(sc, li, r0.u[1], r0.u[2], r0.u[3], org, nL);
         f3 u, v;
         onb(nL, u, v);
         const f3 dl_ = sample_cosine_weighted(r1.u[0], r1.u[1]);
@@ -378,20 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_bd_lgen(SceneDev sc, PathState ps, Q
     const rand4 r1 = rng(uint32_t(i), c.pass, 0u, 1u, seed);
     const int li = min(max(int(r0.u[0] * sc.num_area_lights), 0), sc.num_area_lights - 1);
     const DevLight dl = sc.lights[li];
-    const float* cdf = sc.light_cdf + dl.cdf_off;
-    int lo = 0, hi = dl.count + 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (cdf[mid] <= r0.u[1]) lo = mid + 1;
-        else hi = mid;
-    }
-    const int k = min(max(lo - 1, 0), dl.count - 1);
-    const ShadeTri ls = sc.stri[sc.light_tri[dl.cdf_off - li + k]];
-    const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z), C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
-    const float s = sqrtf(fmaxf(0.f, r0.u[2]));
-    const float bu = 1.f - s, bv = r0.u[3] * s;
-    const f3 org = A * (1.f - bu - bv) + B * bu + C * bv;
-    const f3 nL = normalize(cross(B - A, C - A));
+    f3 org, nL;
+    sample_light_point(sc, li, r0.u[1], r0.u[2], r0.u[3], org, nL);
     f3 u, v;
     onb(nL, u, v);
     const f3 dl_ = sample_cosine_weighted(r1.u[0], r1.u[1]);

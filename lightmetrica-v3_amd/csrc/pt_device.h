@@ -161,6 +161,28 @@ __device__ __forceinline__ const WNode<4>* nodes_of<4>(const SceneDev& sc) { ret
 template <>
 __device__ __forceinline__ const WNode<8>* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
 
+// A point on area light `li` and its geometric normal (light::area:
+// triangle by the area CDF with Dist::sample's upper_bound, math.h:367-370;
+// barycentrics (1 - sqrt(u0), u1 sqrt(u0)), light_area.cpp:47-59).
+__device__ __forceinline__ void sample_light_point(const SceneDev& sc, int li, float ut, float u0, float u1, f3& p,
+                                                   f3& n) {
+    const DevLight dl = sc.lights[li];
+    const float* cdf = sc.light_cdf + dl.cdf_off;
+    int lo = 0, hi = dl.count + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= ut) lo = mid + 1;
+        else hi = mid;
+    }
+    const int k = min(max(lo - 1, 0), dl.count - 1);
+    const ShadeTri ls = sc.stri[sc.light_tri[dl.cdf_off - li + k]];
+    const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z), C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
+    const float s = sqrtf(fmaxf(0.f, u0));
+    const float bu = 1.f - s, bv = u1 * s;
+    p = A * (1.f - bu - bv) + B * bu + C * bv;
+    n = normalize(cross(B - A, C - A));
+}
+
 // pdf of light endpoint `light` seen from the previous vertex (prev normal
 // np, degenerate flag) along d (unit, from the previous vertex to the light),
 // including the uniform selection probability (path::pdf_direct, path.h:563-583).
@@ -315,24 +337,8 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
                 bool ok = false;
                 if (li < sc.num_area_lights) {
                     const DevLight dl = sc.lights[li];
-                    // Dist::sample: upper_bound on the CDF
-                    const float* cdf = sc.light_cdf + dl.cdf_off;
-                    int lo = 0, hi = dl.count + 1;
-                    const float u = r1.u[1];
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (cdf[mid] <= u) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    const int k = min(max(lo - 1, 0), dl.count - 1);
-                    const int lt = sc.light_tri[dl.cdf_off - li + k];
-                    const ShadeTri ls = sc.stri[lt];
-                    const f3 A = mk(ls.p0.x, ls.p0.y, ls.p0.z), B = mk(ls.p1.x, ls.p1.y, ls.p1.z),
-                             C = mk(ls.p2.x, ls.p2.y, ls.p2.z);
-                    const float s = sqrtf(fmaxf(0.f, r1.u[2]));
-                    const float bu = 1.f - s, bv = r1.u[3] * s;
-                    const f3 pL = A * (1.f - bu - bv) + B * bu + C * bv;
-                    const f3 nL = normalize(cross(B - A, C - A));
+                    f3 pL, nL;
+                    sample_light_point(sc, li, r1.u[1], r1.u[2], r1.u[3], pL, nL);
                     const f3 wL = normalize(pos - pL);   // from the light to the point
                     if (dot(wL, nL) > 0.f) {
                         const f3 dd = pL - pos;

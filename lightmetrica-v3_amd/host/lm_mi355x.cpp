@@ -592,6 +592,63 @@ RenderResult Renderer_LT_MI355X::render() {
     return r;
 }
 
+// ---- renderer::bdpt --------------------------------------------------------------------
+
+Renderer_BDPT_MI355X::Renderer_BDPT_MI355X(const Scene* scene, Film* film, const Props& p)
+    : scene_(scene), film_(film) {
+    if (!scene_ || !film_) throw Exception(Error::InvalidArgument, "scene and output film are required");
+    max_verts_ = prop_req<int>(p, "max_verts");
+    min_verts_ = prop_or<int>(p, "min_verts", 2);
+    seed_ = uint64_t(prop_or<int>(p, "seed", 0));
+    const std::string sched = prop_req<std::string>(p, "scheduler");
+    num_samples_ = 0;
+    render_time_ = 0;
+    if (sched == "sample") {
+        num_samples_ = prop_req<int>(p, "num_samples");
+        if (num_samples_ <= 0) throw Exception(Error::InvalidArgument, "num_samples must be positive");
+    } else if (sched == "time") {
+        render_time_ = prop_req<double>(p, "render_time");
+        if (!(render_time_ > 0)) throw Exception(Error::InvalidArgument, "render_time must be positive");
+    } else {
+        throw Exception(Error::InvalidArgument, "invalid scheduler '" + sched + "'");
+    }
+    acc_width_ = prop_or<int>(p, "accel_width", 4);
+    acc_leaf_ = prop_or<int>(p, "accel_leaf_size", 2);
+}
+
+Renderer_BDPT_MI355X::~Renderer_BDPT_MI355X() {
+    if (gs_) lmgpu_scene_destroy(gs_);
+}
+
+RenderResult Renderer_BDPT_MI355X::render() {
+    const auto size = film_->size();
+    if (!gs_ || built_version_ != scene_->version()) {
+        if (gs_) lmgpu_scene_destroy(gs_);
+        gs_ = nullptr;
+        gs_ = make_device_scene(*scene_, acc_width_, acc_leaf_, size.w, size.h);
+        built_version_ = scene_->version();
+    }
+    lmgpu_lt_params lp{};
+    lp.num_samples = num_samples_;
+    lp.max_verts = max_verts_;
+    lp.min_verts = min_verts_;
+    lp.seed = seed_;
+    lp.render_time = render_time_;
+    film_->clear();
+    auto& data = film_->data();
+    lmgpu_pt_stats st{};
+    check(lmgpu_bdpt_render(gs_, &lp, data.data(), 0, &st, nullptr), "lmgpu_bdpt_render");
+    // Film::rescale(w*h / processed) (src/renderer/renderer_bdpt.cpp:398-399)
+    const double s = st.samples > 0 ? double(size.w) * double(size.h) / double(st.samples) : 0.0;
+    for (auto& v : data) v *= s;
+    RenderResult r;
+    r.processed = st.samples;
+    r.elapsed = st.elapsed_s;
+    r.extension_rays = st.extension_rays;
+    r.shadow_rays = st.shadow_rays;
+    return r;
+}
+
 // ---- renderer::raycast ------------------------------------------------------------------
 
 Renderer_Raycast_MI355X::Renderer_Raycast_MI355X(const Scene* scene, Film* film, const Props& p)

@@ -352,6 +352,29 @@ private:
     uint64_t built_version_ = ~0ull;
 };
 
+// renderer::bdpt_mi355x (src/renderer/renderer_bdpt.cpp:339-411, bidirectional
+// path tracing with power-heuristic MIS). Props as Renderer_Path_Base
+// (:43-55): max_verts (int, required), min_verts (2), scheduler ("sample":
+// num_samples, or "time": render_time), seed; accel_width, accel_leaf_size.
+// Area lights only (an environment light is Error::Unsupported).
+class Renderer_BDPT_MI355X final : public Renderer {
+public:
+    Renderer_BDPT_MI355X(const Scene* scene, Film* film, const Props& props);
+    ~Renderer_BDPT_MI355X() override;
+    RenderResult render() override;
+
+private:
+    const Scene* scene_;
+    Film* film_;
+    int max_verts_, min_verts_;
+    long long num_samples_;
+    double render_time_;
+    uint64_t seed_;
+    int acc_width_, acc_leaf_;
+    lmgpu_scene* gs_ = nullptr;
+    uint64_t built_version_ = ~0ull;
+};
+
 // renderer::raycast_mi355x (src/renderer/renderer_raycast.cpp:48-97). Props:
 //   bg_color (Vec3), use_constant_color (bool), visualize_normal (bool), color (Vec3).
 class Renderer_Raycast_MI355X final : public Renderer {

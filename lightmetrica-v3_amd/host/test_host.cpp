@@ -366,6 +366,46 @@ int main() {
         CHECK(res.processed >= 1024 && res.processed % 1024 == 0 && res.elapsed >= 0.2);
     });
 
+    run("renderer::bdpt_mi355x furnace (closed form) and properties", [] {
+        // all path lengths 2..M: E = Le sum_{k=0}^{M-2} rho^k
+        const double rho = 0.5, Le = 1.0;
+        const int M = 8;
+        Scene s;
+        s.add_primitive(closed_box(), Material::diffuse({rho, rho, rho}), AreaLight{{Le, Le, Le}});
+        Camera c;
+        c.position = {0, 0, 0.5};
+        c.center = {0, 0, -1};
+        c.vfov = 60;
+        s.set_camera(c);
+        double expected = 0;
+        for (int k = 0; k < M - 1; k++) expected += Le * std::pow(rho, k);
+        Film film(32, 32);
+        const int K = 16;
+        std::vector<double> means;
+        for (int k = 0; k < K; k++) {
+            Renderer_BDPT_MI355X r(&s, &film, {{"max_verts", M}, {"scheduler", std::string("sample")},
+                                               {"num_samples", 4 * 1024}, {"seed", 200 + k}});
+            CHECK(r.render().processed == 4 * 1024);
+            double m = 0;
+            for (int y = 0; y < 32; y++)
+                for (int x = 0; x < 32; x++) m += film.pixel(x, y).x;
+            means.push_back(m / 1024);
+        }
+        double mu = 0, var = 0;
+        for (double m : means) mu += m / K;
+        for (double m : means) var += (m - mu) * (m - mu) / (K - 1);
+        const double z = (mu - expected) / std::sqrt(var / K);
+        std::printf("  BDPT furnace: mean %.6f expected %.6f z %.2f\n", mu, expected, z);
+        CHECK(std::abs(z) < 5);
+        bool ok = false;
+        try {
+            Renderer_BDPT_MI355X r(&s, &film, {{"max_verts", 8}, {"num_samples", 100}});
+        } catch (const Exception& ex) {
+            ok = ex.error() == Error::InvalidArgument;   // scheduler is required
+        }
+        CHECK(ok);
+    });
+
     run("renderer::raycast_mi355x = CPU ray cast of the same scene", [] {
         const int w = 96, h = 64;
         Scene s = room_scene(w, h);

@@ -1,8 +1,8 @@
 """Profile driver: a fixed number of closest-hit launches over the config-1
 incoherent rays, preceded by a plain device copy of known size that
 calibrates the HBM byte counters; optionally one 1080p config-2
-path-tracing pass (--pt) and 4 plain + 4 guided 1080p passes of config 4
-(--c4).
+path-tracing pass (--pt), 4 plain + 4 guided 1080p passes of config 4
+(--c4), a config-3 batch (--c3 N) and light-tracing / BDPT passes (--lt-bdpt).
 
 Run under rocprofv3, e.g.
   rocprofv3 --kernel-trace --stats -d OUT -o kt -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py --pt
@@ -29,6 +29,8 @@ def main():
     ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
     ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
     ap.add_argument('--c3', type=int, default=0, help='also render the first N config-3 jobs as one batch')
+    ap.add_argument('--lt-bdpt', action='store_true',
+                    help='also run 2 LT passes of config 2 and 2 BDPT passes of config 0 at 1080p')
     args = ap.parse_args()
     import torch
     from lm_amd import scenes
@@ -60,6 +62,18 @@ def main():
         sc.render_raw(spp=1, max_verts=16, seed=1)
         torch.cuda.synchronize()
         print('1 x 1080p config-2 PT pass', flush=True)
+    if args.lt_bdpt:
+        from lm_amd.renderer import GpuScene
+        s2 = scenes.config2().flatten()
+        sc = GpuScene(s2, width=args.width, leaf_size=args.leaf)
+        _, st = sc.render_lt(num_samples=2 * s2.w * s2.h, max_verts=16, seed=1)
+        print('2 x 1080p config-2 LT passes', st, flush=True)
+        f0 = scenes.config0().flatten()
+        f0.w, f0.h = 1920, 1080
+        f0.camera.aspect = 1920 / 1080
+        sb = GpuScene(f0, width=args.width, leaf_size=args.leaf)
+        _, st = sb.render_bdpt(num_samples=2 << 18, max_verts=8, seed=1)
+        print('2 x 2^18-sample config-0 BDPT passes at 1080p', st, flush=True)
     if args.c3:
         from lm_amd.batch import Batch
         jobs = scenes.config3_jobs()[:args.c3]
