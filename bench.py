@@ -248,16 +248,17 @@ def main():
     if not args.no_c4:
         from lm_amd import equal_time
         log('config-4 equal-time comparison (960x540, 10 s each)')
-        et = equal_time.run(width=960, height=540, seconds=10.0, ref_spp=2048,
-                            variants=[{'steps_per_pass': 4, 'learning_rate': 0.005}])
-        v = et['variants'][0]
+        et = equal_time.run(width=960, height=540, seconds=10.0, ref_spp=2048)
+
+        def row(d):
+            return {'spp': d['spp'], 'rr_depth': d['rr_depth'], 'rmse': d['rmse'],
+                    'rmse_corrected': d['rmse_corrected']}
         extra['c4_equal_time'] = {
             'resolution': '960x540', 'seconds_each': 10.0, 'reference_spp': 2048,
-            'plain': {'spp': et['plain']['spp'], 'rmse': et['plain']['rmse']},
-            'guided_defaults': {'spp': et['guided']['spp'], 'rmse': et['guided']['rmse'],
-                                'ratio': round(et['rmse_ratio_plain_over_guided'], 3)},
-            'guided_4_steps_lr_5e-3': {'spp': v['spp'], 'rmse': v['rmse'],
-                                       'ratio': round(v['rmse_ratio_plain_over_guided'], 3)},
+            'plain': row(et['plain']), 'plain_control': row(et['control']),
+            'guided_defaults': dict(row(et['guided']), adam_steps=et['guided']['adam_steps']),
+            'ratio_plain_over_guided': round(et['rmse_ratio_plain_over_guided'], 3),
+            'ratio_control_over_guided': round(et['rmse_ratio_control_over_guided'], 3),
             'full_size_result': 'profiles/guided_c4_equal_time.md'}
         log('c4', extra['c4_equal_time'])
 

@@ -897,11 +897,11 @@ int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lm
     if (!scene || !guide) return fail(LMGPU_ERR_INVALID_ARGUMENT, "scene and guide are required");
     lmgpu::clear_error();
     lmgpu_guide_opts o{};
-    o.alpha = 0.5f;
-    o.learning_rate = 1e-3f;
+    o.alpha = 0.65f;
+    o.learning_rate = 5e-3f;
     o.batch = 1 << 18;
-    o.steps_per_pass = 1;
-    o.record_depth = 8;
+    o.steps_per_pass = 4;
+    o.record_depth = 12;
     if (opts) o = *opts;
     if (!(o.alpha >= 0.f && o.alpha <= 1.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "alpha must be in [0, 1]");
     if (!(o.learning_rate >= 0.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "learning rate must be >= 0");

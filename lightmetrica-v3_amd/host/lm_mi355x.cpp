@@ -446,7 +446,11 @@ Renderer_PT_MI355X::Renderer_PT_MI355X(const Scene* scene, Film* film, const Pro
     else if (mode == "nee") mode_ = LMGPU_SAMPLING_NEE;
     else if (mode == "mis") mode_ = LMGPU_SAMPLING_MIS;
     else throw Exception(Error::InvalidArgument, "invalid sampling_mode '" + mode + "'");
-    rr_depth_ = prop_or<int>(p, "rr_depth", 5);
+    guiding_ = prop_or<bool>(p, "guiding", false);
+    // Roulette on throughput alone cuts the low-throughput paths that a
+    // guide steers toward the light, so guided rendering defaults to none
+    // (paths end at max_verts); plain rendering keeps renderer_pt.cpp:281.
+    rr_depth_ = prop_or<int>(p, "rr_depth", guiding_ ? std::max(max_verts_, 5) : 5);
     seed_ = uint64_t(prop_or<int>(p, "seed", 0));
     const std::string sched = prop_or<std::string>(p, "scheduler", "sample");
     spp_ = 0;
@@ -460,12 +464,11 @@ Renderer_PT_MI355X::Renderer_PT_MI355X(const Scene* scene, Film* film, const Pro
     } else {
         throw Exception(Error::InvalidArgument, "invalid scheduler '" + sched + "'");
     }
-    guiding_ = prop_or<bool>(p, "guiding", false);
-    g_alpha_ = float(prop_or<double>(p, "guide_alpha", 0.5));
-    g_lr_ = float(prop_or<double>(p, "guide_lr", 1e-3));
+    g_alpha_ = float(prop_or<double>(p, "guide_alpha", 0.65));
+    g_lr_ = float(prop_or<double>(p, "guide_lr", 5e-3));
     g_batch_ = prop_or<int>(p, "guide_batch", 1 << 18);
-    g_steps_ = prop_or<int>(p, "guide_steps", 1);
-    g_depth_ = prop_or<int>(p, "guide_depth", 8);
+    g_steps_ = prop_or<int>(p, "guide_steps", 4);
+    g_depth_ = prop_or<int>(p, "guide_depth", 12);
     acc_width_ = prop_or<int>(p, "accel_width", 4);
     acc_leaf_ = prop_or<int>(p, "accel_leaf_size", 2);
 }

@@ -301,8 +301,9 @@ public:
 // renderer::pt_mi355x. Props (src/renderer/renderer_pt.cpp:51-77):
 //   max_verts (int, required), sampling_mode ("mis" | "nee" | "naive"),
 //   scheduler ("sample" | "time"), spp (int) or render_time (seconds),
-//   seed (int), rr_depth (int, 5), guiding (bool) with guide_alpha,
-//   guide_lr, guide_batch, guide_steps, guide_depth;
+//   seed (int), rr_depth (int; 5, or max_verts when guiding), guiding
+//   (bool) with guide_alpha (0.65), guide_lr (5e-3), guide_batch (2^18),
+//   guide_steps (4), guide_depth (12);
 //   accel_width, accel_leaf_size (BVH of the renderer's device scene).
 class Renderer_PT_MI355X final : public Renderer {
 public:

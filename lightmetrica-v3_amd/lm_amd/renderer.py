@@ -178,7 +178,7 @@ class GpuScene:
         p.render_time = float(render_time)
         st = _lib.PTStats()
         if guide is not None:
-            o = _lib.GuideOpts(0.5, 1e-3, 1 << 18, 1, 8)
+            o = _lib.GuideOpts(*GUIDE_DEFAULTS)
             for k, v in (guide_opts or {}).items():
                 if k not in ('alpha', 'learning_rate', 'batch', 'steps_per_pass', 'record_depth', 'weight_clamp'):
                     raise ValueError(f'unknown guide option {k!r}')
@@ -280,6 +280,10 @@ class RendererRaycast:
                   img.ctypes.data_as(ctypes.c_void_p), 0, ctypes.c_void_p(_stream_ptr(stream)))
         self.film.data[:] = img
         return {'elapsed': time.perf_counter() - t0}
+
+
+# lmgpu_guide_opts defaults: alpha, learning_rate, batch, steps_per_pass, record_depth
+GUIDE_DEFAULTS = (0.65, 5e-3, 1 << 18, 4, 12)
 
 
 class RendererPT:

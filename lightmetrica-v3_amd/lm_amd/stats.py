@@ -67,3 +67,19 @@ def binomial_bound(alpha: float, n: int, sigmas: float = 4.0) -> float:
 def rel_mse(img: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
     """rMSE = mean((I - R)^2 / (R^2 + eps)) over pixels and channels."""
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def rel_mse_corrected(img: np.ndarray, ref_sum: np.ndarray, ref_sumsq: np.ndarray, n_ref: int,
+                      eps: float = 1e-2) -> float:
+    """rMSE of `img` against the true image, estimated from a finite
+    reference R (per-pixel sum and sum of squares of n_ref independent
+    samples): for I independent of R with E[R] = mu,
+    E[(I - R)^2] = E[(I - mu)^2] + Var(R), and s^2 / n_ref (the unbiased
+    sample variance over n_ref) estimates Var(R) without bias, so
+    mean(((I - R)^2 - s^2 / n_ref) / (R^2 + eps)) removes the reference's own
+    noise from the error. The weight 1 / (R^2 + eps) uses the noisy R, which
+    leaves a relative bias of about 3 Var(R) / R^2 in pixels with R^2 >> eps
+    (none where eps dominates): small for a reference whose own relative
+    noise is small."""
+    ref, var = mean_var(ref_sum, ref_sumsq, n_ref)
+    return float(np.mean(((img - ref) ** 2 - var / n_ref) / (ref ** 2 + eps)))

@@ -62,7 +62,7 @@ def test_guided_c0_welch_vs_oracle(device):
     flat.w = flat.h = 128
     spp, M = 64, 8
     sc = GpuScene(flat)
-    s, q, st, g = guided_render(sc, spp, M, seed=71, batch=8192)
+    s, q, st, g = guided_render(sc, spp, M, seed=71, batch=8192, steps_per_pass=1)
     ref = orc.pt_render(flat, spp=spp, max_verts=M, sampling_mode=2, seed=993)
     frac, p, a = stats.welch(s, q, spp, ref['sum'], ref['sumsq'], spp)
     bound = stats.binomial_bound(0.01, flat.w * flat.h)
