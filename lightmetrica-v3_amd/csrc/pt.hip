@@ -899,8 +899,8 @@ int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lm
     lmgpu_guide_opts o{};
     o.alpha = 0.65f;
     o.learning_rate = 5e-3f;
-    o.batch = 1 << 18;
-    o.steps_per_pass = 4;
+    o.batch = 1 << 17;
+    o.steps_per_pass = 8;
     o.record_depth = 12;
     if (opts) o = *opts;
     if (!(o.alpha >= 0.f && o.alpha <= 1.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "alpha must be in [0, 1]");

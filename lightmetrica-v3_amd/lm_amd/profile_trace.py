@@ -1,8 +1,8 @@
 """Profile driver: a fixed number of closest-hit launches over the config-1
 incoherent rays, preceded by a plain device copy of known size that
 calibrates the HBM byte counters; optionally one 1080p config-2
-path-tracing pass (--pt), 4 plain + 4 guided 1080p passes of config 4
-(--c4), a config-3 batch (--c3 N) and light-tracing / BDPT passes (--lt-bdpt).
+path-tracing pass (--pt), 4 + 4 plain (roulette from vertex 5 / none) and 4 guided (no
+roulette) 1080p passes of config 4 (--c4), a config-3 batch (--c3 N) and light-tracing / BDPT passes (--lt-bdpt).
 
 Run under rocprofv3, e.g.
   rocprofv3 --kernel-trace --stats -d OUT -o kt -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py --pt
@@ -90,12 +90,15 @@ def main():
         fs = torch.zeros((s4.h, s4.w, 3), dtype=torch.float64, device=dev)
         _, _, st = sc.render_raw(spp=4, max_verts=16, seed=1, film_sum=fs)
         torch.cuda.synchronize()
-        print('4 x 1080p config-4 plain PT passes', st, flush=True)
+        print('4 x 1080p config-4 plain PT passes (roulette from vertex 5)', st, flush=True)
+        _, _, st = sc.render_raw(spp=4, max_verts=16, rr_depth=16, seed=1, film_sum=fs)
+        torch.cuda.synchronize()
+        print('4 x 1080p config-4 plain PT passes (no roulette)', st, flush=True)
         lo, hi = sc.bounds()
         g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-        _, _, st = sc.render_raw(spp=4, max_verts=16, seed=2, film_sum=fs, guide=g)
+        _, _, st = sc.render_raw(spp=4, max_verts=16, rr_depth=16, seed=2, film_sum=fs, guide=g)
         torch.cuda.synchronize()
-        print('4 x 1080p config-4 guided PT passes', st, flush=True)
+        print('4 x 1080p config-4 guided PT passes (default guide options, no roulette)', st, flush=True)
 
 
 if __name__ == '__main__':

@@ -159,18 +159,24 @@ class GpuScene:
         """Scene bounding box (lo, hi) of the triangle vertices."""
         return self._lo.copy(), self._hi.copy()
 
-    def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: int = 5,
+    def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: Optional[int] = None,
                    seed: int = 0, sample_offset: int = 0, render_time: float = 0.0,
                    film_sum=None, film_sumsq=None, stream=None, guide=None, guide_opts: Optional[dict] = None):
         """lmgpu_pt_render (lmgpu_pt_render_guided with a `guide`). With
         torch CUDA tensors for film_sum / film_sumsq ([h,w,3] float64) the
         film stays on the device and is accumulated into; otherwise host
         arrays are returned (fresh sums). guide_opts keys: alpha,
-        learning_rate, batch, steps_per_pass, record_depth."""
+        learning_rate, batch, steps_per_pass, record_depth, weight_clamp.
+        rr_depth: Russian roulette once a path has more vertices than this;
+        default 5 (renderer_pt.cpp:281) without a guide and max_verts (no
+        roulette) with one, because roulette on throughput alone cuts the
+        low-throughput paths that the guide steers toward the light."""
         p = _lib.PTParams()
         p.spp = int(spp)
         p.max_verts = int(max_verts)
         p.sampling_mode = int(sampling_mode)
+        if rr_depth is None:
+            rr_depth = 5 if guide is None else max(int(max_verts), 5)
         p.rr_depth = int(rr_depth)
         p.seed = int(seed)
         p.sample_offset = int(sample_offset)
@@ -283,7 +289,7 @@ class RendererRaycast:
 
 
 # lmgpu_guide_opts defaults: alpha, learning_rate, batch, steps_per_pass, record_depth
-GUIDE_DEFAULTS = (0.65, 5e-3, 1 << 18, 4, 12)
+GUIDE_DEFAULTS = (0.65, 5e-3, 1 << 17, 8, 12)
 
 
 class RendererPT:
