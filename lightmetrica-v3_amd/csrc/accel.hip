@@ -17,6 +17,7 @@
 // bvh.h.
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -331,8 +332,8 @@ constexpr int kPrivDepth = 72;
 // Persistent closest-hit / any-hit kernel over a ray array (bvh.h,
 // trace_persistent). `next` is this launch's ray counter, zeroed by
 // k_zero_counter on the same stream.
-template <int W, bool ANY, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const WNode<W>* __restrict__ nodes,
+template <int W, bool ANY, bool COUNT, class Node>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const Node* __restrict__ nodes,
                                                         const TriV* __restrict__ tris,
                                                         const lmgpu_ray* __restrict__ rays,
                                                         lmgpu_hit* __restrict__ hits, uint8_t* __restrict__ occl,
@@ -359,6 +360,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const WNode<W>* __restric
     };
     trace_persistent<W, ANY, COUNT, kLdsDepth, kPrivDepth, kTraceBlock>(nodes, tris, n, next, lds, fetch, emit,
                                                                          status, counters, refill);
+}
+
+// Quantised copy of a built 4-wide node array (bvh.h, quantise_node4).
+__global__ __launch_bounds__(256) void k_quantise4(const WNode<4>* __restrict__ in, QNode4* __restrict__ out,
+                                                   int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) out[i] = quantise_node4(in[i]);
 }
 
 __global__ void k_zero_counter(unsigned long long* c) {
@@ -545,20 +553,38 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     build_ms = ms;
     root_lo[0] = rlo.x; root_lo[1] = rlo.y; root_lo[2] = rlo.z;
     root_hi[0] = rhi.x; root_hi[1] = rhi.y; root_hi[2] = rhi.z;
+    LMGPU_TRY(make_trav_nodes(s));
     LMGPU_HIP(hipGetLastError());
     built = true;
     return LMGPU_OK;
 }
 
+int quantise_nodes4(const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s) {
+    if (n <= 0) return LMGPU_OK;
+    hipLaunchKernelGGL(k_quantise4, dim3(nblocks(n, 256)), dim3(256), 0, s, reinterpret_cast<const WNode<4>*>(wnodes),
+                       reinterpret_cast<QNode4*>(qnodes), n);
+    LMGPU_HIP(hipGetLastError());
+    return LMGPU_OK;
+}
+
+int Accel::make_trav_nodes(hipStream_t s) {
+    qnodes.reset();
+    if (!std::is_same<TravNode<4>::type, QNode4>::value || width != 4) return LMGPU_OK;
+    LMGPU_TRY(qnodes.alloc(size_t(num_nodes) * (sizeof(QNode4) / sizeof(float4))));
+    LMGPU_TRY(quantise_nodes4(nodes.p, qnodes.p, num_nodes, s));
+    LMGPU_HIP(hipStreamSynchronize(s));
+    return LMGPU_OK;
+}
+
 // Resident grid of the persistent kernel: CUs x active blocks per CU.
-template <int W, bool ANY, bool COUNT>
+template <int W, bool ANY, bool COUNT, class Node>
 static int persistent_grid() {
     static int grid = 0;
     if (grid == 0) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<W, ANY, COUNT>, kTraceBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<W, ANY, COUNT, Node>, kTraceBlock, 0);
         grid = std::max(1, cus) * std::max(1, per);
     }
     return grid;
@@ -567,11 +593,12 @@ static int persistent_grid() {
 template <int W, bool ANY, bool COUNT>
 static void launch_trace(const Accel& a, const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n,
                          unsigned long long* next, int32_t* status, unsigned long long* counters, hipStream_t s) {
-    const int grid = int(std::min<int64_t>(persistent_grid<W, ANY, COUNT>(), (n + kTraceBlock - 1) / kTraceBlock));
+    using Node = typename TravNode<W>::type;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, next);
-    hipLaunchKernelGGL((k_trace<W, ANY, COUNT>), dim3(grid), dim3(kTraceBlock), 0, s,
-                       reinterpret_cast<const WNode<W>*>(a.nodes.p), a.tris.p, rays, hits, occl, n, next, status,
-                       counters, a.refill);
+    const int grid = int(std::min<int64_t>(persistent_grid<W, ANY, COUNT, Node>(), (n + kTraceBlock - 1) / kTraceBlock));
+    hipLaunchKernelGGL((k_trace<W, ANY, COUNT, Node>), dim3(grid), dim3(kTraceBlock), 0, s,
+                       static_cast<const Node*>(a.trav_nodes()), a.tris.p, rays, hits, occl, n, next, status, counters,
+                       a.refill);
 }
 
 int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,

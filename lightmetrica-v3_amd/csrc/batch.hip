@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cstring>
 #include <new>
@@ -173,7 +174,8 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_b_extend(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+__global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>::type* __restrict__ nodes,
+                                                     const TriV* __restrict__ tris,
                                                      const JobDev* __restrict__ jobs, PoolDev P, BatchQ q, int in_q,
                                                      unsigned long long* next, int refill, BatchCtl* ctl) {
     __shared__ int lds[kLds][kBlock];
@@ -249,7 +251,8 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_b_connect(const WNode<W>* __restrict__ nodes, const TriV* __restrict__ tris,
+__global__ __launch_bounds__(kBlock) void k_b_connect(const typename TravNode<W>::type* __restrict__ nodes,
+                                                      const TriV* __restrict__ tris,
                                                       const JobDev* __restrict__ jobs, PoolDev P, BatchQ q,
                                                       unsigned long long* next, int refill, BatchCtl* ctl) {
     __shared__ int lds[kLds][kBlock];
@@ -360,6 +363,7 @@ struct Batch {
     // arena (valid for arena_width / arena_leaf)
     int arena_width = 0, arena_leaf = 0;
     DevBuf<float4> nodes;
+    DevBuf<float4> qnodes;   // quantised copy of a 4-wide arena (TravNode<4> = QNode4)
     DevBuf<TriV> tris;
     DevBuf<ShadeTri> stri;
     DevBuf<DevMat> mats;
@@ -367,6 +371,7 @@ struct Batch {
     DevBuf<float> cdf;
     DevBuf<int32_t> ltri;
     std::vector<JobDev> jobdev;   // per job, work_off / film_off filled per run
+    const void* trav_nodes() const { return qnodes.p ? static_cast<const void*>(qnodes.p) : nodes.p; }
     // run state
     DevBuf<JobDev> djobs;
     DevBuf<int64_t> dwork;
@@ -492,6 +497,15 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
         scenes[j] = nullptr;
     }
     LMGPU_HIP(hipGetLastError());
+    qnodes.reset();
+    if (width == 4 && std::is_same<TravNode<4>::type, QNode4>::value) {
+        // after every job's indices are global: quantise the whole arena once
+        if ((rc = qnodes.alloc(size_t(tot[0]) * (sizeof(QNode4) / sizeof(float4)))) ||
+            (rc = quantise_nodes4(nodes.p, qnodes.p, tot[0], s))) {
+            cleanup();
+            return rc;
+        }
+    }
     LMGPU_HIP(hipStreamSynchronize(s));
     cleanup();
     arena_width = width;
@@ -578,26 +592,26 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
         hipLaunchKernelGGL(k_b_zero2, dim3(1), dim3(64), 0, cs, next.p);
         if (width == 2)
             hipLaunchKernelGGL(k_b_extend<2>, dim3(occupancy_grid(k_b_extend<2>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<2>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<2>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         else if (width == 4)
             hipLaunchKernelGGL(k_b_extend<4>, dim3(occupancy_grid(k_b_extend<4>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<4>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<4>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         else
             hipLaunchKernelGGL(k_b_extend<8>, dim3(occupancy_grid(k_b_extend<8>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<8>*>(nodes.p), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<8>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, cs, counts.p, in_q ^ 1);
         const unsigned sg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 2048 * 256 / block)));
         if (block == 128) hipLaunchKernelGGL(k_b_shade<128>, dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q);
         else hipLaunchKernelGGL(k_b_shade<256>, dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q);
         if (width == 2)
             hipLaunchKernelGGL(k_b_connect<2>, dim3(occupancy_grid(k_b_connect<2>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<2>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<2>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
         else if (width == 4)
             hipLaunchKernelGGL(k_b_connect<4>, dim3(occupancy_grid(k_b_connect<4>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<4>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<4>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
         else
             hipLaunchKernelGGL(k_b_connect<8>, dim3(occupancy_grid(k_b_connect<8>, kBlock)), dim3(kBlock), 0, cs,
-                               reinterpret_cast<const WNode<8>*>(nodes.p), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<8>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
     }
     hipError_t e = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {

@@ -269,6 +269,8 @@ int lmgpu_accel_deserialize(lmgpu_accel* accel, const void* buf, size_t size, co
     LMGPU_HIP(hipMemcpy(a.nodes.p, in, nbytes, hipMemcpyHostToDevice));
     LMGPU_HIP(hipMemcpy(a.tris.p, in + nbytes, tbytes, hipMemcpyHostToDevice));
     a.width = int(h.width);
+    a.num_nodes = h.num_nodes;
+    LMGPU_TRY(a.make_trav_nodes(nullptr));
     a.leaf_size = int(h.leaf_size);
     a.refill = (opts && opts->refill > 0) ? std::min(opts->refill, 64) : 32;
     a.num_tris = h.num_tris;

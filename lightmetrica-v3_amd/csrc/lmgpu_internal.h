@@ -67,8 +67,12 @@ struct DevBuf {
     }
 };
 
+// qnodes[i] = quantise_node4(wnodes[i]) for i < n (bvh.h), enqueued on s.
+int quantise_nodes4(const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s);
+
 struct Accel {
     DevBuf<float4> nodes;   // W-wide nodes, 32*W bytes each
+    DevBuf<float4> qnodes;  // quantised copy of 4-wide nodes (QNode4, 64 B each), when enabled
     DevBuf<TriV> tris;      // triangles in traversal order
     DevBuf<int32_t> status; // bit 0: traversal stack overflow
     DevBuf<unsigned long long> ring;  // per-launch ray counters of the persistent kernel
@@ -85,8 +89,14 @@ struct Accel {
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     bool built = false;
 
+    // node array the traversal reads (TravNode<width>::type)
+    const void* trav_nodes() const { return qnodes.p ? static_cast<const void*>(qnodes.p) : nodes.p; }
+    // builds the traversal copy of the nodes where TravNode differs from WNode
+    int make_trav_nodes(hipStream_t s);
+
     void release() {
         nodes.reset();
+        qnodes.reset();
         tris.reset();
         built = false;
     }

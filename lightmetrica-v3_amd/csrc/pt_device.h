@@ -31,7 +31,7 @@ struct alignas(16) DevLight {
 };
 
 struct SceneDev {
-    const WNode<4>* nodes4;
+    const TravNode<4>::type* nodes4;
     const WNode<2>* nodes2;
     const WNode<8>* nodes8;
     const TriV* tris;
@@ -153,13 +153,13 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
 }
 
 template <int W>
-__device__ __forceinline__ const WNode<W>* nodes_of(const SceneDev& sc);
+__device__ __forceinline__ const typename TravNode<W>::type* nodes_of(const SceneDev& sc);
 template <>
-__device__ __forceinline__ const WNode<2>* nodes_of<2>(const SceneDev& sc) { return sc.nodes2; }
+__device__ __forceinline__ const TravNode<2>::type* nodes_of<2>(const SceneDev& sc) { return sc.nodes2; }
 template <>
-__device__ __forceinline__ const WNode<4>* nodes_of<4>(const SceneDev& sc) { return sc.nodes4; }
+__device__ __forceinline__ const TravNode<4>::type* nodes_of<4>(const SceneDev& sc) { return sc.nodes4; }
 template <>
-__device__ __forceinline__ const WNode<8>* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
+__device__ __forceinline__ const TravNode<8>::type* nodes_of<8>(const SceneDev& sc) { return sc.nodes8; }
 
 // A point on area light `li` and its geometric normal (light::area:
 // triangle by the area CDF with Dist::sample's upper_bound, math.h:367-370;
