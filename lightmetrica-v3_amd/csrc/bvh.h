@@ -272,24 +272,36 @@ LMGPU_HD QNode4 quantise_node4(const WNode<4>& n) {
 
 // Per-lane traversal stack: the first LDS_DEPTH entries live in LDS,
 // [entry][lane] so a wave's accesses hit distinct banks, the rest in
-// private memory (only reached by deep, rare traversals).
+// private memory (only reached by deep, rare traversals). The private
+// entries are a separate array owned by the caller, so that the stack
+// pointer itself stays in a register (an aggregate holding the array would
+// be placed in scratch memory whole, stack pointer included).
 template <int LDS_DEPTH, int PRIV_DEPTH, int BLOCK>
 struct Stack {
     int (*lds)[BLOCK];
-    int priv[PRIV_DEPTH];
+    int* priv;
     int sp;
     int lane;
     static constexpr int capacity = LDS_DEPTH + PRIV_DEPTH;
     __device__ __forceinline__ bool push(int e) {
-        if (sp < LDS_DEPTH) lds[sp][lane] = e;
-        else if (sp < capacity) priv[sp - LDS_DEPTH] = e;
-        else return false;
+        if (sp < LDS_DEPTH) {
+            lds[sp][lane] = e;
+        } else {
+            if (sp >= capacity) return false;
+            priv[sp - LDS_DEPTH] = e;
+        }
         sp++;
         return true;
     }
     __device__ __forceinline__ int pop() {
         --sp;
-        return sp < LDS_DEPTH ? lds[sp][lane] : priv[sp - LDS_DEPTH];
+        // an LDS read on every pop (a clamped slot when the entry is
+        // private), so the common case stays a ds_read and is not merged
+        // with the private read into one generic-address load
+        int v = lds[sp < LDS_DEPTH ? sp : 0][lane];
+        asm volatile("" : "+v"(v));
+        if (sp >= LDS_DEPTH) v = priv[sp - LDS_DEPTH];
+        return v;
     }
 };
 
@@ -377,8 +389,10 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
                                                  Fetch fetch, Emit emit, int32_t* status,
                                                  unsigned long long* counters, int refill) {
     const int lane = threadIdx.x & 63;
+    int priv[PRIV_DEPTH];
     Stack<LDS_DEPTH, PRIV_DEPTH, BLOCK> st;
     st.lds = lds;
+    st.priv = priv;
     st.sp = 0;
     st.lane = threadIdx.x;
     RayPre r{};
