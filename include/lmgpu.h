@@ -393,10 +393,10 @@ int lmgpu_guide_stats(const lmgpu_guide* guide, int64_t* steps, float* last_loss
  * included, is one captured hipGraph. */
 typedef struct lmgpu_guide_opts {
     float alpha;              /* guide sampling probability (0.65) */
-    float learning_rate;      /* Adam learning rate (5e-3) */
-    int32_t batch;            /* records per Adam step, rounded up to 128 (2^17) */
+    float learning_rate;      /* Adam learning rate (1e-2) */
+    int32_t batch;            /* records per Adam step, rounded up to 128 (2^16) */
     int32_t steps_per_pass;   /* Adam steps per pass (8); 0 = frozen guide, nothing recorded */
-    int32_t record_depth;     /* recording surface vertices per path (12, at most 16) */
+    int32_t record_depth;     /* recording surface vertices per path (8, at most 16) */
     float weight_clamp;       /* clamp of training weights relative to the batch mean (0 = 50) */
     int32_t reserved[6];
 } lmgpu_guide_opts;

@@ -302,8 +302,8 @@ public:
 //   max_verts (int, required), sampling_mode ("mis" | "nee" | "naive"),
 //   scheduler ("sample" | "time"), spp (int) or render_time (seconds),
 //   seed (int), rr_depth (int; 5, or max_verts when guiding), guiding
-//   (bool) with guide_alpha (0.65), guide_lr (5e-3), guide_batch (2^17),
-//   guide_steps (8), guide_depth (12);
+//   (bool) with guide_alpha (0.65), guide_lr (1e-2), guide_batch (2^16),
+//   guide_steps (8), guide_depth (8);
 //   accel_width, accel_leaf_size (BVH of the renderer's device scene).
 class Renderer_PT_MI355X final : public Renderer {
 public:
