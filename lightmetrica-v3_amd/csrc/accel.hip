@@ -362,11 +362,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Node* __restrict__ 
                                                                          status, counters, refill);
 }
 
-// Quantised copy of a built 4-wide node array (bvh.h, quantise_node4).
-__global__ __launch_bounds__(256) void k_quantise4(const WNode<4>* __restrict__ in, QNode4* __restrict__ out,
-                                                   int64_t n) {
+// Quantised copy of a built W-wide node array (bvh.h, quantise_node).
+template <int W>
+__global__ __launch_bounds__(256) void k_quantise(const WNode<W>* __restrict__ in, QNode<W>* __restrict__ out,
+                                                  int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) out[i] = quantise_node4(in[i]);
+    if (i < n) out[i] = quantise_node<W>(in[i]);
 }
 
 __global__ void k_zero_counter(unsigned long long* c) {
@@ -559,19 +560,30 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     return LMGPU_OK;
 }
 
-int quantise_nodes4(const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s) {
+bool quantised_width(int width) {
+    return (width == 4 && std::is_same<TravNode<4>::type, QNode<4>>::value) ||
+           (width == 8 && std::is_same<TravNode<8>::type, QNode<8>>::value);
+}
+
+int quantise_nodes(int width, const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s) {
     if (n <= 0) return LMGPU_OK;
-    hipLaunchKernelGGL(k_quantise4, dim3(nblocks(n, 256)), dim3(256), 0, s, reinterpret_cast<const WNode<4>*>(wnodes),
-                       reinterpret_cast<QNode4*>(qnodes), n);
+    if (width == 4)
+        hipLaunchKernelGGL(k_quantise<4>, dim3(nblocks(n, 256)), dim3(256), 0, s,
+                           reinterpret_cast<const WNode<4>*>(wnodes), reinterpret_cast<QNode<4>*>(qnodes), n);
+    else if (width == 8)
+        hipLaunchKernelGGL(k_quantise<8>, dim3(nblocks(n, 256)), dim3(256), 0, s,
+                           reinterpret_cast<const WNode<8>*>(wnodes), reinterpret_cast<QNode<8>*>(qnodes), n);
+    else
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "quantised nodes are 4- or 8-wide");
     LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }
 
 int Accel::make_trav_nodes(hipStream_t s) {
     qnodes.reset();
-    if (!std::is_same<TravNode<4>::type, QNode4>::value || width != 4) return LMGPU_OK;
-    LMGPU_TRY(qnodes.alloc(size_t(num_nodes) * (sizeof(QNode4) / sizeof(float4))));
-    LMGPU_TRY(quantise_nodes4(nodes.p, qnodes.p, num_nodes, s));
+    if (!quantised_width(width)) return LMGPU_OK;
+    LMGPU_TRY(qnodes.alloc(size_t(num_nodes) * (size_t(width) * 16 / sizeof(float4))));
+    LMGPU_TRY(quantise_nodes(width, nodes.p, qnodes.p, num_nodes, s));
     LMGPU_HIP(hipStreamSynchronize(s));
     return LMGPU_OK;
 }

@@ -363,7 +363,7 @@ struct Batch {
     // arena (valid for arena_width / arena_leaf)
     int arena_width = 0, arena_leaf = 0;
     DevBuf<float4> nodes;
-    DevBuf<float4> qnodes;   // quantised copy of a 4-wide arena (TravNode<4> = QNode4)
+    DevBuf<float4> qnodes;   // quantised copy of the arena, when quantised_width(width)
     DevBuf<TriV> tris;
     DevBuf<ShadeTri> stri;
     DevBuf<DevMat> mats;
@@ -498,10 +498,10 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
     }
     LMGPU_HIP(hipGetLastError());
     qnodes.reset();
-    if (width == 4 && std::is_same<TravNode<4>::type, QNode4>::value) {
+    if (quantised_width(width)) {
         // after every job's indices are global: quantise the whole arena once
-        if ((rc = qnodes.alloc(size_t(tot[0]) * (sizeof(QNode4) / sizeof(float4)))) ||
-            (rc = quantise_nodes4(nodes.p, qnodes.p, tot[0], s))) {
+        if ((rc = qnodes.alloc(size_t(tot[0]) * (size_t(width) * 16 / sizeof(float4)))) ||
+            (rc = quantise_nodes(width, nodes.p, qnodes.p, tot[0], s))) {
             cleanup();
             return rc;
         }

@@ -28,9 +28,10 @@
 #define LMGPU_LEAF_VOTE_X4 2
 #endif
 
-// Traversal of a 4-wide BVH reads a quantised copy of its nodes (QNode4,
-// 64 B instead of 128 B) when 1: +15% incoherent closest-hit rays on the
-// 1.32M-triangle mesh with identical hits; coherent rays unchanged.
+// Traversal of a 4- or 8-wide BVH reads a quantised copy of its nodes
+// (QNode<W>: 64 B instead of 128 B for W = 4, 128 B instead of 256 B for
+// W = 8) when 1: +15% incoherent closest-hit rays on the 1.32M-triangle mesh
+// at W = 4 with identical hits.
 #ifndef LMGPU_QNODE
 #define LMGPU_QNODE 1
 #endif
@@ -46,22 +47,26 @@ struct alignas(16) WNode {
 };
 static_assert(sizeof(WNode<4>) == 128, "BVH4 node must be one 128-B line");
 
-// Quantised 4-wide node, 64 B (half a 128-B line): the child boxes are
-// 8-bit planes on a per-node, per-axis grid, plane = fma(q, 2^e, origin),
-// rounded outward when built (quantise_node4), so the decoded boxes contain
-// the exact ones and traversal visits a superset of the nodes of the FP32
-// tree: hits are unchanged. Child c's lower / upper plane on axis a is byte
-// c of qlo[a] / qhi[a]; ent[c] is its stack entry (inner node index >= 0, or
+// Quantised W-wide node (W = 4: 64 B, half a 128-B line; W = 8: 128 B):
+// the child boxes are 8-bit planes on a per-node, per-axis grid,
+// plane = fma(q, 2^e, origin), rounded outward when built (quantise_node),
+// so the decoded boxes contain the exact ones and traversal visits a
+// superset of the nodes of the FP32 tree: hits are unchanged. Child c's
+// lower / upper plane on axis a is byte c % 4 of qlo[a][c / 4] / qhi[a][c / 4];
+// ent[c] is its stack entry (inner node index >= 0, or
 // encode_leaf(first, count) < 0), kQEmpty for an empty slot.
-struct alignas(64) QNode4 {
+template <int W>
+struct alignas(W * 16) QNode {
     float ox, oy, oz;
     uint32_t e;        // biased FP32 exponents of the x, y, z grid steps in bytes 0, 1, 2
-    uint32_t qlo[3];
-    uint32_t qhi[3];
-    int32_t ent[4];
-    int32_t pad[2];
+    uint32_t qlo[3][W / 4];
+    uint32_t qhi[3][W / 4];
+    int32_t ent[W];
+    int32_t pad[W == 4 ? 2 : 8];
 };
-static_assert(sizeof(QNode4) == 64, "quantised BVH4 node must be 64 B");
+using QNode4 = QNode<4>;
+static_assert(sizeof(QNode<4>) == 64, "quantised BVH4 node must be 64 B");
+static_assert(sizeof(QNode<8>) == 128, "quantised BVH8 node must be 128 B");
 constexpr int32_t kQEmpty = INT32_MIN;
 
 // Node type the traversal of a W-wide BVH reads.
@@ -72,7 +77,11 @@ struct TravNode {
 #if LMGPU_QNODE
 template <>
 struct TravNode<4> {
-    using type = QNode4;
+    using type = QNode<4>;
+};
+template <>
+struct TravNode<8> {
+    using type = QNode<8>;
 };
 #endif
 
@@ -198,18 +207,19 @@ LMGPU_HD float qplane(uint32_t q, int c, float step, float origin) {
 }
 
 template <int W>
-__device__ __forceinline__ void child_keys(const QNode4& nd, const RayPre& r, float tmin, float tmax,
+__device__ __forceinline__ void child_keys(const QNode<W>& nd, const RayPre& r, float tmin, float tmax,
                                            float (&key)[W], int (&ent)[W]) {
-    static_assert(W == 4, "quantised nodes are 4-wide");
     const float sx = __uint_as_float((nd.e & 0xffu) << 23);
     const float sy = __uint_as_float(((nd.e >> 8) & 0xffu) << 23);
     const float sz = __uint_as_float(((nd.e >> 16) & 0xffu) << 23);
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < W; c++) {
         bool hit;
-        const float tn = box_near(r, qplane(nd.qlo[0], c, sx, nd.ox), qplane(nd.qlo[1], c, sy, nd.oy),
-                                  qplane(nd.qlo[2], c, sz, nd.oz), qplane(nd.qhi[0], c, sx, nd.ox),
-                                  qplane(nd.qhi[1], c, sy, nd.oy), qplane(nd.qhi[2], c, sz, nd.oz), tmin, tmax, hit);
+        const int w = c >> 2, b = c & 3;
+        const float tn = box_near(r, qplane(nd.qlo[0][w], b, sx, nd.ox), qplane(nd.qlo[1][w], b, sy, nd.oy),
+                                  qplane(nd.qlo[2][w], b, sz, nd.oz), qplane(nd.qhi[0][w], b, sx, nd.ox),
+                                  qplane(nd.qhi[1][w], b, sy, nd.oy), qplane(nd.qhi[2][w], b, sz, nd.oz), tmin, tmax,
+                                  hit);
         const int e = nd.ent[c];
         hit = hit && e != kQEmpty;
         key[c] = hit ? tn : __builtin_inff();
@@ -217,19 +227,20 @@ __device__ __forceinline__ void child_keys(const QNode4& nd, const RayPre& r, fl
     }
 }
 
-// Quantises a built 4-wide node: per axis the grid starts at the smallest
+// Quantises a built W-wide node: per axis the grid starts at the smallest
 // child plane, with the smallest power-of-two step that spans the node in
 // 255 steps; each plane is rounded outward in the exact FP32 expression the
 // traversal decodes (qplane), so every decoded box contains the original.
-LMGPU_HD QNode4 quantise_node4(const WNode<4>& n) {
-    QNode4 q;
+template <int W>
+LMGPU_HD QNode<W> quantise_node(const WNode<W>& n) {
+    QNode<W> q;
     const float* lo[3] = {n.lox, n.loy, n.loz};
     const float* hi[3] = {n.hix, n.hiy, n.hiz};
     float org[3];
     uint32_t ebits = 0;
     for (int a = 0; a < 3; a++) {
         float mn = __builtin_inff(), mx = -__builtin_inff();
-        for (int c = 0; c < 4; c++)
+        for (int c = 0; c < W; c++)
             if (n.count[c] >= 0) {
                 mn = std::min(mn, lo[a][c]);
                 mx = std::max(mx, hi[a][c]);
@@ -242,9 +253,9 @@ LMGPU_HD QNode4 quantise_node4(const WNode<4>& n) {
         if (span > 0.0) e = std::max(-126, int(std::ceil(std::log2(span / 255.0))));
         while (e < 127 && std::fma(255.f, std::ldexp(1.f, e), mn) < mx) e++;
         ebits |= uint32_t(e + 127) << (8 * a);
-        uint32_t ql = 0, qh = 0;
+        uint32_t ql[W / 4] = {}, qh[W / 4] = {};
         const float step = std::ldexp(1.f, e);
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < W; c++) {
             int l = 0, h = 0;
             if (n.count[c] >= 0) {
                 l = int(std::floor((double(lo[a][c]) - double(mn)) / double(step)));
@@ -254,19 +265,21 @@ LMGPU_HD QNode4 quantise_node4(const WNode<4>& n) {
                 h = std::min(std::max(h, 0), 255);
                 while (h < 255 && std::fma(float(h), step, mn) < hi[a][c]) h++;
             }
-            ql |= uint32_t(l) << (8 * c);
-            qh |= uint32_t(h) << (8 * c);
+            ql[c >> 2] |= uint32_t(l) << (8 * (c & 3));
+            qh[c >> 2] |= uint32_t(h) << (8 * (c & 3));
         }
-        q.qlo[a] = ql;
-        q.qhi[a] = qh;
+        for (int k = 0; k < W / 4; k++) {
+            q.qlo[a][k] = ql[k];
+            q.qhi[a][k] = qh[k];
+        }
     }
     q.ox = org[0];
     q.oy = org[1];
     q.oz = org[2];
     q.e = ebits;
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < W; c++)
         q.ent[c] = n.count[c] < 0 ? kQEmpty : (n.count[c] == 0 ? n.child[c] : encode_leaf(n.child[c], n.count[c]));
-    q.pad[0] = q.pad[1] = 0;
+    for (int k = 0; k < int(sizeof(q.pad) / sizeof(q.pad[0])); k++) q.pad[k] = 0;
     return q;
 }
 

@@ -67,12 +67,15 @@ struct DevBuf {
     }
 };
 
-// qnodes[i] = quantise_node4(wnodes[i]) for i < n (bvh.h), enqueued on s.
-int quantise_nodes4(const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s);
+// True when the traversal of a `width`-wide BVH reads quantised nodes
+// (TravNode<width> = QNode<width>, 16 * width bytes each).
+bool quantised_width(int width);
+// qnodes[i] = quantise_node<width>(wnodes[i]) for i < n (bvh.h), enqueued on s.
+int quantise_nodes(int width, const float4* wnodes, float4* qnodes, int64_t n, hipStream_t s);
 
 struct Accel {
     DevBuf<float4> nodes;   // W-wide nodes, 32*W bytes each
-    DevBuf<float4> qnodes;  // quantised copy of 4-wide nodes (QNode4, 64 B each), when enabled
+    DevBuf<float4> qnodes;  // quantised copy of the nodes (QNode<width>), when quantised_width(width)
     DevBuf<TriV> tris;      // triangles in traversal order
     DevBuf<int32_t> status; // bit 0: traversal stack overflow
     DevBuf<unsigned long long> ring;  // per-launch ray counters of the persistent kernel

@@ -550,7 +550,7 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
     dev = SceneDev{};
     if (accel.width == 2) dev.nodes2 = reinterpret_cast<const WNode<2>*>(accel.nodes.p);
     if (accel.width == 4) dev.nodes4 = static_cast<const TravNode<4>::type*>(accel.trav_nodes());
-    if (accel.width == 8) dev.nodes8 = reinterpret_cast<const WNode<8>*>(accel.nodes.p);
+    if (accel.width == 8) dev.nodes8 = static_cast<const TravNode<8>::type*>(accel.trav_nodes());
     dev.tris = accel.tris.p;
     dev.stri = stri.p;
     dev.mats = mats.p;

@@ -33,7 +33,7 @@ struct alignas(16) DevLight {
 struct SceneDev {
     const TravNode<4>::type* nodes4;
     const WNode<2>* nodes2;
-    const WNode<8>* nodes8;
+    const TravNode<8>::type* nodes8;
     const TriV* tris;
     const ShadeTri* stri;
     const DevMat* mats;

@@ -1,5 +1,5 @@
-// Host check of the quantised BVH4 node (csrc/bvh.h, quantise_node4 /
-// qplane): for random 4-wide nodes over many scales and offsets, every
+// Host check of the quantised BVH nodes (csrc/bvh.h, quantise_node /
+// qplane): for random 4- and 8-wide nodes over many scales and offsets, every
 // decoded child box contains the original one, the grid is tight (within
 // one step plus rounding of the exact planes), and stack entries and empty
 // slots are encoded as the traversal expects. Prints "ok <nodes>" or the
@@ -9,19 +9,18 @@
 
 #include "../../lightmetrica-v3_amd/csrc/bvh.h"
 
-using lmgpu::QNode4;
+using lmgpu::QNode;
 using lmgpu::WNode;
 
-int main() {
-    std::mt19937_64 rng(12345);
+template <int W>
+int check(std::mt19937_64& rng, int n_nodes) {
     std::uniform_real_distribution<double> U(0.0, 1.0);
-    const int kNodes = 200000;
-    for (int it = 0; it < kNodes; it++) {
-        WNode<4> n;
+    for (int it = 0; it < n_nodes; it++) {
+        WNode<W> n;
         const double offset = (U(rng) - 0.5) * std::pow(10.0, int(U(rng) * 9) - 4);   // up to +-5e4
         const double extent = std::pow(10.0, U(rng) * 10 - 7);                          // 1e-7 .. 1e3
-        const int nvalid = 1 + int(U(rng) * 4);
-        for (int c = 0; c < 4; c++) {
+        const int nvalid = 1 + int(U(rng) * W);
+        for (int c = 0; c < W; c++) {
             float* lo[3] = {&n.lox[c], &n.loy[c], &n.loz[c]};
             float* hi[3] = {&n.hix[c], &n.hiy[c], &n.hiz[c]};
             for (int a = 0; a < 3; a++) {
@@ -37,36 +36,43 @@ int main() {
                 n.child[c] = 0;
             }
         }
-        const QNode4 q = lmgpu::quantise_node4(n);
+        const QNode<W> q = lmgpu::quantise_node<W>(n);
         const float org[3] = {q.ox, q.oy, q.oz};
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < W; c++) {
             const int want = n.count[c] < 0 ? lmgpu::kQEmpty
                              : n.count[c] == 0 ? n.child[c] : lmgpu::encode_leaf(n.child[c], n.count[c]);
             if (q.ent[c] != want) {
-                std::printf("node %d child %d: entry %d, expected %d\n", it, c, q.ent[c], want);
+                std::printf("W%d node %d child %d: entry %d, expected %d\n", W, it, c, q.ent[c], want);
                 return 1;
             }
             if (n.count[c] < 0) continue;
             const float lo[3] = {n.lox[c], n.loy[c], n.loz[c]}, hi[3] = {n.hix[c], n.hiy[c], n.hiz[c]};
             for (int a = 0; a < 3; a++) {
                 const float step = std::ldexp(1.f, int((q.e >> (8 * a)) & 0xffu) - 127);
-                const float dl = lmgpu::qplane(q.qlo[a], c, step, org[a]);
-                const float dh = lmgpu::qplane(q.qhi[a], c, step, org[a]);
+                const float dl = lmgpu::qplane(q.qlo[a][c / 4], c % 4, step, org[a]);
+                const float dh = lmgpu::qplane(q.qhi[a][c / 4], c % 4, step, org[a]);
                 if (!(dl <= lo[a] && dh >= hi[a])) {
-                    std::printf("node %d child %d axis %d: decoded [%.9g, %.9g] misses [%.9g, %.9g]\n", it, c, a, dl,
-                                dh, lo[a], hi[a]);
+                    std::printf("W%d node %d child %d axis %d: decoded [%.9g, %.9g] misses [%.9g, %.9g]\n", W, it, c,
+                                a, dl, dh, lo[a], hi[a]);
                     return 1;
                 }
                 // tight: at most one grid step (plus the rounding of the decoded plane) outside
                 const double slack = double(step) + 4.0 * std::fabs(double(org[a])) * 1.2e-7;
                 if (double(lo[a]) - dl > slack || dh - double(hi[a]) > slack) {
-                    std::printf("node %d child %d axis %d: loose decode [%.9g, %.9g] of [%.9g, %.9g], step %g\n", it,
-                                c, a, dl, dh, lo[a], hi[a], step);
+                    std::printf("W%d node %d child %d axis %d: loose decode [%.9g, %.9g] of [%.9g, %.9g], step %g\n",
+                                W, it, c, a, dl, dh, lo[a], hi[a], step);
                     return 1;
                 }
             }
         }
     }
-    std::printf("ok %d\n", kNodes);
+    return 0;
+}
+
+int main() {
+    std::mt19937_64 rng(12345);
+    const int kNodes = 200000;
+    if (check<4>(rng, kNodes) || check<8>(rng, kNodes)) return 1;
+    std::printf("ok %d\n", 2 * kNodes);
     return 0;
 }

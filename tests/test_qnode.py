@@ -1,8 +1,9 @@
-"""Quantised BVH4 nodes (csrc/bvh.h): the host build of quantise_node4 and
-qplane, compiled with hipcc, decodes boxes that contain the exact child
-boxes on 200,000 random nodes (1e-7..1e3 extents, offsets up to 5e4, flat
-boxes, partly empty nodes), stays within one grid step of them, and encodes
-stack entries and empty slots as the traversal reads them. GPU hit parity
+"""Quantised BVH nodes (csrc/bvh.h): the host build of quantise_node and
+qplane, compiled with hipcc, decodes 4- and 8-wide boxes that contain the
+exact child boxes on 200,000 random nodes of each width (1e-7..1e3 extents,
+offsets up to 5e4, flat boxes, partly empty nodes), stays within one grid
+step of them, and encodes stack entries and empty slots as the traversal
+reads them. GPU hit parity
 with the quantised traversal is covered by the accel tests."""
 import os
 import shutil
