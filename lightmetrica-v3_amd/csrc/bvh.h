@@ -36,6 +36,14 @@
 #define LMGPU_QNODE 1
 #endif
 
+// Rays a wave takes from the global ray counter with one atomic (at least
+// 64, so one chunk always covers the idle lanes of a refill); the wave hands
+// them to its idle lanes over several refills.
+#ifndef LMGPU_RAY_CHUNK
+#define LMGPU_RAY_CHUNK 64
+#endif
+static_assert(LMGPU_RAY_CHUNK >= 64, "a ray chunk must cover a whole wave");
+
 namespace lmgpu {
 
 template <int W>
@@ -384,11 +392,13 @@ __device__ __forceinline__ void sort_children(float (&k)[W], int (&e)[W]) {
 }
 
 // Persistent traversal: every lane owns one ray at a time; when at least
-// `refill` lanes of a wave are idle (or the whole wave is), the wave takes
-// that many new rays from one global counter with one atomic (Aila & Laine,
+// `refill` lanes of a wave are idle (or the whole wave is), the idle lanes
+// take new rays from the wave's pool, which the wave refills by chunks of
+// LMGPU_RAY_CHUNK from one global counter with one atomic (Aila & Laine,
 // HPG 2009, "persistent threads"), so lanes do not sit idle behind a wave's
-// slowest ray. refill = 64 keeps 64 consecutive rays per wave (best for
-// coherent rays), small values keep lanes busy (best for incoherent rays).
+// slowest ray and the counter sees few atomics. refill = 64 keeps whole
+// waves of consecutive rays together (best for coherent rays), smaller
+// values keep lanes busy.
 // Each loop iteration advances the lanes of one kind of step: a wide node or
 // a leaf, chosen per wave by vote. Closest hit (ANY = false) keeps shrinking
 // tmax; an accepted triangle is replaced by a later one at t == tmax, as the
@@ -412,6 +422,8 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
     TraceResult res{};
     int64_t rid = -1;       // ray of this lane, -1 idle
     bool exhausted = false;
+    // wave-uniform: ray ids taken from `next` and not yet handed to a lane
+    int64_t pool_next = 0, pool_end = 0;
     float tmin = 0.f;
     int cur = 0;
     unsigned long long nv = 0, nt = 0;
@@ -421,13 +433,19 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
         const unsigned long long busy_mask = __ballot(rid >= 0);
         const int nidle = __popcll(idle_mask);
         if (nidle >= refill || (busy_mask == 0 && nidle > 0)) {
-            const int leader = __ffsll(idle_mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(next, (unsigned long long)nidle);
-            base = (unsigned long long)__shfl((long long)base, leader);
+            // idle lanes take, in lane order, the rest of the wave's pool and
+            // then the start of a new chunk of LMGPU_RAY_CHUNK ids
+            const int64_t avail = pool_end - pool_next;
+            int64_t chunk = 0;
+            if (avail < nidle) {
+                const int leader = __ffsll(idle_mask) - 1;
+                unsigned long long b = 0;
+                if (lane == leader) b = atomicAdd(next, (unsigned long long)LMGPU_RAY_CHUNK);
+                chunk = int64_t(__shfl((long long)b, leader));
+            }
             if (idle) {
                 const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull));
-                const int64_t id = int64_t(base) + rank;
+                const int64_t id = rank < avail ? pool_next + rank : chunk + (rank - avail);
                 if (id < n) {
                     rid = id;
                     f3 o, d;
@@ -443,6 +461,12 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
                 } else {
                     exhausted = true;
                 }
+            }
+            if (avail < nidle) {
+                pool_next = chunk + (nidle - avail);
+                pool_end = chunk + LMGPU_RAY_CHUNK;
+            } else {
+                pool_next += nidle;
             }
         }
         if (__ballot(rid >= 0) == 0) break;
