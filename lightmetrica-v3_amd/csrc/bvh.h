@@ -441,7 +441,10 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
                 const int leader = __ffsll(idle_mask) - 1;
                 unsigned long long b = 0;
                 if (lane == leader) b = atomicAdd(next, (unsigned long long)LMGPU_RAY_CHUNK);
-                chunk = int64_t(__shfl((long long)b, leader));
+                b = (unsigned long long)__shfl((long long)b, leader);
+                // wave-uniform from here on: scalar registers for the pool
+                chunk = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(b >> 32)))) << 32) |
+                                uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(b))));
             }
             if (idle) {
                 const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull));
