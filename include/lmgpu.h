@@ -94,7 +94,8 @@ typedef struct lmgpu_accel_info {
     int32_t leaf_size;
     int32_t depth;           /* wide-tree depth (root = 1) */
     int32_t stack_capacity;  /* traversal stack entries available per ray */
-    int64_t node_bytes;
+    int64_t node_bytes;      /* exact nodes (32 * width B each) plus, for width 4 and 8, their
+                                quantised traversal copy (16 * width B each) */
     int64_t tri_bytes;
     double sah_cost;         /* SAH cost of the wide tree (node=1, tri=1 units) */
     double build_ms;         /* device time of the last build */

@@ -89,7 +89,7 @@ int lmgpu_accel_info_get(const lmgpu_accel* accel, lmgpu_accel_info* info) {
     info->leaf_size = a.leaf_size;
     info->depth = a.depth;
     info->stack_capacity = 24 + 72;
-    info->node_bytes = a.num_nodes * 32 * a.width;
+    info->node_bytes = a.num_nodes * 32 * a.width + int64_t(a.qnodes.n) * int64_t(sizeof(float4));
     info->tri_bytes = a.num_tris * int64_t(sizeof(lmgpu::TriV));
     info->sah_cost = a.sah_cost;
     info->build_ms = a.build_ms;
