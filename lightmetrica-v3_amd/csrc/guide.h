@@ -135,12 +135,23 @@ __device__ __forceinline__ f3 mixture_sample(const float* __restrict__ o, float 
 // at every third octave and the two octaves after it follow from the
 // double-angle identities sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x,
 // which at most quadruple the FP32 error of the anchor (below 1e-6).
-__device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box_lo, const float* box_inv,
-                                             uint16_t* out) {
+// Layout: features 24a .. 24a + 23 for axis a, then normal (72..74),
+// outgoing direction (75..77) and two zero pads.
+constexpr int kTailFeature = 6 * kFreq;   // 72
+
+__device__ __forceinline__ void scene_coords(f3 p, const float* box_lo, const float* box_inv, float (&q)[3]) {
+    q[0] = (p.x - box_lo[0]) * box_inv[0];
+    q[1] = (p.y - box_lo[1]) * box_inv[1];
+    q[2] = (p.z - box_lo[2]) * box_inv[2];
+}
+
+// features of axes [A0, A1) into out[2 * kFreq * A0 ...]
+template <int A0, int A1>
+__device__ __forceinline__ void encode_axes(const float (&q)[3], uint16_t* out) {
     static_assert(kFreq % 3 == 0, "octaves come in groups of three");
-    const float q[3] = {(p.x - box_lo[0]) * box_inv[0], (p.y - box_lo[1]) * box_inv[1], (p.z - box_lo[2]) * box_inv[2]};
-    int o = 0;
-    for (int a = 0; a < 3; a++) {
+    int o = 2 * kFreq * A0;
+#pragma unroll
+    for (int a = A0; a < A1; a++) {
         const float f = 3.14159265358979f * q[a];
 #pragma unroll
         for (int j0 = 0; j0 < kFreq; j0 += 3) {
@@ -156,14 +167,65 @@ __device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box
             }
         }
     }
-    out[o++] = f2bf(n.x);
-    out[o++] = f2bf(n.y);
-    out[o++] = f2bf(n.z);
-    out[o++] = f2bf(wo.x);
-    out[o++] = f2bf(wo.y);
-    out[o++] = f2bf(wo.z);
-    out[o++] = 0;
-    out[o++] = 0;
+}
+
+// normal, outgoing direction and pads into out[kTailFeature ...]
+__device__ __forceinline__ void encode_tail(f3 n, f3 wo, uint16_t* out) {
+    uint16_t* t = out + kTailFeature;
+    t[0] = f2bf(n.x);
+    t[1] = f2bf(n.y);
+    t[2] = f2bf(n.z);
+    t[3] = f2bf(wo.x);
+    t[4] = f2bf(wo.y);
+    t[5] = f2bf(wo.z);
+    t[6] = 0;
+    t[7] = 0;
+}
+
+__device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box_lo, const float* box_inv,
+                                             uint16_t* out) {
+    float q[3];
+    scene_coords(p, box_lo, box_inv, q);
+    encode_axes<0, 3>(q, out);
+    encode_tail(n, wo, out);
+}
+
+// Half of the encoding for a pair of lanes sharing one row: half 0 writes
+// axes x and y (features 0..47), half 1 axis z, normal, direction and pads
+// (48..79).
+__device__ __forceinline__ void encode_input_half(int half, f3 p, f3 n, f3 wo, const float* box_lo,
+                                                  const float* box_inv, uint16_t* out) {
+    float q[3];
+    scene_coords(p, box_lo, box_inv, q);
+    if (half == 0) {
+        encode_axes<0, 2>(q, out);
+    } else {
+        encode_axes<2, 3>(q, out);
+        encode_tail(n, wo, out);
+    }
+}
+
+// Decoded lobes [k0, k0 + 4) of raw outputs y (40 floats) written as
+// o[k] = pi, o[kLobes + k] = kappa, o[2 kLobes + 3k ..] = mu: the same
+// arithmetic as decode_mixture, so that two lanes can decode one row.
+__device__ __forceinline__ void decode_lobes4(const float* y, int k0, float* __restrict__ o) {
+    float mx = y[0];
+    for (int k = 1; k < kLobes; k++) mx = fmaxf(mx, y[k]);
+    float s = 0.f;
+    for (int k = 0; k < kLobes; k++) s += expf(y[k] - mx);
+    const float inv = 1.f / s;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+        const int k = k0 + kk;
+        o[k] = expf(y[k] - mx) * inv;
+        o[kLobes + k] = fminf(fmaxf(softplus(y[kLobes + k]), kKappaMin), kKappaMax);
+        const f3 r = mk(y[2 * kLobes + 3 * k], y[2 * kLobes + 3 * k + 1], y[2 * kLobes + 3 * k + 2]);
+        const float l = length(r);
+        const f3 mu = l > 1e-8f ? r / l : mk(0.f, 0.f, 1.f);
+        o[2 * kLobes + 3 * k] = mu.x;
+        o[2 * kLobes + 3 * k + 1] = mu.y;
+        o[2 * kLobes + 3 * k + 2] = mu.z;
+    }
 }
 
 }  // namespace lmgpu

@@ -66,21 +66,31 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
     uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
     uint16_t* lh = lx + 32 * kPitchX;
     float* ly = reinterpret_cast<float*>(lx);
+    // two lanes per row: lanes r and r + 32 share row r of the wave's 32-row
+    // tile and each encode, save and decode half of it
+    const int r = lane & 31, half = lane >> 5;
+    const int k_lo = half * (kIn / 2), k_hi = k_lo + kIn / 2;   // feature half for zeroing and saving
     for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < n; base += int64_t(gridDim.x) * kRowsPerBlock) {
         const int64_t i0 = base + wave * 32;
-        if (lane < 32) {
-            const int64_t i = i0 + lane;
-            uint16_t* row = lx + lane * kPitchX;
+        {
+            const int64_t i = i0 + r;
+            uint16_t* row = lx + r * kPitchX;
             if (i < n) {
                 const float* q = query + i * qs;
-                encode_input(mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), box.lo, box.inv, row);
+                encode_input_half(half, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), box.lo,
+                                  box.inv, row);
             } else {
-                for (int k = 0; k < kIn; k++) row[k] = 0;
+                for (int k = k_lo; k < k_hi; k++) row[k] = 0;
             }
-            if (TRAIN && i < B)
-                for (int k = 0; k < kIn; k++) xT[int64_t(k) * B + i] = row[k];
         }
         __syncthreads();
+        if (TRAIN) {
+            // the feature-major copy for the weight gradients, after both halves of each row are written
+            const int64_t i = i0 + r;
+            const uint16_t* row = lx + r * kPitchX;
+            if (i < B)
+                for (int k = k_lo; k < k_hi; k++) xT[int64_t(k) * B + i] = row[k];
+        }
         mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int row, int col, uint16_t v) {
             if (TRAIN) {
                 const int64_t i = i0 + row;
@@ -88,23 +98,14 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
             }
         });
         __syncthreads();
-        if (lane < 32) {
-            const int64_t i = i0 + lane;
+        {
+            const int64_t i = i0 + r;
             if (i < n) {
-                const float* y = ly + lane * kPitchY;
+                const float* y = ly + r * kPitchY;
                 if (TRAIN) {
-                    for (int k = 0; k < kOut; k++) yout[i * kOut + k] = y[k];
+                    for (int k = half * (kOut / 2); k < (half + 1) * (kOut / 2); k++) yout[i * kOut + k] = y[k];
                 } else {
-                    Mixture m;
-                    decode_mixture(y, m);
-                    float* o = mix + i * kOut;
-                    for (int k = 0; k < kLobes; k++) {
-                        o[k] = m.pi[k];
-                        o[kLobes + k] = m.kappa[k];
-                        o[2 * kLobes + 3 * k] = m.mu[k].x;
-                        o[2 * kLobes + 3 * k + 1] = m.mu[k].y;
-                        o[2 * kLobes + 3 * k + 2] = m.mu[k].z;
-                    }
+                    decode_lobes4(y, 4 * half, mix + i * kOut);
                 }
             }
         }

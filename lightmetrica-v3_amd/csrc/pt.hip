@@ -177,11 +177,14 @@ __global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, i
     uint16_t* lh = lx + 32 * kPitchX;
     float* ly = reinterpret_cast<float*>(lx);
     const int n = q.counts[in_q];
+    // two lanes per row: lanes r and r + 32 share row r of the wave's 32-row
+    // tile and each encode and decode half of it
+    const int r = lane & 31, half = lane >> 5;
     for (int base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
         const int i0 = base + wave * 32;
-        if (lane < 32) {
-            const int i = i0 + lane;
-            uint16_t* row = lx + lane * kPitchX;
+        {
+            const int i = i0 + r;
+            uint16_t* row = lx + r * kPitchX;
             const float4 h = i < n ? reinterpret_cast<const float4*>(q.hit)[i] : make_float4(__int_as_float(-1), 0, 0, 0);
             const int tri = __float_as_int(h.x);
             if (tri >= 0) {
@@ -191,29 +194,18 @@ __global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, i
                                mk(st.p2.x, st.p2.y, st.p2.z) * b2;
                 const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
                                          mk(st.n2.x, st.n2.y, st.n2.z) * b2);
-                const lmgpu_ray& r = q.ext[in_q][i];
-                encode_input(pos, nsh, mk(-r.dir[0], -r.dir[1], -r.dir[2]), box.lo, box.inv, row);
+                const lmgpu_ray& ray = q.ext[in_q][i];
+                encode_input_half(half, pos, nsh, mk(-ray.dir[0], -ray.dir[1], -ray.dir[2]), box.lo, box.inv, row);
             } else {
-                for (int k = 0; k < kIn; k++) row[k] = 0;
+                for (int k = half * (kIn / 2); k < (half + 1) * (kIn / 2); k++) row[k] = 0;
             }
         }
         __syncthreads();
         mlp_tile_forward(wl, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
         __syncthreads();
-        if (lane < 32) {
-            const int i = i0 + lane;
-            if (i < n) {
-                Mixture m;
-                decode_mixture(ly + lane * kPitchY, m);
-                float* o = mix + int64_t(i) * kOut;
-                for (int k = 0; k < kLobes; k++) {
-                    o[k] = m.pi[k];
-                    o[kLobes + k] = m.kappa[k];
-                    o[2 * kLobes + 3 * k] = m.mu[k].x;
-                    o[2 * kLobes + 3 * k + 1] = m.mu[k].y;
-                    o[2 * kLobes + 3 * k + 2] = m.mu[k].z;
-                }
-            }
+        {
+            const int i = i0 + r;
+            if (i < n) decode_lobes4(ly + r * kPitchY, 4 * half, mix + int64_t(i) * kOut);
         }
         __syncthreads();
     }
