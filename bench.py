@@ -116,7 +116,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU only; --gpus must be 1 '
-                          '(rays and samples shard by index range with a replicated BVH; see DESIGN.md)'}))
+                          '(rays and samples would shard by index range over a replicated BVH, '
+                          'with one sum of the FP64 films at the end)'}))
         sys.exit(2)
 
     import numpy as np
