@@ -2,7 +2,8 @@
 incoherent rays, preceded by a plain device copy of known size that
 calibrates the HBM byte counters; optionally one 1080p config-2
 path-tracing pass (--pt), 4 + 4 plain (roulette from vertex 5 / none) and 4 guided (no
-roulette) 1080p passes of config 4 (--c4), a config-3 batch (--c3 N) and light-tracing / BDPT passes (--lt-bdpt).
+roulette) 1080p passes of config 4 (--c4), a config-3 batch (--c3 N), light-tracing / BDPT passes (--lt-bdpt) and
+the guide MLP's training and inference kernels (--guide).
 
 Run under rocprofv3, e.g.
   rocprofv3 --kernel-trace --stats -d OUT -o kt -- python3 lightmetrica-v3_amd/lm_amd/profile_trace.py --pt
@@ -29,6 +30,8 @@ def main():
     ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
     ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
     ap.add_argument('--c3', type=int, default=0, help='also render the first N config-3 jobs as one batch')
+    ap.add_argument('--guide', action='store_true',
+                    help='also run 20 guide training steps of 2^16 records and 20 inferences of 2M queries')
     ap.add_argument('--lt-bdpt', action='store_true',
                     help='also run 2 LT passes of config 2 and 2 BDPT passes of config 0 at 1080p')
     args = ap.parse_args()
@@ -99,6 +102,36 @@ def main():
         _, _, st = sc.render_raw(spp=4, max_verts=16, rr_depth=16, seed=2, film_sum=fs, guide=g)
         torch.cuda.synchronize()
         print('4 x 1080p config-4 guided PT passes (default guide options, no roulette)', st, flush=True)
+    if args.guide:
+        guide_workload(dev)
+
+
+def guide_workload(dev):
+    """20 Adam steps of the guide MLP on 2^16 random records and 20
+    inferences of 2M queries (the training kernels and the query forward)."""
+    import numpy as np
+    import torch
+    from lm_amd.guide import Guide
+    rng = np.random.default_rng(0)
+    n = 1 << 16
+
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    rec = np.zeros((n, 16), dtype=np.float32)
+    rec[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    rec[:, 3:6], rec[:, 6:9], rec[:, 9:12] = unit(n), unit(n), unit(n)
+    rec[:, 12] = 1.0 / (4 * np.pi)
+    rec[:, 13:16] = rng.uniform(0.1, 1.0, (n, 3))
+    g = Guide(np.array([-1.1] * 3), np.array([1.1] * 3), seed=1)
+    r = torch.from_numpy(rec).to(dev)
+    q = torch.cat([torch.from_numpy(np.ascontiguousarray(rec[:, :9])).to(dev)] * 32)
+    for _ in range(20):
+        g.train(r, lr=1e-3)
+        g.infer(q)
+    torch.cuda.synchronize()
+    print('20 guide training steps of 2^16 records and 20 inferences of 2M queries', flush=True)
 
 
 if __name__ == '__main__':
