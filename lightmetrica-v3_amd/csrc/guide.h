@@ -182,14 +182,6 @@ __device__ __forceinline__ void encode_tail(f3 n, f3 wo, uint16_t* out) {
     t[7] = 0;
 }
 
-__device__ __forceinline__ void encode_input(f3 p, f3 n, f3 wo, const float* box_lo, const float* box_inv,
-                                             uint16_t* out) {
-    float q[3];
-    scene_coords(p, box_lo, box_inv, q);
-    encode_axes<0, 3>(q, out);
-    encode_tail(n, wo, out);
-}
-
 // Half of the encoding for a pair of lanes sharing one row: half 0 writes
 // axes x and y (features 0..47), half 1 axis z, normal, direction and pads
 // (48..79).
