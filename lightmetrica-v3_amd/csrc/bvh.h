@@ -111,7 +111,7 @@ struct RayPre {
     float dkz_abs;   // |d[kz]|, relates the watertight det to the MT det
 };
 
-__device__ __forceinline__ RayPre ray_prepare(f3 o, f3 d) {
+LMGPU_HD RayPre ray_prepare(f3 o, f3 d) {
     RayPre r;
     r.o = o;
     r.d = d;
@@ -141,8 +141,14 @@ __device__ __forceinline__ RayPre ray_prepare(f3 o, f3 d) {
 // Watertight ray/triangle test. On acceptance (t in [tmin, tmax], closed
 // edges, |det_MT| >= 1e-8 as the reference), writes t and the barycentric
 // weights (u, v) of v1 and v2.
-__device__ __forceinline__ bool tri_intersect(const RayPre& r, float4 p0, float4 p1, float4 p2, float tmin,
-                                              float tmax, float& t, float& u, float& v) {
+// Watertightness needs each edge function to be the exact negative of the
+// same edge's function in the neighbouring triangle (a*b - c*d against
+// c*d - a*b). Both products must therefore be rounded before the
+// subtraction: fusing one of them into an FMA would break the antisymmetry
+// and let a ray through a shared edge miss both triangles, so contraction is
+// switched off for the three edge functions.
+LMGPU_HD bool tri_intersect(const RayPre& r, float4 p0, float4 p1, float4 p2, float tmin, float tmax, float& t,
+                            float& u, float& v) {
     const f3 A = mk(p0.x - r.o.x, p0.y - r.o.y, p0.z - r.o.z);
     const f3 B = mk(p1.x - r.o.x, p1.y - r.o.y, p1.z - r.o.z);
     const f3 C = mk(p2.x - r.o.x, p2.y - r.o.y, p2.z - r.o.z);
@@ -153,9 +159,13 @@ __device__ __forceinline__ bool tri_intersect(const RayPre& r, float4 p0, float4
     const float By = fmaf(-r.Sy, Bkz, comp(B, r.ky));
     const float Cx = fmaf(-r.Sx, Ckz, comp(C, r.kx));
     const float Cy = fmaf(-r.Sy, Ckz, comp(C, r.ky));
-    float U = Cx * By - Cy * Bx;
-    float V = Ax * Cy - Ay * Cx;
-    float Wt = Bx * Ay - By * Ax;
+    float U, V, Wt;
+    {
+#pragma clang fp contract(off)
+        U = Cx * By - Cy * Bx;
+        V = Ax * Cy - Ay * Cx;
+        Wt = Bx * Ay - By * Ax;
+    }
     if (U == 0.0f || V == 0.0f || Wt == 0.0f) {
         // Exact edge: recompute the edge functions in FP64 (Woop et al., sec. 3).
         U = float(double(Cx) * double(By) - double(Cy) * double(Bx));

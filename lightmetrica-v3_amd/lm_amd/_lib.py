@@ -228,3 +228,47 @@ def check(rc: int, func: str):
 
 def call(func: str, *args):
     check(getattr(lib(), func)(*args), func)
+
+
+def device_tensor(t, name: str, dtype, cols=None, min_rows: int = 0, shape=None, device=None,
+                  cuda: bool = True) -> int:
+    """Checks a torch tensor handed to a device entry point and returns its
+    data pointer. The kernels behind the C ABI index the buffer by the row
+    count they are given, so a wrong dtype, a short or non-contiguous buffer,
+    or host memory would be read or written out of bounds on the device:
+    every such case raises ValueError here, before any library call.
+    `cols`: required size of the last dimension of a 2-D tensor (None for a
+    1-D tensor); `min_rows`: rows the call touches; `shape`: an exact shape
+    instead; `device`: the torch.device the tensor must live on; `cuda`:
+    False defers the device check to on_device()."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f'{name} must be a torch tensor, got {type(t).__name__}')
+    if t.dtype != dtype:
+        raise ValueError(f'{name} must be {dtype}, got {t.dtype}')
+    if shape is not None:
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f'{name} must have shape {tuple(shape)}, got {tuple(t.shape)}')
+    elif cols is None:
+        if t.dim() != 1 or t.shape[0] < min_rows:
+            raise ValueError(f'{name} must be 1-D with at least {min_rows} entries, got {tuple(t.shape)}')
+    elif t.dim() != 2 or t.shape[1] != cols or t.shape[0] < min_rows:
+        raise ValueError(f'{name} must be [>={min_rows}, {cols}], got {tuple(t.shape)}')
+    if not t.is_contiguous():
+        raise ValueError(f'{name} must be contiguous')
+    if cuda:
+        on_device((name, t))
+    if device is not None and t.device != device:
+        raise ValueError(f'{name} is on {t.device}, expected {device}')
+    return t.data_ptr()
+
+
+def on_device(*named):
+    """(name, tensor) pairs: all CUDA (HIP device) tensors on one device."""
+    dev = None
+    for name, t in named:
+        if not t.is_cuda:
+            raise ValueError(f'{name} must be a CUDA (HIP device) tensor')
+        if dev is not None and t.device != dev:
+            raise ValueError(f'{name} is on {t.device}, expected {dev}')
+        dev = t.device

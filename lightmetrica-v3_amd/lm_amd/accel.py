@@ -105,10 +105,9 @@ class Accel:
         except Exception:
             is_tensor = False
         if is_tensor:
-            if verts.dtype != torch.float32 or not verts.is_contiguous() or not verts.is_cuda:
-                raise ValueError('verts tensor must be a contiguous float32 CUDA tensor')
-            n = verts.numel() // 9
-            _lib.call('lmgpu_accel_build', self._h, ctypes.c_void_p(verts.data_ptr()), n, 1,
+            vp = _lib.device_tensor(verts, 'verts', torch.float32, cols=9)
+            n = verts.shape[0]
+            _lib.call('lmgpu_accel_build', self._h, ctypes.c_void_p(vp), n, 1,
                       ctypes.byref(opts), ctypes.c_void_p(_stream_ptr(stream)))
         else:
             v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
@@ -167,24 +166,43 @@ class Accel:
         return unpack_hits(h)
 
     # -- batched device entry points ----------------------------------------------
-    def intersect_device(self, rays, hits, stream=None):
-        """rays: [n,8] float32 CUDA tensor; hits: [n,4] float32 CUDA tensor
-        (column 0 holds the int32 triangle index bits). Asynchronous."""
+    @staticmethod
+    def _buffers(rays, out, name):
+        """Checks rays ([n,8] float32) and the output buffer (hits [>=n,4]
+        float32 or occl [>=n] uint8), all before any call; returns (n, ray
+        pointer, output pointer)."""
+        import torch
+        rp = _lib.device_tensor(rays, 'rays', torch.float32, cols=RAY_FLOATS, cuda=False)
         n = rays.shape[0]
-        _lib.call('lmgpu_accel_intersect', self._h, ctypes.c_void_p(rays.data_ptr()),
-                  ctypes.c_void_p(hits.data_ptr()), n, ctypes.c_void_p(_stream_ptr(stream)))
+        if name == 'occl':
+            op = _lib.device_tensor(out, 'occl', torch.uint8, min_rows=n, cuda=False)
+        else:
+            op = _lib.device_tensor(out, 'hits', torch.float32, cols=HIT_FLOATS, min_rows=n, cuda=False)
+        _lib.on_device(('rays', rays), (name, out))
+        return n, rp, op
+
+    def intersect_device(self, rays, hits, stream=None):
+        """rays: [n,8] float32 CUDA tensor; hits: [>=n,4] float32 CUDA tensor
+        (column 0 holds the int32 triangle index bits). Asynchronous. Shapes,
+        dtypes, contiguity and the device are checked before the call."""
+        n, rp, hp = self._buffers(rays, hits, 'hits')
+        _lib.call('lmgpu_accel_intersect', self._h, ctypes.c_void_p(rp), ctypes.c_void_p(hp), n,
+                  ctypes.c_void_p(_stream_ptr(stream)))
 
     def occluded_device(self, rays, occl, stream=None):
-        n = rays.shape[0]
-        _lib.call('lmgpu_accel_occluded', self._h, ctypes.c_void_p(rays.data_ptr()),
-                  ctypes.c_void_p(occl.data_ptr()), n, ctypes.c_void_p(_stream_ptr(stream)))
+        """rays: [n,8] float32 CUDA tensor; occl: [>=n] uint8 CUDA tensor."""
+        n, rp, op = self._buffers(rays, occl, 'occl')
+        _lib.call('lmgpu_accel_occluded', self._h, ctypes.c_void_p(rp), ctypes.c_void_p(op), n,
+                  ctypes.c_void_p(_stream_ptr(stream)))
 
     def intersect_counted(self, rays, hits, stream=None):
+        """As intersect_device; returns (node visits, triangle tests) summed
+        over the rays. Synchronous."""
+        n, rp, hp = self._buffers(rays, hits, 'hits')
         nv = ctypes.c_int64(0)
         nt = ctypes.c_int64(0)
-        _lib.call('lmgpu_accel_intersect_counted', self._h, ctypes.c_void_p(rays.data_ptr()),
-                  ctypes.c_void_p(hits.data_ptr()), rays.shape[0], ctypes.byref(nv), ctypes.byref(nt),
-                  ctypes.c_void_p(_stream_ptr(stream)))
+        _lib.call('lmgpu_accel_intersect_counted', self._h, ctypes.c_void_p(rp), ctypes.c_void_p(hp), n,
+                  ctypes.byref(nv), ctypes.byref(nt), ctypes.c_void_p(_stream_ptr(stream)))
         return nv.value, nt.value
 
     def status(self, stream=None):

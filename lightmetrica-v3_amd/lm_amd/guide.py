@@ -66,10 +66,12 @@ class Guide:
     def infer(self, query, mix=None, stream=None):
         """query: [n,9] float32 CUDA tensor (pos, nrm, wo) -> mix [n,40]."""
         import torch
+        qp = _lib.device_tensor(query, 'query', torch.float32, cols=9)
         n = query.shape[0]
         if mix is None:
             mix = torch.empty((n, MIX), dtype=torch.float32, device=query.device)
-        _lib.call('lmgpu_guide_infer', self._h, ctypes.c_void_p(query.data_ptr()), ctypes.c_void_p(mix.data_ptr()),
+        mp = _lib.device_tensor(mix, 'mix', torch.float32, cols=MIX, min_rows=n, device=query.device)
+        _lib.call('lmgpu_guide_infer', self._h, ctypes.c_void_p(qp), ctypes.c_void_p(mp),
                   n, ctypes.c_void_p(_stream_ptr(stream)))
         return mix
 
@@ -82,9 +84,11 @@ class Guide:
 
     def train(self, records, lr: float = 1e-3, want_grad: bool = False, stream=None):
         """records: [n,16] float32 CUDA tensor. Returns (loss, grad or None)."""
+        import torch
+        rp = _lib.device_tensor(records, 'records', torch.float32, cols=RECORD)
         loss = ctypes.c_float(0.0)
         g = np.zeros(NPARAMS, dtype=np.float32) if want_grad else None
-        _lib.call('lmgpu_guide_train', self._h, ctypes.c_void_p(records.data_ptr()), records.shape[0],
+        _lib.call('lmgpu_guide_train', self._h, ctypes.c_void_p(rp), records.shape[0],
                   float(lr), ctypes.byref(loss), g.ctypes.data_as(ctypes.c_void_p) if g is not None else None,
                   ctypes.c_void_p(_stream_ptr(stream)))
         return loss.value, g
@@ -92,10 +96,12 @@ class Guide:
 
 def sample(mix, u, stream=None):
     import torch
+    mp = _lib.device_tensor(mix, 'mix', torch.float32, cols=MIX)
     n = mix.shape[0]
+    up = _lib.device_tensor(u, 'u', torch.float32, cols=3, min_rows=n, device=mix.device)
     dirs = torch.empty((n, 3), dtype=torch.float32, device=mix.device)
     pdf = torch.empty((n,), dtype=torch.float32, device=mix.device)
-    _lib.call('lmgpu_guide_sample', ctypes.c_void_p(mix.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+    _lib.call('lmgpu_guide_sample', ctypes.c_void_p(mp), ctypes.c_void_p(up),
               ctypes.c_void_p(dirs.data_ptr()), ctypes.c_void_p(pdf.data_ptr()), n,
               ctypes.c_void_p(_stream_ptr(stream)))
     return dirs, pdf
@@ -103,8 +109,10 @@ def sample(mix, u, stream=None):
 
 def pdf(mix, dirs, stream=None):
     import torch
+    mp = _lib.device_tensor(mix, 'mix', torch.float32, cols=MIX)
     n = mix.shape[0]
+    dp = _lib.device_tensor(dirs, 'dirs', torch.float32, cols=3, min_rows=n, device=mix.device)
     out = torch.empty((n,), dtype=torch.float32, device=mix.device)
-    _lib.call('lmgpu_guide_pdf', ctypes.c_void_p(mix.data_ptr()), ctypes.c_void_p(dirs.data_ptr()),
+    _lib.call('lmgpu_guide_pdf', ctypes.c_void_p(mp), ctypes.c_void_p(dp),
               ctypes.c_void_p(out.data_ptr()), n, ctypes.c_void_p(_stream_ptr(stream)))
     return out

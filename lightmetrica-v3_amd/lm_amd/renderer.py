@@ -159,6 +159,12 @@ class GpuScene:
         """Scene bounding box (lo, hi) of the triangle vertices."""
         return self._lo.copy(), self._hi.copy()
 
+    def _film(self, t, name, device=None):
+        """Device pointer of a [h, w, 3] float64 contiguous CUDA film tensor
+        of this scene's size (ValueError otherwise, before any call)."""
+        import torch
+        return _lib.device_tensor(t, name, torch.float64, shape=(self.h, self.w, 3), device=device)
+
     def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: Optional[int] = None,
                    seed: int = 0, sample_offset: int = 0, render_time: float = 0.0,
                    film_sum=None, film_sumsq=None, stream=None, guide=None, guide_opts: Optional[dict] = None):
@@ -194,8 +200,10 @@ class GpuScene:
             fn, pre = 'lmgpu_pt_render', (self._h, ctypes.byref(p))
         on_device = film_sum is not None and hasattr(film_sum, 'data_ptr')
         if on_device:
-            sq = ctypes.c_void_p(film_sumsq.data_ptr()) if film_sumsq is not None else None
-            _lib.call(fn, *pre, ctypes.c_void_p(film_sum.data_ptr()), sq, 1, ctypes.byref(st),
+            sp = self._film(film_sum, 'film_sum')
+            sq = ctypes.c_void_p(self._film(film_sumsq, 'film_sumsq', film_sum.device)) \
+                if film_sumsq is not None else None
+            _lib.call(fn, *pre, ctypes.c_void_p(sp), sq, 1, ctypes.byref(st),
                       ctypes.c_void_p(_stream_ptr(stream)))
             s, q = film_sum, film_sumsq
         else:
@@ -221,7 +229,7 @@ class GpuScene:
         p.render_time = float(render_time)
         st = _lib.PTStats()
         if film is not None and hasattr(film, 'data_ptr'):
-            _lib.call('lmgpu_lt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()), 1,
+            _lib.call('lmgpu_lt_render', self._h, ctypes.byref(p), ctypes.c_void_p(self._film(film, 'film')), 1,
                       ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
             s = film
         else:
@@ -245,7 +253,7 @@ class GpuScene:
         p.min_verts = int(min_verts)
         st = _lib.PTStats()
         if film is not None and hasattr(film, 'data_ptr'):
-            _lib.call('lmgpu_bdpt_render', self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()), 1,
+            _lib.call('lmgpu_bdpt_render', self._h, ctypes.byref(p), ctypes.c_void_p(self._film(film, 'film')), 1,
                       ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
             s = film
         else:

@@ -382,3 +382,136 @@ def test_area_light_irradiance_closed_form(mode):
     E = _polygon_irradiance(np.zeros(3), np.array([0, 1.0, 0]), poly, Le)
     expected = rho / math.pi * E
     assert abs(mean - expected) < 5 * se + 1e-9, (mean, expected, se)
+
+
+# ---- closed-form pins of material::glossy and material::glass -----------------------
+#
+# material::glossy (src/material/material_glossy.cpp:43-58,102-109) is a GGX
+# microfacet BRDF f = Ks * F(wo.h) * D(h) * G1(wi) G1(wo) / (4 cos_i cos_o)
+# with Schlick F = Ks + (1 - Ks)(1 - wo.h)^5, i.e. Ks enters twice. The
+# checks below write D and G1 in the spherical-angle form of Walter et al.
+# (EGSR 2007, eq. 33-34) and Heitz (JCGT 3(2), 2014, eq. 85-86) for the
+# anisotropic case, not in the oracle's dot-product form. With N = (0,0,1) the
+# reference's two-sided basis (include/lm/math.h:539-546, surface.h:192-195)
+# is u = x, v = y, so ax belongs to the x axis.
+
+def _ggx_walter(ks, ax, ay, wi, wo):
+    """GGX BRDF and projected-solid-angle pdf from angles (N = z)."""
+    ks = np.asarray(ks, dtype=np.float64)
+    h = (wi + wo) / np.linalg.norm(wi + wo)
+    th = math.acos(h[2])
+    ph = math.atan2(h[1], h[0])
+    t2 = math.tan(th) ** 2
+    D = 1.0 / (math.pi * ax * ay * math.cos(th) ** 4 *
+               (1.0 + t2 * (math.cos(ph) ** 2 / ax ** 2 + math.sin(ph) ** 2 / ay ** 2)) ** 2)
+
+    def G1(w):
+        tw = math.acos(w[2])
+        pw = math.atan2(w[1], w[0])
+        a2 = math.cos(pw) ** 2 * ax ** 2 + math.sin(pw) ** 2 * ay ** 2
+        return 2.0 / (1.0 + math.sqrt(1.0 + a2 * math.tan(tw) ** 2))
+
+    F = ks + (1 - ks) * (1 - float(np.dot(wo, h))) ** 5
+    f = ks * F * D * G1(wi) * G1(wo) / (4 * wi[2] * wo[2])
+    pdf = D * h[2] / (4 * float(np.dot(wo, h)) * wo[2])
+    return f, pdf
+
+
+def test_glossy_normal_incidence_closed_form():
+    """wi = wo = n: h = n, D = 1/(pi ax ay), G1(n) = 1, F = Ks, so
+    f = Ks^2 / (4 pi ax ay) per channel (a single Ks would give Ks / ...)."""
+    ks, ax, ay = np.array([0.9, 0.6, 0.3]), 0.3, 0.2
+    f, pdf = orc.mat_eval(scenes.MAT_GLOSSY, orc._mat_params(ks, ax, ay), N, N[None], N[None])
+    assert np.allclose(f[0], ks ** 2 / (4 * math.pi * ax * ay), rtol=1e-12)
+    assert abs(pdf[0] - 1 / (4 * math.pi * ax * ay)) < 1e-12 * pdf[0]
+
+
+@pytest.mark.parametrize('theta_deg,alpha', [(50.0, 0.25), (80.0, 0.05), (20.0, 0.5)])
+def test_glossy_specular_direction_isotropic(theta_deg, alpha):
+    """Mirror configuration (h = n) at an oblique angle, isotropic alpha:
+    f = Ks (Ks + (1-Ks)(1-cos)^5) G1^2 / (4 pi alpha^2 cos^2) with the Smith
+    G1 = 2 / (1 + sqrt(1 + alpha^2 tan^2)) of Walter et al.; the max(Eps, sin)
+    clamp of material_glossy.cpp:52 does not bite at these angles."""
+    th = math.radians(theta_deg)
+    wi = np.array([math.sin(th), 0.0, math.cos(th)])
+    wo = np.array([-math.sin(th), 0.0, math.cos(th)])
+    ks = np.array([0.8, 0.5, 0.2])
+    g1 = 2 / (1 + math.sqrt(1 + alpha ** 2 * math.tan(th) ** 2))
+    fr = ks + (1 - ks) * (1 - math.cos(th)) ** 5
+    exp = ks * fr * g1 * g1 / (4 * math.pi * alpha ** 2 * math.cos(th) ** 2)
+    f, pdf = orc.mat_eval(scenes.MAT_GLOSSY, orc._mat_params(ks, alpha, alpha), N, wi[None], wo[None])
+    assert np.allclose(f[0], exp, rtol=1e-10), (f[0], exp)
+    assert abs(pdf[0] - 1 / (4 * math.pi * alpha ** 2 * math.cos(th) ** 2)) < 1e-10 * pdf[0]
+
+
+def test_glossy_anisotropic_off_specular_closed_form():
+    """Random off-specular pairs against the spherical-angle anisotropic GGX
+    (Heitz 2014), which pins the axis each roughness belongs to (swapping ax
+    and ay changes f by up to 3x here) and the uncorrelated Smith product."""
+    rng = np.random.default_rng(11)
+    ks, ax, ay = (0.7, 0.4, 0.9), 0.15, 0.45
+    for _ in range(200):
+        a = rng.normal(size=3); a[2] = abs(a[2]) + 0.05; a /= np.linalg.norm(a)
+        b = rng.normal(size=3); b[2] = abs(b[2]) + 0.05; b /= np.linalg.norm(b)
+        f, pdf = orc.mat_eval(scenes.MAT_GLOSSY, orc._mat_params(ks, ax, ay), N, a[None], b[None])
+        ef, ep = _ggx_walter(ks, ax, ay, a, b)
+        assert np.allclose(f[0], ef, rtol=1e-9), (a, b, f[0], ef)
+        assert abs(pdf[0] - ep) <= 1e-9 * ep
+    # the roughness axes matter: swapped ax / ay gives another value
+    a = np.array([0.6, 0.1, 0.79]); a /= np.linalg.norm(a)
+    b = np.array([-0.2, 0.5, 0.84]); b /= np.linalg.norm(b)
+    f1, _ = orc.mat_eval(scenes.MAT_GLOSSY, orc._mat_params(ks, ax, ay), N, a[None], b[None])
+    f2, _ = _ggx_walter(ks, ay, ax, a, b)
+    assert not np.allclose(f1[0], f2, rtol=1e-3)
+
+
+def _glass_reflect_fraction_brackets(wi, fr, ni=1.5):
+    """material::glass picks reflection when uc < Fr (material_glass.cpp:
+    124-141): components drawn just below and just above the expected Fr."""
+    us = np.array([[fr * (1 - 1e-9), 0, 0, 0], [min(fr * (1 + 1e-9), 1 - 1e-12), 0, 0, 0]])
+    wo, w, comp, ok = orc.mat_sample(scenes.MAT_GLASS, orc._mat_params(ni=ni), N, np.repeat(wi[None], 2, 0), us)
+    return comp, wo, w
+
+
+def test_glass_fresnel_entering_oblique():
+    """Entering at 60 deg: Schlick with cos = wi.n = 0.5, r0 = 0.04:
+    Fr = 0.04 + 0.96 * 0.5^5 = 0.07."""
+    th = math.radians(60.0)
+    wi = np.array([math.sin(th), 0.0, math.cos(th)])
+    comp, wo, w = _glass_reflect_fraction_brackets(wi, 0.07)
+    assert comp.tolist() == [0, 1]
+
+
+def test_glass_fresnel_leaving_uses_transmitted_cosine():
+    """Leaving the glass (wi.n < 0) at 30 deg inside: Snell gives
+    sin(theta_t) = 1.5 * 0.5 = 0.75 outside, and the reference's Schlick uses
+    cos = wt.n = sqrt(1 - 0.5625) (material_glass.cpp:90-98):
+    Fr = 0.04 + 0.96 (1 - 0.661437828)^5 = 0.044270... Using |wi.n| = cos 30
+    instead would give 0.040041, which the brackets exclude."""
+    th = math.radians(30.0)
+    wi = np.array([math.sin(th), 0.0, -math.cos(th)])
+    cos_t = math.sqrt(1 - (1.5 * math.sin(th)) ** 2)
+    fr = 0.04 + 0.96 * (1 - cos_t) ** 5
+    assert abs(fr - 0.0442704) < 1e-6
+    comp, wo, w = _glass_reflect_fraction_brackets(wi, fr)
+    assert comp.tolist() == [0, 1]
+    wrong = 0.04 + 0.96 * (1 - math.cos(th)) ** 5
+    comp_w, _, _ = _glass_reflect_fraction_brackets(wi, wrong)
+    assert comp_w.tolist() == [0, 0]           # both draws lie below the true Fr: reflection
+    # the refracted direction leaves into the upper side with sin = 0.75, and
+    # the refraction weight is eta^2 = Ni^2 for importance transport
+    t = wo[1]
+    assert t[2] > 0 and abs(math.hypot(t[0], t[1]) - 0.75) < 1e-12
+    assert np.allclose(w[1], 1.5 ** 2)
+
+
+def test_glass_total_internal_reflection():
+    """Leaving at 60 deg inside: 1.5 sin 60 > 1, total internal reflection,
+    Fr = 1: every component draw reflects, with weight 1 about the normal."""
+    th = math.radians(60.0)
+    wi = np.array([math.sin(th), 0.0, -math.cos(th)])
+    us = np.array([[0.0, 0, 0, 0], [0.5, 0, 0, 0], [1 - 1e-12, 0, 0, 0]])
+    wo, w, comp, ok = orc.mat_sample(scenes.MAT_GLASS, orc._mat_params(ni=1.5), N, np.repeat(wi[None], 3, 0), us)
+    assert comp.tolist() == [0, 0, 0] and ok.all()
+    assert np.allclose(wo, [[-math.sin(th), 0.0, -math.cos(th)]] * 3)
+    assert np.allclose(w, 1.0)
