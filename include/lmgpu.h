@@ -351,8 +351,9 @@ int lmgpu_batch_job_film(const lmgpu_batch* batch, int32_t job, double* film_sum
  * No counterpart in the reference: an MLP maps (position, shading normal,
  * outgoing direction) to a mixture of 8 von Mises-Fisher lobes over incident
  * directions; it is trained online on radiance recorded by the path tracer
- * and mixed 50/50 with BSDF sampling by one-sample MIS (lmgpu_pt_render_guided),
- * so the rendered estimate stays unbiased for any network state. */
+ * and mixed with BSDF sampling by one-sample MIS (lmgpu_pt_render_guided: the
+ * guide is chosen with probability alpha in [0, 1], default 0.65; 0.5 is an
+ * even split), so the rendered estimate stays unbiased for any network state. */
 #define LMGPU_GUIDE_NPARAMS 17664   /* packed FP32 parameter slots (15976 trained) */
 #define LMGPU_GUIDE_MIX 40          /* decoded mixture: pi[8], kappa[8], mu[8][3] */
 #define LMGPU_GUIDE_RECORD 16       /* training record floats: pos3 nrm3 wo3 wi3 pdf L3 */

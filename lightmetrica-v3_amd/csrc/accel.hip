@@ -615,6 +615,17 @@ static void launch_trace(const Accel& a, const lmgpu_ray* rays, lmgpu_hit* hits,
 
 int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
                  unsigned long long* counters, hipStream_t s) const {
+    LMGPU_TRY(check_trace_args(rays, hits, occl, n, any));
+    if (n == 0) return LMGPU_OK;
+    int32_t* st = status_dev();
+    if (!st) return fail(LMGPU_ERR_OUT_OF_MEMORY, "status buffer");
+    unsigned long long* next = next_counter();
+    if (!next) return fail(LMGPU_ERR_OUT_OF_MEMORY, "ray counter");
+    return trace_with(rays, hits, occl, n, any, counters, next, st, s);
+}
+
+int Accel::check_trace_args(const lmgpu_ray* rays, const lmgpu_hit* hits, const uint8_t* occl, int64_t n,
+                            bool any) const {
     if (!built) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel is not built");
     if (n < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "negative ray count");
     if (n == 0) return LMGPU_OK;
@@ -622,10 +633,13 @@ int Accel::trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t 
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "NULL ray or result buffer");
     if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15)))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "rays and hits must be 16-byte aligned");
-    int32_t* st = status_dev();
-    if (!st) return fail(LMGPU_ERR_OUT_OF_MEMORY, "status buffer");
-    unsigned long long* next = next_counter();
-    if (!next) return fail(LMGPU_ERR_OUT_OF_MEMORY, "ray counter");
+    return LMGPU_OK;
+}
+
+int Accel::trace_with(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
+                      unsigned long long* counters, unsigned long long* next, int32_t* st, hipStream_t s) const {
+    LMGPU_TRY(check_trace_args(rays, hits, occl, n, any));
+    if (n == 0) return LMGPU_OK;
 #define LMGPU_DISPATCH(WW)                                                                          \
     if (width == WW) {                                                                              \
         if (any) launch_trace<WW, true, false>(*this, rays, hits, occl, n, next, st, counters, s);  \

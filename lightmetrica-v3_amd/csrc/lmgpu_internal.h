@@ -128,6 +128,13 @@ struct Accel {
     int build(const float* verts, int64_t n, bool on_device, const lmgpu_build_opts* opts, hipStream_t s);
     int trace(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
               unsigned long long* counters, hipStream_t s) const;
+    // As trace, with the caller's ray counter (zeroed by the launch itself)
+    // and overflow status word (bit 0 set on overflow, never cleared here)
+    // instead of the accel's own, so nothing is allocated.
+    int trace_with(const lmgpu_ray* rays, lmgpu_hit* hits, uint8_t* occl, int64_t n, bool any,
+                   unsigned long long* counters, unsigned long long* next, int32_t* status, hipStream_t s) const;
+    int check_trace_args(const lmgpu_ray* rays, const lmgpu_hit* hits, const uint8_t* occl, int64_t n,
+                         bool any) const;
     // Synchronizes and reports a traversal stack overflow since the last check.
     int check_status(hipStream_t s) const;
 };

@@ -105,3 +105,40 @@ def test_batch_errors(device):
         b.run(width=4, path_capacity=100)
     with pytest.raises(LmError):
         b.add_job(flat, spp=0)
+
+
+def test_c3_jobs_welch_vs_oracle(device):
+    """Config-3 jobs through the batched renderer against the FP64 oracle of
+    renderer::pt (src/renderer/renderer_pt.cpp:80-315), independent seeds:
+    the largest job of the 64 (about 2M triangles) and two smaller ones, at
+    64x64 and 64 spp with 16 vertices, rendered in one batch; every job's
+    per-pixel Welch fail fraction (alpha 0.01) stays within the binomial
+    bound and its mean radiance within 4 standard errors of the oracle's."""
+    from lm_amd import stats
+    from oracle import oracle as orc
+    jobs = scenes.config3_jobs()
+    order = sorted(range(len(jobs)), key=lambda j: jobs[j]['tris'])
+    pick = [order[-1], order[len(order) // 2], order[0]]
+    spp, M = 64, 16
+    b = Batch()
+    flats = []
+    for j in pick:
+        jj = dict(jobs[j], res=64)
+        flat = scenes.config3_scene(jj).flatten()
+        flats.append(flat)
+        b.add_job(flat, spp=spp, max_verts=M, seed=jobs[j]['seed'])
+    st = b.run(width=4, path_capacity=1 << 20, block=256, sumsq=True)
+    print('batch', st)
+    for k, (j, flat) in enumerate(zip(pick, flats)):
+        s, q, _ = b.film(k)
+        ref = orc.pt_render(flat, spp=spp, max_verts=M, sampling_mode=2, seed=jobs[j]['seed'] ^ 0x5A5A)
+        frac, _, _ = stats.welch(s, q, spp, ref['sum'], ref['sumsq'], spp)
+        bound = stats.binomial_bound(0.01, flat.w * flat.h)
+        n = flat.w * flat.h * spp
+        mg, mo = s.sum() / n, ref['sum'].sum() / n
+        se = np.sqrt((q.sum() / n - (s.sum(axis=(0, 1)) / n) ** 2).sum() / n +
+                     (ref['sumsq'].sum() / n - (ref['sum'].sum(axis=(0, 1)) / n) ** 2).sum() / n)
+        print(f'job {j}: {jobs[j]["tris"]} tris (flattened {flat.verts.shape[0]}), Welch fail {frac:.4f} '
+              f'(bound {bound:.4f}), mean {mg:.5f} vs oracle {mo:.5f}')
+        assert frac <= bound
+        assert abs(mg - mo) <= 4 * se + 1e-12
