@@ -80,8 +80,19 @@ typedef struct lmgpu_build_opts {
                             64 lanes are idle, 1..64 (0 = 32; 64 suits coherent rays) */
     int32_t builder;     /* LMGPU_BUILDER_LBVH (1) or LMGPU_BUILDER_PLOC (0); with opts == NULL: LBVH */
     int32_t ploc_radius; /* PLOC nearest-neighbour window, 1..64 (0 = 16) */
-    int32_t reserved[3];
+    int32_t collapse;    /* LMGPU_COLLAPSE_SAH (0) or LMGPU_COLLAPSE_GREEDY (1) */
+    float sah_tri_cost;  /* SAH collapse: cost of a triangle test relative to a wide-node visit (0 = default) */
+    int32_t reserved[1];
 } lmgpu_build_opts;
+
+/* Binary tree -> W-wide tree. SAH: the collapse of least SAH cost over all
+ * ways of cutting the binary tree into W-wide nodes and leaves of at most
+ * leaf_size triangles (a bottom-up dynamic program, Ylitie, Karras and Laine,
+ * HPG 2017), which also decides where leaves go. GREEDY: each wide node opens
+ * its largest-area child until it has W children; every subtree of at most
+ * leaf_size triangles is a leaf. */
+#define LMGPU_COLLAPSE_SAH 0
+#define LMGPU_COLLAPSE_GREEDY 1
 
 #define LMGPU_BUILDER_PLOC 0  /* locally-ordered clustering over Morton order */
 #define LMGPU_BUILDER_LBVH 1  /* Karras radix tree over Morton order */

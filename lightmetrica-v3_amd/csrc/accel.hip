@@ -5,13 +5,16 @@
 // pipeline:
 //   1. per-triangle bounds + centroid bounds (ordered-int atomics),
 //   2. 63-bit Morton codes of the centroids,
-//   3. radix sort of (code, index) pairs (rocPRIM onesweep),
-//   4. binary LBVH topology (Karras, HPG 2012), ties broken by index,
-//   5. bottom-up bounds refit (second-arriving thread merges),
-//   6. collapse into a W-wide tree (W = 2, 4, 8): each wide node repeatedly
-//      opens its largest-area child until it has W children; subtrees of at
-//      most `leaf_size` triangles become leaves over a contiguous range of the
-//      Morton-sorted triangle array,
+//   3. stable radix sort of (code, index) pairs (sort.hip), checked ordered,
+//   4. binary LBVH topology (Karras, HPG 2012), ties broken by index, or
+//      PLOC (ploc.hip),
+//   5. bottom-up bounds refit, level-synchronous over a frontier, which also
+//      fills the SAH cost tables of the collapse (sah_tables),
+//   6. collapse into a W-wide tree (W = 2, 4, 8) of least SAH cost: the
+//      dynamic program decides which binary subtrees become wide nodes and
+//      which become leaves of at most `leaf_size` triangles (a contiguous
+//      range of the sorted triangle array); the greedy alternative opens the
+//      largest-area child until a node has W children,
 //   7. triangles copied into traversal order (48 B each).
 // and Accel_SAHBVH::intersect (:269-299) with the ordered wide traversal of
 // bvh.h.
@@ -24,13 +27,16 @@
 #include <string>
 #include <vector>
 
-#include <rocprim/rocprim.hpp>
-
 #include "build.h"
 #include "bvh.h"
 #include "lmgpu_internal.h"
 
 namespace lmgpu {
+
+// Build defaults: leaves of at most 2 triangles, and a triangle test costing
+// as much as a wide-node visit in the SAH collapse.
+constexpr int kDefaultLeafSize = 2;
+constexpr float kDefaultSahTriCost = 1.0f;
 
 // ---------------------------------------------------------------------------
 // build kernels
@@ -131,6 +137,13 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tlo, 
 }
 
 
+// flag = 1 if keys[i] > keys[i + 1] for some i
+__global__ __launch_bounds__(256) void k_check_sorted(const uint64_t* __restrict__ keys, int64_t n,
+                                                      int32_t* __restrict__ flag) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i + 1 < n && keys[i] > keys[i + 1]) *flag = 1;
+}
+
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
     const uint64_t a = k[i], b = k[j];
@@ -202,13 +215,89 @@ __global__ __launch_bounds__(256) void k_refit_leaves(int64_t n, const int32_t* 
     if (atomicAdd(&flags[p], 1) == 1) frontier[atomicAdd(fcount, 1)] = p;
 }
 
+__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Parent links of a binary tree given by its nodes (parent_int[root] stays
+// as initialised).
+__global__ __launch_bounds__(256) void k_parents(const BNode* __restrict__ bn, int64_t m,
+                                                 int32_t* __restrict__ parent_int, int32_t* __restrict__ parent_leaf) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const BNode nd = bn[p];
+    if (nd.left < 0) parent_leaf[~nd.left] = int32_t(p); else parent_int[nd.left] = int32_t(p);
+    if (nd.right < 0) parent_leaf[~nd.right] = int32_t(p); else parent_int[nd.right] = int32_t(p);
+}
+
+// Cost tables of the SAH collapse (Ylitie, Karras and Laine, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017,
+// sec. 4): for internal binary node p and i = 1..W, C(p, i) is the least SAH
+// cost (half areas; a wide-node visit costs 1, a triangle test `ctri`) of
+// covering p's subtree with at most i roots, each a wide node or a leaf.
+//   C(p, 1) = min(A(p) * N(p) * ctri  [leaf, N(p) <= pmax],
+//                 A(p) + D(p, W)       [wide node]),
+//   C(p, i) = min(C(p, i - 1), D(p, i)),   D(p, j) = min_k C(L, k) + C(R, j - k),
+// and a single triangle j costs A(j) * ctri for every i. choice[p*W + 0] is
+// 0xFF for a leaf, else the k of D(p, W); choice[p*W + i - 1] (i >= 2) is 0
+// when C(p, i) = C(p, i - 1), else the k of D(p, i).
+struct SahDP {
+    float* cost;
+    uint8_t* choice;
+    int W;
+    int pmax;
+    float ctri;
+};
+
+__device__ __forceinline__ void sah_tables(const SahDP& dp, int32_t p, const BNode& nd, float area,
+                                           const float4& l0, const float4& h0, const float4& l1, const float4& h1) {
+    const int W = dp.W;
+    float cl[8], cr[8];
+    const float al = half_area(l0, h0) * dp.ctri, ar = half_area(l1, h1) * dp.ctri;
+    for (int i = 0; i < W; i++) {
+        cl[i] = nd.left < 0 ? al : dp.cost[int64_t(nd.left) * W + i];
+        cr[i] = nd.right < 0 ? ar : dp.cost[int64_t(nd.right) * W + i];
+    }
+    float dist[8];
+    int arg[8];
+    for (int j = 2; j <= W; j++) {
+        float best = __builtin_inff();
+        int bk = 1;
+        for (int k = 1; k < j; k++) {
+            const float c = cl[k - 1] + cr[j - k - 1];
+            if (c < best) { best = c; bk = k; }
+        }
+        dist[j - 1] = best;
+        arg[j - 1] = bk;
+    }
+    const int cnt = nd.last - nd.first + 1;
+    const float cint = area + dist[W - 1];
+    const float cprim = cnt <= dp.pmax ? area * float(cnt) * dp.ctri : __builtin_inff();
+    float c = fminf(cprim, cint);
+    float* out = dp.cost + int64_t(p) * W;
+    uint8_t* ch = dp.choice + int64_t(p) * W;
+    out[0] = c;
+    ch[0] = cprim <= cint ? uint8_t(0xFF) : uint8_t(arg[W - 1]);
+    for (int i = 2; i <= W; i++) {
+        if (c <= dist[i - 1]) {
+            ch[i - 1] = 0;
+        } else {
+            c = dist[i - 1];
+            ch[i - 1] = uint8_t(arg[i - 1]);
+        }
+        out[i - 1] = c;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_refit_nodes(const int32_t* __restrict__ fin, const int32_t* __restrict__ fin_count,
-                                                     const BNode* __restrict__ bn,
+                                                     const BNode* __restrict__ bn, int32_t root,
                                                      const int32_t* __restrict__ parent_int,
                                                      const float4* __restrict__ leaf_lo,
                                                      const float4* __restrict__ leaf_hi, float4* __restrict__ node_lo,
                                                      float4* __restrict__ node_hi, int32_t* __restrict__ flags,
-                                                     int32_t* __restrict__ fout, int32_t* __restrict__ fout_count) {
+                                                     int32_t* __restrict__ fout, int32_t* __restrict__ fout_count,
+                                                     SahDP dp) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= *fin_count) return;
     const int32_t p = fin[idx];
@@ -217,9 +306,12 @@ __global__ __launch_bounds__(256) void k_refit_nodes(const int32_t* __restrict__
     const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
     const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
     const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
-    node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
-    if (p == 0) return;
+    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    node_lo[p] = lo;
+    node_hi[p] = hi;
+    if (dp.cost) sah_tables(dp, p, nd, half_area(lo, hi), l0, h0, l1, h1);
+    if (p == root) return;
     const int32_t q = parent_int[p];
     if (atomicAdd(&flags[q], 1) == 1) fout[atomicAdd(fout_count, 1)] = q;
 }
@@ -237,18 +329,116 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t n, const int32_t* __res
     tris[j] = t;
 }
 
-__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
-    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
-    return dx * dy + dy * dz + dz * dx;
-}
-
 struct CollapseItem {
     int32_t bref;   // binary ref (internal index or ~leaf)
     int32_t wnode;  // wide node slot
 };
 
-// One work item per wide node: open children until W, then write the node
-// and emit the internal children as next-level items.
+// Writes wide node `it.wnode` from its nc <= W binary refs (each a leaf when
+// is_leaf(ref), else a wide child). The wide children of one node take
+// consecutive slots, so siblings share cache lines; they become next-level
+// items.
+template <int W, class IsLeaf>
+__device__ __forceinline__ void emit_wide_node(const CollapseItem& it, const int32_t (&ref)[W], int nc, IsLeaf is_leaf,
+                                               const BNode* __restrict__ bn, const float4* __restrict__ node_lo,
+                                               const float4* __restrict__ node_hi, const float4* __restrict__ leaf_lo,
+                                               const float4* __restrict__ leaf_hi, CollapseItem* __restrict__ out,
+                                               int32_t* __restrict__ n_out, int32_t* __restrict__ node_counter,
+                                               WNode<W>* __restrict__ wnodes, double* sah_acc, int32_t* leaf_counter) {
+    int nint = 0;
+    for (int c = 0; c < nc; c++) nint += is_leaf(ref[c]) ? 0 : 1;
+    int32_t slot = 0, q = 0;
+    if (nint) {
+        slot = atomicAdd(node_counter, nint);
+        q = atomicAdd(n_out, nint);
+    }
+    WNode<W> node;
+    double sah = 0.0;
+    int nleaf = 0;
+    for (int c = 0; c < W; c++) {
+        if (c < nc) {
+            const int32_t r = ref[c];
+            const float4 lo = r < 0 ? leaf_lo[~r] : node_lo[r];
+            const float4 hi = r < 0 ? leaf_hi[~r] : node_hi[r];
+            node.lox[c] = lo.x; node.loy[c] = lo.y; node.loz[c] = lo.z;
+            node.hix[c] = hi.x; node.hiy[c] = hi.y; node.hiz[c] = hi.z;
+            const int cnt = r < 0 ? 1 : bn[r].last - bn[r].first + 1;
+            const double a = double(half_area(lo, hi));
+            if (is_leaf(r)) {
+                node.child[c] = r < 0 ? ~r : bn[r].first;
+                node.count[c] = cnt;
+                sah += a * cnt;
+                nleaf++;
+            } else {
+                node.child[c] = slot;
+                node.count[c] = 0;
+                sah += a;
+                out[q++] = CollapseItem{r, slot++};
+            }
+        } else {
+            node.lox[c] = node.loy[c] = node.loz[c] = 1e30f;
+            node.hix[c] = node.hiy[c] = node.hiz[c] = -1e30f;
+            node.child[c] = 0;
+            node.count[c] = -1;
+        }
+    }
+    wnodes[it.wnode] = node;
+    atomicAdd(sah_acc, sah);
+    atomicAdd(leaf_counter, nleaf);
+}
+
+// SAH collapse: one work item per wide node; its children are the roots the
+// cost tables chose for D(p, W) (sah_tables), found by unfolding the choices
+// with a stack of (ref, slots) pairs whose slots sum to at most W.
+template <int W>
+__global__ __launch_bounds__(256) void k_collapse_sah(const CollapseItem* __restrict__ in,
+                                                      const int32_t* __restrict__ n_in, CollapseItem* __restrict__ out,
+                                                      int32_t* __restrict__ n_out, int32_t* __restrict__ node_counter,
+                                                      const BNode* __restrict__ bn, const float4* __restrict__ node_lo,
+                                                      const float4* __restrict__ node_hi,
+                                                      const float4* __restrict__ leaf_lo,
+                                                      const float4* __restrict__ leaf_hi,
+                                                      const uint8_t* __restrict__ choice, WNode<W>* __restrict__ wnodes,
+                                                      double* sah_acc, int32_t* leaf_counter) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *n_in) return;
+    const CollapseItem it = in[idx];
+    auto is_leaf = [&](int32_t r) { return r < 0 || choice[int64_t(r) * W] == 0xFF; };
+    int32_t ref[W];
+    int nc = 0;
+    if (is_leaf(it.bref)) {
+        ref[nc++] = it.bref;   // the whole tree is one leaf (tiny scenes)
+    } else {
+        int32_t sr[W];
+        int sj[W];
+        int sp = 0;
+        const BNode b = bn[it.bref];
+        const int k = choice[int64_t(it.bref) * W];
+        sr[sp] = b.right; sj[sp++] = W - k;
+        sr[sp] = b.left; sj[sp++] = k;
+        while (sp > 0) {
+            --sp;
+            const int32_t m = sr[sp];
+            int j = sj[sp];
+            if (m >= 0)
+                while (j > 1 && choice[int64_t(m) * W + j - 1] == 0) j--;
+            if (m < 0 || j == 1) {
+                ref[nc++] = m;
+                continue;
+            }
+            const int c = choice[int64_t(m) * W + j - 1];
+            const BNode bm = bn[m];
+            sr[sp] = bm.right; sj[sp++] = j - c;
+            sr[sp] = bm.left; sj[sp++] = c;
+        }
+    }
+    emit_wide_node<W>(it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter, wnodes,
+                      sah_acc, leaf_counter);
+}
+
+// Greedy collapse: one work item per wide node; open the largest-area child
+// until W, then write the node and emit the internal children as next-level
+// items.
 template <int W>
 __global__ __launch_bounds__(256) void k_collapse(const CollapseItem* __restrict__ in, const int32_t* __restrict__ n_in,
                                                   CollapseItem* __restrict__ out, int32_t* __restrict__ n_out,
@@ -285,40 +475,9 @@ __global__ __launch_bounds__(256) void k_collapse(const CollapseItem* __restrict
             ref[nc++] = b.right;
         }
     }
-    WNode<W> node;
-    double sah = 0.0;
-    int nleaf = 0;
-    for (int c = 0; c < W; c++) {
-        if (c < nc) {
-            const int32_t r = ref[c];
-            const float4 lo = lo_of(r), hi = hi_of(r);
-            node.lox[c] = lo.x; node.loy[c] = lo.y; node.loz[c] = lo.z;
-            node.hix[c] = hi.x; node.hiy[c] = hi.y; node.hiz[c] = hi.z;
-            const int cnt = cnt_of(r);
-            const double a = double(half_area(lo, hi));
-            if (r < 0 || cnt <= leaf_size) {
-                node.child[c] = r < 0 ? ~r : bn[r].first;
-                node.count[c] = cnt;
-                sah += a * cnt;
-                nleaf++;
-            } else {
-                const int32_t slot = atomicAdd(node_counter, 1);
-                node.child[c] = slot;
-                node.count[c] = 0;
-                sah += a;
-                const int32_t q = atomicAdd(n_out, 1);
-                out[q] = CollapseItem{r, slot};
-            }
-        } else {
-            node.lox[c] = node.loy[c] = node.loz[c] = 1e30f;
-            node.hix[c] = node.hiy[c] = node.hiz[c] = -1e30f;
-            node.child[c] = 0;
-            node.count[c] = -1;
-        }
-    }
-    wnodes[it.wnode] = node;
-    atomicAdd(sah_acc, sah);
-    atomicAdd(leaf_counter, nleaf);
+    auto is_leaf = [&](int32_t r) { return r < 0 || cnt_of(r) <= leaf_size; };
+    emit_wide_node<W>(it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter, wnodes,
+                      sah_acc, leaf_counter);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,23 +540,34 @@ __global__ void k_zero_counter(unsigned long long* c) {
 static inline unsigned nblocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
 template <int W>
-static void launch_collapse(const CollapseItem* in, const int32_t* n_in, CollapseItem* out, int32_t* n_out,
-                            int32_t* node_counter, const BNode* bn, const float4* node_lo, const float4* node_hi,
-                            const float4* leaf_lo, const float4* leaf_hi, void* wnodes, int leaf_size,
-                            double* sah, int32_t* leaves, int64_t count, hipStream_t s) {
-    hipLaunchKernelGGL(k_collapse<W>, dim3(nblocks(count, 256)), dim3(256), 0, s, in, n_in, out, n_out, node_counter,
-                       bn, node_lo, node_hi, leaf_lo, leaf_hi, static_cast<WNode<W>*>(wnodes), leaf_size, sah,
-                       leaves);
+static void launch_collapse(bool sah_collapse, const CollapseItem* in, const int32_t* n_in, CollapseItem* out,
+                            int32_t* n_out, int32_t* node_counter, const BNode* bn, const float4* node_lo,
+                            const float4* node_hi, const float4* leaf_lo, const float4* leaf_hi,
+                            const uint8_t* choice, void* wnodes, int leaf_size, double* sah, int32_t* leaves,
+                            int64_t count, hipStream_t s) {
+    if (sah_collapse)
+        hipLaunchKernelGGL(k_collapse_sah<W>, dim3(nblocks(count, 256)), dim3(256), 0, s, in, n_in, out, n_out,
+                           node_counter, bn, node_lo, node_hi, leaf_lo, leaf_hi, choice,
+                           static_cast<WNode<W>*>(wnodes), sah, leaves);
+    else
+        hipLaunchKernelGGL(k_collapse<W>, dim3(nblocks(count, 256)), dim3(256), 0, s, in, n_in, out, n_out,
+                           node_counter, bn, node_lo, node_hi, leaf_lo, leaf_hi, static_cast<WNode<W>*>(wnodes),
+                           leaf_size, sah, leaves);
 }
 
 int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_build_opts* opts, hipStream_t s) {
     int W = opts && opts->width ? opts->width : 4;
-    int leaf = opts && opts->leaf_size ? opts->leaf_size : 2;
+    int leaf = opts && opts->leaf_size ? opts->leaf_size : kDefaultLeafSize;
     const int rf = opts && opts->refill ? opts->refill : 32;
     if (rf < 1 || rf > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "refill must be in [1, 64]");
     const int builder = opts ? opts->builder : LMGPU_BUILDER_LBVH;
     if (builder != LMGPU_BUILDER_PLOC && builder != LMGPU_BUILDER_LBVH)
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "unknown builder");
+    const int collapse = opts ? opts->collapse : LMGPU_COLLAPSE_SAH;
+    if (collapse != LMGPU_COLLAPSE_SAH && collapse != LMGPU_COLLAPSE_GREEDY)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "unknown collapse");
+    const float ctri = opts && opts->sah_tri_cost > 0.f ? opts->sah_tri_cost : kDefaultSahTriCost;
+    if (!(ctri > 0.f && ctri < 1e6f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "sah_tri_cost out of range");
     const int radius = opts && opts->ploc_radius ? opts->ploc_radius : 16;
     if (W != 2 && W != 4 && W != 8) return fail(LMGPU_ERR_INVALID_ARGUMENT, "BVH width must be 2, 4 or 8");
     if (leaf < 1 || leaf > 15) return fail(LMGPU_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 15]");
@@ -421,8 +591,10 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     LMGPU_HIP(hipEventCreate(&e1));
     LMGPU_HIP(hipEventRecord(e0, s));
 
+    const int64_t m = std::max<int64_t>(n - 1, 1);   // binary internal nodes
     DevBuf<float4> tlo, thi, leaf_lo, leaf_hi, node_lo, node_hi;
-    DevBuf<float> cb, partial;
+    DevBuf<float> cb, partial, dp_cost;
+    DevBuf<uint8_t> dp_choice;
     DevBuf<uint64_t> keys, keys2;
     DevBuf<int32_t> vals, vals2, parent_int, parent_leaf, flags, counters;
     DevBuf<BNode> bn;
@@ -430,68 +602,86 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     DevBuf<double> sah;
     LMGPU_TRY(tlo.alloc(n)); LMGPU_TRY(thi.alloc(n));
     LMGPU_TRY(leaf_lo.alloc(n)); LMGPU_TRY(leaf_hi.alloc(n));
-    LMGPU_TRY(node_lo.alloc(std::max<int64_t>(n - 1, 1))); LMGPU_TRY(node_hi.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(node_lo.alloc(m)); LMGPU_TRY(node_hi.alloc(m));
     LMGPU_TRY(cb.alloc(6));
     const unsigned nb_prep = nblocks(n, 256);
     LMGPU_TRY(partial.alloc(size_t(nb_prep) * 6));
     LMGPU_TRY(keys.alloc(n)); LMGPU_TRY(keys2.alloc(n));
     LMGPU_TRY(vals.alloc(n)); LMGPU_TRY(vals2.alloc(n));
-    LMGPU_TRY(parent_int.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(parent_int.alloc(m));
     LMGPU_TRY(parent_leaf.alloc(n));
-    LMGPU_TRY(flags.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(flags.alloc(m));
     LMGPU_TRY(counters.alloc(8));
-    LMGPU_TRY(bn.alloc(std::max<int64_t>(n - 1, 1)));
+    LMGPU_TRY(bn.alloc(m));
     LMGPU_TRY(q0.alloc(n)); LMGPU_TRY(q1.alloc(n));
     LMGPU_TRY(sah.alloc(1));
+    SahDP dp{nullptr, nullptr, W, leaf, ctri};
+    if (collapse == LMGPU_COLLAPSE_SAH) {
+        LMGPU_TRY(dp_cost.alloc(size_t(m) * W));
+        LMGPU_TRY(dp_choice.alloc(size_t(m) * W));
+        dp.cost = dp_cost.p;
+        dp.choice = dp_choice.p;
+    }
 
     hipLaunchKernelGGL(k_prep, dim3(nb_prep), dim3(256), 0, s, verts, n, tlo.p, thi.p, partial.p);
     hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(256), 0, s, partial.p, int(nb_prep), cb.p);
     hipLaunchKernelGGL(k_morton, dim3(nblocks(n, 256)), dim3(256), 0, s, tlo.p, thi.p, n, cb.p, keys.p, vals.p);
-    size_t temp_bytes = 0;
-    LMGPU_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
-    DevBuf<char> temp;
-    LMGPU_TRY(temp.alloc(std::max<size_t>(temp_bytes, 16)));
-    LMGPU_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys.p, keys2.p, vals.p, vals2.p, size_t(n), 0, 64, s));
-    int32_t root_ref = n > 1 ? 0 : ~0;
-    if (builder == LMGPU_BUILDER_PLOC) {
-        DevBuf<int32_t> perm;
-        LMGPU_TRY(perm.alloc(n));
-        LMGPU_TRY(build_ploc(n, vals2.p, tlo.p, thi.p, radius, bn.p, node_lo.p, node_hi.p, leaf_lo.p, leaf_hi.p,
-                             perm.p, &root_ref, s));
-        LMGPU_TRY(tris.alloc(n));
-        hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, verts, tris.p);
+    bool in_alt = false;
+    LMGPU_TRY(radix_sort_pairs_u64(keys.p, vals.p, keys2.p, vals2.p, n, 63, &in_alt, s));
+    const uint64_t* skeys = in_alt ? keys2.p : keys.p;
+    const int32_t* svals = in_alt ? vals2.p : vals.p;
+    if (n > 1) {
+        // the sort's result must be ordered before the tree is built on it
+        LMGPU_HIP(hipMemsetAsync(counters.p, 0, sizeof(int32_t), s));
+        hipLaunchKernelGGL(k_check_sorted, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, skeys, n, counters.p);
+        int32_t unsorted = 0;
+        LMGPU_HIP(hipMemcpyAsync(&unsorted, counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LMGPU_HIP(hipStreamSynchronize(s));
-    } else {
-        if (n > 1) {
-            LMGPU_HIP(hipMemsetAsync(flags.p, 0, sizeof(int32_t) * (n - 1), s));
-            LMGPU_HIP(hipMemsetAsync(parent_int.p, 0xff, sizeof(int32_t) * (n - 1), s));
-            hipLaunchKernelGGL(k_lbvh, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, keys2.p, n, bn.p, parent_int.p,
-                               parent_leaf.p);
-        }
-        {
-            // frontier refit: q0/q1 (reinterpreted as int32 lists) hold the frontiers
-            int32_t* fr[2] = {reinterpret_cast<int32_t*>(q0.p), reinterpret_cast<int32_t*>(q1.p)};
-            LMGPU_HIP(hipMemsetAsync(counters.p, 0, 2 * sizeof(int32_t), s));
-            hipLaunchKernelGGL(k_refit_leaves, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, tlo.p, thi.p,
-                               parent_leaf.p, leaf_lo.p, leaf_hi.p, flags.p, fr[0], counters.p);
-            int cur = 0;
-            int levels_refit = 0;
-            while (n > 1) {
-                int32_t cnt = 0;
-                LMGPU_HIP(hipMemcpyAsync(&cnt, counters.p + cur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-                LMGPU_HIP(hipStreamSynchronize(s));
-                if (cnt == 0) break;
-                LMGPU_HIP(hipMemsetAsync(counters.p + (cur ^ 1), 0, sizeof(int32_t), s));
-                hipLaunchKernelGGL(k_refit_nodes, dim3(nblocks(cnt, 256)), dim3(256), 0, s, fr[cur], counters.p + cur,
-                                   bn.p, parent_int.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p, fr[cur ^ 1],
-                                   counters.p + (cur ^ 1));
-                cur ^= 1;
-                if (++levels_refit > 4096) return fail(LMGPU_ERR_DEVICE, "BVH refit did not terminate");
-            }
-        }
-        LMGPU_TRY(tris.alloc(n));
-        hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vals2.p, verts, tris.p);
+        if (unsorted) return fail(LMGPU_ERR_DEVICE, "Morton sort produced an unordered key array");
     }
+    // binary tree over the sorted triangles: bn, parent links, and `order`
+    // (traversal position -> triangle)
+    int32_t root_ref = n > 1 ? 0 : ~0;
+    const int32_t* order = svals;
+    DevBuf<int32_t> perm;
+    if (n > 1) LMGPU_HIP(hipMemsetAsync(parent_int.p, 0xff, sizeof(int32_t) * (n - 1), s));
+    if (builder == LMGPU_BUILDER_PLOC) {
+        LMGPU_TRY(perm.alloc(n));
+        LMGPU_TRY(build_ploc(n, svals, tlo.p, thi.p, radius, bn.p, node_lo.p, node_hi.p, leaf_lo.p, leaf_hi.p,
+                             perm.p, &root_ref, s));
+        order = perm.p;
+        if (n > 1)
+            hipLaunchKernelGGL(k_parents, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, bn.p, n - 1, parent_int.p,
+                               parent_leaf.p);
+    } else if (n > 1) {
+        hipLaunchKernelGGL(k_lbvh, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, skeys, n, bn.p, parent_int.p,
+                           parent_leaf.p);
+    }
+    {
+        // frontier refit (bounds, and the SAH cost tables when collapsing by
+        // SAH): q0/q1 (reinterpreted as int32 lists) hold the frontiers
+        if (n > 1) LMGPU_HIP(hipMemsetAsync(flags.p, 0, sizeof(int32_t) * (n - 1), s));
+        int32_t* fr[2] = {reinterpret_cast<int32_t*>(q0.p), reinterpret_cast<int32_t*>(q1.p)};
+        LMGPU_HIP(hipMemsetAsync(counters.p, 0, 2 * sizeof(int32_t), s));
+        hipLaunchKernelGGL(k_refit_leaves, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, tlo.p, thi.p,
+                           parent_leaf.p, leaf_lo.p, leaf_hi.p, flags.p, fr[0], counters.p);
+        int cur = 0;
+        int levels_refit = 0;
+        while (n > 1) {
+            int32_t cnt = 0;
+            LMGPU_HIP(hipMemcpyAsync(&cnt, counters.p + cur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            LMGPU_HIP(hipStreamSynchronize(s));
+            if (cnt == 0) break;
+            LMGPU_HIP(hipMemsetAsync(counters.p + (cur ^ 1), 0, sizeof(int32_t), s));
+            hipLaunchKernelGGL(k_refit_nodes, dim3(nblocks(cnt, 256)), dim3(256), 0, s, fr[cur], counters.p + cur,
+                               bn.p, root_ref, parent_int.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p,
+                               fr[cur ^ 1], counters.p + (cur ^ 1), dp);
+            cur ^= 1;
+            if (++levels_refit > 4096) return fail(LMGPU_ERR_DEVICE, "BVH refit did not terminate");
+        }
+    }
+    LMGPU_TRY(tris.alloc(n));
+    hipLaunchKernelGGL(k_reorder, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, verts, tris.p);
 
     // collapse, level by level
     const size_t node_bytes = size_t(32) * W;
@@ -509,15 +699,19 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
     while (count > 0) {
         levels++;
         LMGPU_HIP(hipMemsetAsync(counters.p + 1, 0, sizeof(int32_t), s));
+        const bool by_sah = collapse == LMGPU_COLLAPSE_SAH;
         if (W == 2)
-            launch_collapse<2>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
-                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+            launch_collapse<2>(by_sah, qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p,
+                               node_hi.p, leaf_lo.p, leaf_hi.p, dp.choice, nodes.p, leaf, sah.p, counters.p + 3, count,
+                               s);
         else if (W == 4)
-            launch_collapse<4>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
-                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+            launch_collapse<4>(by_sah, qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p,
+                               node_hi.p, leaf_lo.p, leaf_hi.p, dp.choice, nodes.p, leaf, sah.p, counters.p + 3, count,
+                               s);
         else
-            launch_collapse<8>(qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p, node_hi.p,
-                               leaf_lo.p, leaf_hi.p, nodes.p, leaf, sah.p, counters.p + 3, count, s);
+            launch_collapse<8>(by_sah, qin, counters.p, qout, counters.p + 1, counters.p + 2, bn.p, node_lo.p,
+                               node_hi.p, leaf_lo.p, leaf_hi.p, dp.choice, nodes.p, leaf, sah.p, counters.p + 3, count,
+                               s);
         int32_t nout = 0;
         LMGPU_HIP(hipMemcpyAsync(&nout, counters.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LMGPU_HIP(hipStreamSynchronize(s));

@@ -169,7 +169,7 @@ constexpr uint32_t kSerialVersion = 1;
 template <int W>
 int validate_nodes(const std::vector<unsigned char>& nb, int64_t num_nodes, int64_t num_tris, int* depth_out) {
     const auto* nodes = reinterpret_cast<const lmgpu::WNode<W>*>(nb.data());
-    std::vector<unsigned char> seen(size_t(num_nodes), 0);
+    std::vector<unsigned char> seen(size_t(num_nodes), 0), covered(size_t(num_tris), 0);
     std::vector<std::pair<int64_t, int>> st{{0, 1}};
     int64_t visited = 0;
     int depth = 0;
@@ -189,10 +189,17 @@ int validate_nodes(const std::vector<unsigned char>& nb, int64_t num_nodes, int6
             } else if (cnt > 0) {
                 if (cnt > 15 || ch < 0 || int64_t(ch) + cnt > num_tris)
                     return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: leaf range out of range");
+                for (int k = 0; k < cnt; k++) {
+                    if (covered[size_t(ch) + k])
+                        return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: a triangle is in two leaves");
+                    covered[size_t(ch) + k] = 1;
+                }
             }
         }
     }
     if (visited != num_nodes) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH has unreachable nodes");
+    for (unsigned char c : covered)
+        if (!c) return fail(LMGPU_ERR_INVALID_ARGUMENT, "serialized BVH: a triangle is in no leaf");
     *depth_out = depth;
     return LMGPU_OK;
 }

@@ -21,8 +21,6 @@
 #include <string>
 #include <vector>
 
-#include <rocprim/rocprim.hpp>
-
 #include "build.h"
 #include "lmgpu_internal.h"
 
@@ -235,11 +233,9 @@ int build_ploc(int64_t n, const int32_t* order, const float4* tlo, const float4*
     LMGPU_TRY(nodes.alloc(std::max<int64_t>(n - 1, 1)));
     hipLaunchKernelGGL(k_ploc_init, dim3(nb(n)), dim3(kB), 0, s, n, order, tlo, thi, cid[0].p, clo[0].p, chi[0].p,
                        slo.p, shi.p);
-    size_t temp_bytes = 0;
-    LMGPU_HIP(rocprim::exclusive_scan(nullptr, temp_bytes, flags.p, pre.p, 0ull, size_t(n),
-                                      rocprim::plus<unsigned long long>(), s));
+    const size_t temp_bytes = exclusive_scan_u64_temp_bytes(n);
     DevBuf<char> temp;
-    LMGPU_TRY(temp.alloc(std::max<size_t>(temp_bytes, 16)));
+    LMGPU_TRY(temp.alloc(temp_bytes));
     int count = int(n);
     int node_base = 0;
     int cur = 0;
@@ -249,9 +245,8 @@ int build_ploc(int64_t n, const int32_t* order, const float4* tlo, const float4*
         if (force_pairs) hipLaunchKernelGGL(k_ploc_pairs, dim3(nb(count)), dim3(kB), 0, s, count, nn.p);
         else hipLaunchKernelGGL(k_ploc_nn, dim3(nb(count)), dim3(kB), 0, s, count, clo[cur].p, chi[cur].p, nn.p, radius);
         hipLaunchKernelGGL(k_ploc_flags, dim3(nb(count)), dim3(kB), 0, s, count, nn.p, flags.p);
-        size_t tb = temp_bytes;
-        LMGPU_HIP(rocprim::exclusive_scan(temp.p, tb, flags.p, pre.p, 0ull, size_t(count),
-                                          rocprim::plus<unsigned long long>(), s));
+        LMGPU_TRY(exclusive_scan_u64(reinterpret_cast<const uint64_t*>(flags.p), reinterpret_cast<uint64_t*>(pre.p),
+                                     count, temp.p, temp_bytes, s));
         hipLaunchKernelGGL(k_ploc_commit, dim3(nb(count)), dim3(kB), 0, s, count, nn.p, cid[cur].p, clo[cur].p,
                            chi[cur].p, flags.p, pre.p, node_base, nodes.p, node_lo, node_hi, parent_leaf.p,
                            parent_int.p, cid[cur ^ 1].p, clo[cur ^ 1].p, chi[cur ^ 1].p, totals.p);

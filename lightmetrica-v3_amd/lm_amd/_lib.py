@@ -54,11 +54,14 @@ class LmError(RuntimeError):
 
 class BuildOpts(ctypes.Structure):
     _fields_ = [('width', ctypes.c_int32), ('leaf_size', ctypes.c_int32), ('refill', ctypes.c_int32),
-                ('builder', ctypes.c_int32), ('ploc_radius', ctypes.c_int32), ('reserved', ctypes.c_int32 * 3)]
+                ('builder', ctypes.c_int32), ('ploc_radius', ctypes.c_int32), ('collapse', ctypes.c_int32),
+                ('sah_tri_cost', ctypes.c_float), ('reserved', ctypes.c_int32 * 1)]
 
 
 BUILDER_PLOC = 0
 BUILDER_LBVH = 1
+COLLAPSE_SAH = 0
+COLLAPSE_GREEDY = 1
 
 
 class AccelInfo(ctypes.Structure):

@@ -67,12 +67,15 @@ def _stream_ptr(stream) -> Optional[int]:
 class Accel:
     """accel::mi355x: GPU LBVH collapsed to a W-wide BVH + ordered traversal."""
 
-    def __init__(self, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1, radius: int = 16):
+    def __init__(self, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1, radius: int = 16,
+                 collapse: int = 0, sah_tri_cost: float = 0.0):
         self.width = width
         self.leaf_size = leaf_size
         self.refill = refill
         self.builder = builder
         self.radius = radius
+        self.collapse = collapse
+        self.sah_tri_cost = sah_tri_cost
         h = ctypes.c_void_p()
         _lib.call('lmgpu_accel_create', ctypes.byref(h))
         self._h = h
@@ -98,7 +101,8 @@ class Accel:
             verts = scene_or_verts.verts
             self.tri_prim = np.asarray(scene_or_verts.tri_prim)
             self.tri_face = np.asarray(scene_or_verts.tri_face)
-        opts = _lib.BuildOpts(self.width, self.leaf_size, self.refill, self.builder, self.radius)
+        opts = _lib.BuildOpts(self.width, self.leaf_size, self.refill, self.builder, self.radius, self.collapse,
+                              self.sah_tri_cost)
         try:
             import torch
             is_tensor = isinstance(verts, torch.Tensor)

@@ -127,10 +127,10 @@ class GpuScene:
     lm_amd.scenes.FlatScene through lmgpu_scene_create."""
 
     def __init__(self, flat, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1,
-                 radius: int = 16):
+                 radius: int = 16, collapse: int = 0, sah_tri_cost: float = 0.0):
         self.flat = flat
         self._keep, d = scene_desc(flat)
-        opts = _lib.BuildOpts(width, leaf_size, refill, builder, radius)
+        opts = _lib.BuildOpts(width, leaf_size, refill, builder, radius, collapse, sah_tri_cost)
         h = ctypes.c_void_p()
         _lib.call('lmgpu_scene_create', ctypes.byref(h), ctypes.byref(d), ctypes.byref(opts))
         self._h = h

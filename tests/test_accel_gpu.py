@@ -323,3 +323,52 @@ def test_serialize_roundtrip(device, tmp_path):
         torch.cuda.synchronize()
         assert torch.equal(ha.view(torch.int32), hc.view(torch.int32))
         assert blob == c.serialize()
+
+
+def spread21(x):
+    x = x & np.uint64(0x1fffff)
+    for sh, mask in ((32, 0x1f00000000ffff), (16, 0x1f0000ff0000ff), (8, 0x100f00f00f00f00f),
+                     (4, 0x10c30c30c30c30c3), (2, 0x1249249249249249)):
+        x = (x | (x << np.uint64(sh))) & np.uint64(mask)
+    return x
+
+
+def morton_codes(tris):
+    """The builder's 63-bit Morton codes of the triangle centroids, in the
+    same FP32 arithmetic (bounds, centroid, normalisation to the centroid
+    box, 21 bits per axis, x in the highest bit of each triple)."""
+    v = np.asarray(tris, dtype=np.float32).reshape(-1, 3, 3)
+    lo, hi = v.min(axis=1), v.max(axis=1)
+    c = (lo + hi) * np.float32(0.5)
+    mn, mx = c.min(axis=0), c.max(axis=0)
+    ext = mx - mn
+    code = np.zeros(v.shape[0], dtype=np.uint64)
+    for a in range(3):
+        f = (c[:, a] - mn[a]) / ext[a] if ext[a] > 0 else np.full(v.shape[0], 0.5, np.float32)
+        f = np.clip(f, np.float32(0), np.float32(1))
+        q = np.minimum(f * np.float32(2097152.0), np.float32(2097151.0)).astype(np.uint64)
+        code |= spread21(q) << np.uint64(2 - a)
+    return code
+
+
+def test_morton_sort_order(device):
+    """The device radix sort (sort.hip) orders the triangles exactly as a
+    stable sort of their Morton codes: the LBVH's triangle array (read back
+    through the serialized BVH) equals numpy's stable argsort, on a mesh with
+    duplicated triangles (equal keys keep input order) and enough triangles
+    for several sort tiles and a two-level scan of the digit histograms."""
+    sph, cl = scenes.config1_mesh(levels=7, clutter_count=5000)
+    tris = scenes.merge_meshes([sph, cl]).triangles()
+    rng = np.random.default_rng(3)
+    dup = tris[rng.integers(0, tris.shape[0], 20000)]
+    tris = np.ascontiguousarray(np.concatenate([tris, dup, tris[:7]]))
+    acc = Accel(4, 2)
+    acc.build(tris)
+    info = acc.info()
+    blob = np.frombuffer(acc.serialize(), dtype=np.uint8)
+    off = 128 + info['num_nodes'] * 32 * 4
+    tv = blob[off:off + tris.shape[0] * 48].view(np.float32).reshape(-1, 12)
+    got = tv[:, 3].view(np.int32)
+    want = np.argsort(morton_codes(tris), kind='stable')
+    print(f'{tris.shape[0]} triangles, {np.count_nonzero(got != want)} positions differ')
+    assert np.array_equal(got, want)

@@ -30,7 +30,7 @@ class _Calls:
 def _accel():
     a = Accel.__new__(Accel)   # no library handle needed: nothing may be called
     a._h = ctypes.c_void_p()
-    a.width, a.leaf_size, a.refill, a.builder, a.radius = 4, 2, 32, 1, 16
+    a.width, a.leaf_size, a.refill, a.builder, a.radius, a.collapse, a.sah_tri_cost = 4, 2, 32, 1, 16, 0, 0.0
     return a
 
 
