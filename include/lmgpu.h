@@ -332,8 +332,16 @@ typedef struct lmgpu_batch_opts {
     int32_t block;          /* block size of the generate / shade kernels, 128 or 256; 0 = autotune */
     int32_t leaf_size;      /* 0 = 2 */
     int32_t film_sumsq;     /* 1: also accumulate per-pixel sums of squares (lmgpu_batch_job_film) */
-    int32_t reserved[3];
+    int32_t queue_order;    /* LMGPU_QUEUE_JOB_MAJOR (0) or LMGPU_QUEUE_INTERLEAVED (1) */
+    int32_t reserved[2];
 } lmgpu_batch_opts;
+
+/* Order of the device work queue's (job, sample, pixel) items. JOB_MAJOR:
+ * job by job, each sample by sample (a job finishes as early as it can).
+ * INTERLEAVED: sample 0 of every job, then sample 1 of every job that has
+ * it, and so on (jobs share the device until their last sample). */
+#define LMGPU_QUEUE_JOB_MAJOR 0
+#define LMGPU_QUEUE_INTERLEAVED 1
 
 typedef struct lmgpu_batch_stats {
     int64_t jobs, samples, extension_rays, shadow_rays, iterations;

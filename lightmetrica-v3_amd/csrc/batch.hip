@@ -45,8 +45,14 @@ struct JobDev {
     int root;             // root of the job's BVH in the node arena
     int npix, spp, max_verts, mode, rr_depth;
     uint32_t seed_lo, seed_hi;
-    int64_t work_off;     // first work item of the job
     int64_t film_off;     // first double of the job's film (3 per pixel)
+};
+
+// A run of consecutive work items: the pixels 0.. of one sample of one job,
+// starting at queue position `start`.
+struct WorkSeg {
+    int64_t start;
+    int32_t job, sample;
 };
 
 constexpr int32_t kFree = 0, kActive = 1, kFinishing = 2;
@@ -71,6 +77,7 @@ struct BatchCtl {
     unsigned long long remaining;   // work items not yet accumulated
     unsigned long long ext_rays, shadow_rays;
     unsigned long long t_start;     // device clock at the start of the run
+    unsigned long long iterations;  // iterations that found work left
 };
 
 __device__ __forceinline__ uint64_t job_seed(const JobDev& J) { return (uint64_t(J.seed_hi) << 32) | J.seed_lo; }
@@ -78,10 +85,15 @@ __device__ __forceinline__ uint64_t job_seed(const JobDev& J) { return (uint64_t
 // Frees finishing paths (their sample goes to the job film) and hands the
 // free slots new work items, appending their camera rays to queue in_q.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jobs, const int64_t* __restrict__ work_offs,
-                                                   int njobs, int64_t total, PoolDev P, BatchQ q, int in_q, int cap,
+__global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jobs, const WorkSeg* __restrict__ segs,
+                                                   int nsegs, int64_t total, PoolDev P, BatchQ q, int in_q, int cap,
                                                    BatchCtl* ctl, unsigned long long* job_done,
                                                    unsigned long long* job_time, double* film_sum, double* film_sq) {
+    // Every item accumulated: no slot is finishing and nothing is left to
+    // hand out, so this and the rest of the iteration's kernels are no-ops
+    // (the host enqueues iterations up to a bound without watching).
+    if (__hip_atomic_load(&ctl->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->iterations, 1ull);
     const int stride = gridDim.x * BLOCK;
     const int n_iter = (cap + stride - 1) / stride;
     const int lane = __lane_id();
@@ -143,17 +155,17 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
         if (want) {
             const unsigned long long w = base + __popcll(mask & ((1ull << lane) - 1ull));
             if (w < (unsigned long long)total) {
-                // job of item w: last job whose first item is <= w
-                int lo = 0, hi = njobs - 1;
+                // segment of item w: the last one starting at or before w
+                int lo = 0, hi = nsegs - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (work_offs[mid] <= int64_t(w)) lo = mid;
+                    if (segs[mid].start <= int64_t(w)) lo = mid;
                     else hi = mid - 1;
                 }
-                const JobDev& J = jobs[lo];
-                const int64_t local = int64_t(w) - J.work_off;
-                const int sample = int(local / J.npix), pix = int(local % J.npix);
-                P.job[s] = lo;
+                const WorkSeg seg = segs[lo];
+                const JobDev& J = jobs[seg.job];
+                const int sample = seg.sample, pix = int(int64_t(w) - seg.start);
+                P.job[s] = seg.job;
                 P.pixel[s] = pix;
                 P.sample[s] = sample;
                 P.state[s] = kActive;
@@ -302,6 +314,7 @@ __global__ void k_b_init(BatchCtl* ctl, int64_t total, int32_t* counts) {
         ctl->remaining = (unsigned long long)total;
         ctl->ext_rays = 0;
         ctl->shadow_rays = 0;
+        ctl->iterations = 0;
     }
     if (threadIdx.x < 16) counts[threadIdx.x] = 0;
 }
@@ -370,11 +383,11 @@ struct Batch {
     DevBuf<DevLight> lights;
     DevBuf<float> cdf;
     DevBuf<int32_t> ltri;
-    std::vector<JobDev> jobdev;   // per job, work_off / film_off filled per run
+    std::vector<JobDev> jobdev;   // per job, film_off filled per run
     const void* trav_nodes() const { return qnodes.p ? static_cast<const void*>(qnodes.p) : nodes.p; }
     // run state
     DevBuf<JobDev> djobs;
-    DevBuf<int64_t> dwork;
+    DevBuf<WorkSeg> dsegs;
     DevBuf<unsigned long long> job_done, job_time;
     DevBuf<double> film, film_sq;
     int64_t film_doubles = 0;
@@ -397,8 +410,8 @@ struct Batch {
     int build_arena(int width, int leaf, hipStream_t s);
     int ensure_pool(int cap);
     bool sumsq = false;   // the last run accumulated sums of squares
-    int run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, double* elapsed,
-                 int64_t* iters, hipStream_t s);
+    int run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, int queue_order,
+                 double* elapsed, int64_t* iters, hipStream_t s);
 };
 
 int Batch::build_arena(int width, int leaf, hipStream_t s) {
@@ -529,11 +542,11 @@ int Batch::ensure_pool(int c) {
 
 // Renders every job once with the current arena and pool: at most spp_cap
 // samples per pixel per job when spp_cap > 0 (autotuning trials).
-int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, double* elapsed,
-                    int64_t* iters, hipStream_t s) {
+int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, int queue_order,
+                    double* elapsed, int64_t* iters, hipStream_t s) {
     const int nj = int(jobs.size());
     int64_t total = 0, film_n = 0;
-    std::vector<int64_t> work(nj);
+    int max_spp = 0;
     for (int j = 0; j < nj; j++) {
         JobDev& J = jobdev[j];
         const BatchJob& B = jobs[j];
@@ -544,14 +557,31 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
         J.rr_depth = B.rr_depth;
         J.seed_lo = uint32_t(B.seed);
         J.seed_hi = uint32_t(B.seed >> 32);
-        J.work_off = total;
         J.film_off = film_n;
-        work[j] = total;
         total += int64_t(J.npix) * J.spp;
         film_n += int64_t(J.npix) * 3;
+        max_spp = std::max(max_spp, J.spp);
     }
+    // the work queue as segments of one (job, sample) each, in queue order
+    std::vector<WorkSeg> segs;
+    int64_t cursor = 0;
+    auto add_seg = [&](int j, int smp) {
+        segs.push_back(WorkSeg{cursor, j, smp});
+        cursor += jobdev[j].npix;
+    };
+    if (queue_order == LMGPU_QUEUE_INTERLEAVED) {
+        for (int smp = 0; smp < max_spp; smp++)
+            for (int j = 0; j < nj; j++)
+                if (smp < jobdev[j].spp) add_seg(j, smp);
+    } else {
+        for (int j = 0; j < nj; j++)
+            for (int smp = 0; smp < jobdev[j].spp; smp++) add_seg(j, smp);
+    }
+    if (cursor != total || segs.size() >= size_t(1) << 31)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch work queue too large");
+    const int nsegs = int(segs.size());
     if (djobs.n != size_t(nj)) LMGPU_TRY(djobs.alloc(nj));
-    if (dwork.n != size_t(nj)) LMGPU_TRY(dwork.alloc(nj));
+    if (dsegs.n != segs.size()) LMGPU_TRY(dsegs.alloc(segs.size()));
     if (job_done.n != size_t(nj)) {
         LMGPU_TRY(job_done.alloc(nj));
         LMGPU_TRY(job_time.alloc(nj));
@@ -563,7 +593,7 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
     }
     LMGPU_HIP(hipStreamSynchronize(s));
     LMGPU_HIP(hipMemcpy(djobs.p, jobdev.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice));
-    LMGPU_HIP(hipMemcpy(dwork.p, work.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice));
+    LMGPU_HIP(hipMemcpy(dsegs.p, segs.data(), sizeof(WorkSeg) * segs.size(), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_done.p, int64_t(nj));
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_time.p, int64_t(nj));
     hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film.p, film_n);
@@ -584,10 +614,10 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
         const int in_q = par;
         const unsigned rg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)));
         if (block == 128)
-            hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
+            hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dsegs.p, nsegs, total, P, q, in_q, cap,
                                ctl.p, job_done.p, job_time.p, film.p, fsq);
         else
-            hipLaunchKernelGGL(k_b_regen<256>, dim3(rg), dim3(256), 0, cs, djobs.p, dwork.p, nj, total, P, q, in_q, cap,
+            hipLaunchKernelGGL(k_b_regen<256>, dim3(rg), dim3(256), 0, cs, djobs.p, dsegs.p, nsegs, total, P, q, in_q, cap,
                                ctl.p, job_done.p, job_time.p, film.p, fsq);
         hipLaunchKernelGGL(k_b_zero2, dim3(1), dim3(64), 0, cs, next.p);
         if (width == 2)
@@ -632,29 +662,25 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
     LMGPU_HIP(hipEventCreate(&e1));
     LMGPU_HIP(hipEventRecord(e0, s));
     hipLaunchKernelGGL(k_b_clock, dim3(1), dim3(64), 0, s, ctl.p);
-    int64_t it = 0;
+    // No polling: the replays up to the bound are enqueued back to back; once
+    // the device has accumulated every item its kernels return at once
+    // (k_b_regen), and the host synchronizes once, after the last replay.
     int rc = LMGPU_OK;
-    while (true) {
-        for (int r = 0; r < 8; r++) {
-            if ((e = hipGraphLaunch(ge, s)) != hipSuccess) break;
-            it += 2;
-        }
-        if (e != hipSuccess) {
+    for (int64_t r = 0; r < (iter_cap + 1) / 2; r++) {
+        if ((e = hipGraphLaunch(ge, s)) != hipSuccess) {
             rc = fail(LMGPU_ERR_DEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
             break;
         }
-        BatchCtl hc;
-        if ((e = hipMemcpyAsync(&hc, ctl.p, sizeof(hc), hipMemcpyDeviceToHost, s)) != hipSuccess ||
-            (e = hipStreamSynchronize(s)) != hipSuccess) {
-            rc = fail(LMGPU_ERR_DEVICE, std::string("batch progress: ") + hipGetErrorString(e));
-            break;
-        }
-        if (hc.remaining == 0) break;
-        if (it > iter_cap) {
-            rc = fail(LMGPU_ERR_FAILED_TO_RENDER, "batch did not finish within its iteration bound");
-            break;
-        }
     }
+    BatchCtl hc{};
+    if (rc == LMGPU_OK) {
+        if ((e = hipMemcpyAsync(&hc, ctl.p, sizeof(hc), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            rc = fail(LMGPU_ERR_DEVICE, std::string("batch run: ") + hipGetErrorString(e));
+        else if (hc.remaining != 0)
+            rc = fail(LMGPU_ERR_FAILED_TO_RENDER, "batch did not finish within its iteration bound");
+    }
+    const int64_t it = int64_t(hc.iterations);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -774,6 +800,8 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "width must be 0 (autotune), 2, 4 or 8");
     if (o.path_capacity != 0 && (o.path_capacity < 1024 || o.path_capacity > (1 << 24)))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "path_capacity must be 0 (autotune) or in [1024, 2^24]");
+    if (o.queue_order != LMGPU_QUEUE_JOB_MAJOR && o.queue_order != LMGPU_QUEUE_INTERLEAVED)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "unknown queue_order");
     if (o.block != 0 && o.block != 128 && o.block != 256)
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "block must be 0 (autotune), 128 or 256");
     const int leaf = o.leaf_size > 0 ? o.leaf_size : 2;
@@ -796,7 +824,7 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
                 tb += std::chrono::duration<double>(now() - tb0).count();
                 LMGPU_TRY(b.ensure_pool(c));
                 int64_t it;
-                return b.run_once(w, bl, 1, false, o.film_sumsq != 0, sec, &it, s);
+                return b.run_once(w, bl, 1, false, o.film_sumsq != 0, o.queue_order, sec, &it, s);
             };
             const int widths[3] = {4, 8, 2}, caps[3] = {1 << 20, 1 << 19, 1 << 21}, blocks[2] = {256, 128};
             int bw = width ? width : 4, bc = capv ? capv : (1 << 20), bb = block ? block : 256;
@@ -832,7 +860,7 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
     LMGPU_TRY(b.ensure_pool(capv));
     double sec = 0;
     int64_t iters = 0;
-    LMGPU_TRY(b.run_once(width, block, 0, true, o.film_sumsq != 0, &sec, &iters, s));
+    LMGPU_TRY(b.run_once(width, block, 0, true, o.film_sumsq != 0, o.queue_order, &sec, &iters, s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         lmgpu::BatchCtl hc;

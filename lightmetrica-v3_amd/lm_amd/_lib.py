@@ -140,7 +140,12 @@ class JobDesc(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [('width', ctypes.c_int32), ('path_capacity', ctypes.c_int32), ('block', ctypes.c_int32),
-                ('leaf_size', ctypes.c_int32), ('film_sumsq', ctypes.c_int32), ('reserved', ctypes.c_int32 * 3)]
+                ('leaf_size', ctypes.c_int32), ('film_sumsq', ctypes.c_int32), ('queue_order', ctypes.c_int32),
+                ('reserved', ctypes.c_int32 * 2)]
+
+
+QUEUE_JOB_MAJOR = 0
+QUEUE_INTERLEAVED = 1
 
 
 class BatchStats(ctypes.Structure):

@@ -38,8 +38,11 @@ class Batch:
         return jid.value
 
     def run(self, width: int = 0, path_capacity: int = 0, block: int = 0, leaf_size: int = 0, sumsq: bool = False,
-            stream=None) -> dict:
-        o = _lib.BatchOpts(int(width), int(path_capacity), int(block), int(leaf_size), 1 if sumsq else 0)
+            interleaved: bool = False, stream=None) -> dict:
+        """Renders every job; interleaved=True orders the work queue sample
+        by sample across jobs instead of job by job."""
+        o = _lib.BatchOpts(int(width), int(path_capacity), int(block), int(leaf_size), 1 if sumsq else 0,
+                           _lib.QUEUE_INTERLEAVED if interleaved else _lib.QUEUE_JOB_MAJOR)
         self._sumsq = bool(sumsq)
         st = _lib.BatchStats()
         _lib.call('lmgpu_batch_run', self._h, ctypes.byref(o), ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))

@@ -70,6 +70,30 @@ def test_batch_jobs_equal_single_renders(device, width, cap, block):
     assert st['samples'] == total and st['jobs'] == len(jobs)
 
 
+def test_batch_interleaved_queue(device):
+    """The interleaved work queue (sample 0 of every job, then sample 1, ...)
+    renders the same films as the job-major queue and the single-scene
+    renderer, and moves every job's completion toward the end of the run:
+    the earliest job finishes later than under the job-major order."""
+    jobs = small_jobs()
+    b = Batch()
+    for flat, p in jobs:
+        b.add_job(flat, **p)
+    st = b.run(width=4, path_capacity=4096, block=256)
+    lat_major = [b.latency_ms(j) for j in range(len(jobs))]
+    st_i = b.run(width=4, path_capacity=4096, block=256, interleaved=True)
+    lat_inter = [b.latency_ms(j) for j in range(len(jobs))]
+    print('job-major', st, lat_major)
+    print('interleaved', st_i, lat_inter)
+    assert st_i['samples'] == st['samples']
+    assert st['iterations'] > 0 and st_i['iterations'] > 0
+    ref = single_renders(jobs, 4)
+    for j in range(len(jobs)):
+        s, _, _ = b.film(j)
+        assert np.allclose(s, ref[j][0], rtol=1e-9, atol=1e-9)
+    assert min(lat_inter) > min(lat_major)
+
+
 def test_batch_autotune(device):
     """Fields left 0 are chosen by the autotuner; the films do not depend on
     the choice beyond the BVH width, and a second run reuses it."""
@@ -105,6 +129,11 @@ def test_batch_errors(device):
         b.run(width=4, path_capacity=100)
     with pytest.raises(LmError):
         b.add_job(flat, spp=0)
+    from lm_amd import _lib
+    o = _lib.BatchOpts(4, 4096, 256, 0, 0, 7)
+    st = _lib.BatchStats()
+    with pytest.raises(LmError, match='queue_order'):
+        _lib.call('lmgpu_batch_run', b._h, o, st, None)
 
 
 def test_c3_jobs_welch_vs_oracle(device):
