@@ -358,6 +358,12 @@ struct OrcPTStats {
     int64_t extension_rays;
     int64_t shadow_rays;
     int64_t path_vertices;
+    // renderer::pt only: node visits and triangle tests of the SAH BVH
+    // traversal (accel_sahbvh.cpp:269-299), extension and shadow rays apart
+    int64_t extension_node_visits;
+    int64_t extension_tri_tests;
+    int64_t shadow_node_visits;
+    int64_t shadow_tri_tests;
 };
 
 }  // extern "C"
@@ -746,9 +752,10 @@ Mat material_at(const Scene& S, int prim, int64_t tri, double u, double v) {
 }
 
 // Scene_::intersect (src/scene.cpp:319-345) with Mesh_Raw::surface_point.
-Hit scene_intersect(const Scene& S, V3 o, V3 d, double tmin, double tmax) {
+Hit scene_intersect(const Scene& S, V3 o, V3 d, double tmin, double tmax, int64_t* node_visits = nullptr,
+                    int64_t* tri_tests = nullptr) {
     Hit h;
-    const RayHit r = intersect(*S.accel, o, d, tmin, tmax);
+    const RayHit r = intersect(*S.accel, o, d, tmin, tmax, node_visits, tri_tests);
     if (r.prim < 0) {
         if (tmax < Inf) return h;
         if (!S.has_env) return h;
@@ -772,7 +779,8 @@ Hit scene_intersect(const Scene& S, V3 o, V3 d, double tmin, double tmax) {
 }
 
 // Scene::visible (include/lm/scene.h:343-364) for a finite sp1.
-bool visible(const Scene& S, const Geom& g1, const Geom& g2) {
+bool visible(const Scene& S, const Geom& g1, const Geom& g2, int64_t* node_visits = nullptr,
+             int64_t* tri_tests = nullptr) {
     V3 wo;
     double tmax;
     if (g2.infinite) {
@@ -782,7 +790,7 @@ bool visible(const Scene& S, const Geom& g1, const Geom& g2) {
         wo = normalize(g2.p - g1.p);
         tmax = length(g1.p - g2.p) * (1 - Eps);
     }
-    return intersect(*S.accel, g1.p, wo, Eps, tmax).prim < 0;
+    return intersect(*S.accel, g1.p, wo, Eps, tmax, node_visits, tri_tests).prim < 0;
 }
 
 // camera::pinhole primary ray (camera_pinhole.cpp:128-132)
@@ -1043,9 +1051,9 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
     const int w = scene->w, h = scene->h;
     const int64_t npix = int64_t(w) * h;
     if (P->nthreads > 0) omp_set_num_threads(P->nthreads);
-    int64_t n_ext = 0, n_shadow = 0, n_vert = 0;
+    int64_t n_ext = 0, n_shadow = 0, n_vert = 0, e_nv = 0, e_nt = 0, s_nv = 0, s_nt = 0;
     const int mode = P->sampling_mode;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_ext, n_shadow, n_vert)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_ext, n_shadow, n_vert, e_nv, e_nt, s_nv, s_nt)
     for (int64_t pix = 0; pix < npix; pix++) {
         const int x = int(pix % w), y = int(pix / w);
         const double dx = 1.0 / w, dy = 1.0 / h;
@@ -1107,7 +1115,7 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     }
                     if (ok) {
                         n_shadow++;
-                        if (visible(S, sp, gL)) {
+                        if (visible(S, sp, gL, &s_nv, &s_nt)) {
                             const V3 wo = -wL;
                             const V3 fs = mat_eval_delta(*m, sp, wi, wo, comp) * shading_normal_correction(sp, wi, wo);
                             if (!is_zero(fs)) {
@@ -1134,7 +1142,7 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     sw = sw * shading_normal_correction(sp, wi, wo);
                 }
                 n_ext++;
-                const Hit hit = scene_intersect(S, sp.p, wo, Eps, Inf);
+                const Hit hit = scene_intersect(S, sp.p, wo, Eps, Inf, &e_nv, &e_nt);
                 if (!hit.valid) break;
                 n_vert++;
                 thr = thr * sw;
@@ -1185,6 +1193,10 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
         stats->extension_rays += n_ext;
         stats->shadow_rays += n_shadow;
         stats->path_vertices += n_vert;
+        stats->extension_node_visits += e_nv;
+        stats->extension_tri_tests += e_nt;
+        stats->shadow_node_visits += s_nv;
+        stats->shadow_tri_tests += s_nt;
     }
     return 0;
 }
@@ -1353,11 +1365,11 @@ int orc_lt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, doub
 // A vertex of a subpath or a full path (Vert, include/lm/bidir.h:23-39).
 struct BVert {
     enum Kind { Camera, Light, Surface } kind = Surface;
-    Geom g;
+    Geom g;           // light::envconst vertices are infinite (g.infinite, g.wo into the scene)
     Mat mat{};
     int comp = 0;
     double ul = 0;    // the component's random number rescaled (a mixture's lobe selection)
-    int light = -1;   // area light of the primitive (-1: none)
+    int light = -1;   // area light of the primitive (-1: none); S.lights.size() = light::envconst
 };
 
 // Material::eval with eval_delta = false of component `comp`, with the
@@ -1386,6 +1398,12 @@ double mat_pdf_full(const Mat& m, const Geom& g, V3 wi, V3 wo, int comp) {
 struct BDPT {
     const Scene& S;
     int w, h;
+    double sph_r;   // radius of the scene's bounding sphere (Light_EnvConst::set_scene_bound)
+    bool is_env(const BVert& v) const { return v.kind == BVert::Light && v.light == int(S.lights.size()); }
+    // path::is_connectable_endpoint: the pinhole and area lights are, light::envconst is not
+    bool connectable(const BVert& v) const { return !is_env(v); }
+    // path::pdf_primary_ray of light::envconst: p_sel / (4 pi) / (pi r^2) (light_envconst.cpp:70-74)
+    double env_pdf_ray() const { return 1.0 / S.num_lights / (4 * Pi) / (Pi * sph_r * sph_r); }
     // camera::pinhole raster position and importance (camera_pinhole.cpp:102-109, 207-226)
     bool raster(V3 wo, double& rx, double& ry) const {
         const double ez = dot(wo, S.cw);
@@ -1404,6 +1422,7 @@ struct BDPT {
             double rx, ry;
             return raster(wo, rx, ry) ? V3(J(wo)) : V3(0);
         }
+        if (is_env(v)) return S.env_Le;   // Light_EnvConst::eval
         if (v.kind == BVert::Light) return dot(wo, v.g.n) <= 0 ? V3(0) : S.lights[v.light].Ke;
         const V3 f = eval_delta ? mat_eval_delta(v.mat, v.g, *wi, wo, v.comp)
                                 : mat_eval_full(v.mat, v.g, *wi, wo, v.comp, le);
@@ -1434,7 +1453,13 @@ struct BDPT {
     double pdf_pos(const BVert& v) const {
         return v.kind == BVert::Camera ? 1.0 : S.lights[v.light].invA / S.num_lights;
     }
-    static V3 dir(const BVert& a, const BVert& b) { return normalize(b.g.p - a.g.p); }
+    // Path::direction (bidir.h:137-150): from or toward a point at infinity
+    // along its direction
+    static V3 dir(const BVert& a, const BVert& b) {
+        if (a.g.infinite) return a.g.wo;
+        if (b.g.infinite) return -b.g.wo;
+        return normalize(b.g.p - a.g.p);
+    }
 
     // Path::eval_subpath_sampling_weight (bidir.h:239-282) of the first l
     // vertices of `p` seen from its endpoint `from0` (0 = light side).
@@ -1445,8 +1470,18 @@ struct BDPT {
             if (i < 0 || i >= n) return nullptr;
             return &p[light_side ? i : n - 1 - i];
         };
-        V3 a = V3(1.0 / (pdf_pos(*at(0)) * 1.0));
-        for (int i = 0; i < l - 1; i++) {
+        V3 a;
+        int i0 = 0;
+        if (connectable(*at(0))) {
+            a = V3(1.0 / (pdf_pos(*at(0)) * 1.0));
+        } else {
+            // the joint (direction, position) sample of light::envconst
+            // (bidir.h:248-258): f / (p_ray p_comp(v0) p_comp(v1))
+            const V3 d01 = dir(*at(0), *at(1));
+            a = eval(*at(0), nullptr, d01, light_side, false) / (env_pdf_ray() * pdf_comp(*at(1), -d01));
+            i0 = 1;
+        }
+        for (int i = i0; i < l - 1; i++) {
             const BVert* v = at(i);
             const BVert* vp = at(i - 1);
             const BVert* vn = at(i + 1);
@@ -1478,6 +1513,7 @@ struct BDPT {
         const int n = int(p.size()), t = n - s;
         if (s == 0) return !specular(p[0]);
         if (t == 0) return false;   // the pinhole is degenerate
+        if (s == 1 && !connectable(p[0])) return false;
         return !specular(p[s - 1]) && !specular(p[s]);
     }
     // Path::pdf_bidir (bidir.h:392-437)
@@ -1490,8 +1526,17 @@ struct BDPT {
                 if (i < 0 || i >= n) return nullptr;
                 return &p[light_side ? i : n - 1 - i];
             };
-            double q = pdf_pos(*at(0));
-            for (int i = 0; i < l - 1; i++) {
+            double q;
+            int i0 = 0;
+            if (connectable(*at(0))) {
+                q = pdf_pos(*at(0));
+            } else {
+                // convert_pdf_to_area(p_ray, v0, v1) p_comp(v1) (bidir.h:405-411)
+                const V3 d01 = dir(*at(0), *at(1));
+                q = env_pdf_ray() * geometry_term(at(0)->g, at(1)->g) * pdf_comp(*at(1), -d01);
+                i0 = 1;
+            }
+            for (int i = i0; i < l - 1; i++) {
                 const BVert* v = at(i);
                 const BVert* vp = at(i - 1);
                 const BVert* vn = at(i + 1);
@@ -1521,7 +1566,7 @@ struct BDPT {
 
 // Restates Renderer_BDPT::render (src/renderer/renderer_bdpt.cpp:339-411) with
 // scheduler::spi::sample (in passes of min(w*h, 2^18) samples) for area
-// lights and the pinhole camera: per sample
+// lights, light::envconst and the pinhole camera: per sample
 // an eye subpath and a light subpath of at most max_verts vertices each
 // (path::sample_subpath, bidir.h:489-548: no Russian roulette), every
 // (s, t) with 2 <= s + t <= max_verts connected (path::connect_subpaths,
@@ -1535,10 +1580,20 @@ struct BDPT {
 int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, double* film, OrcPTStats* stats) {
     Scene S;
     init_scene(S, scene, static_cast<Accel*>(accel));
-    if (S.lights.empty() || S.has_env) return -1;
+    if (S.num_lights == 0) return -1;
     const int w = scene->w, h = scene->h;
     const int64_t npix = int64_t(w) * h;
-    const BDPT B{S, w, h};
+    // scene bound for light::envconst (Light_EnvConst::set_scene_bound)
+    V3 bmin(Inf), bmax(-Inf);
+    for (int64_t k = 0; k < scene->num_tris * 3; k++) {
+        const V3 p = load3(scene->verts + 3 * k);
+        bmin = vmin(bmin, p);
+        bmax = vmax(bmax, p);
+    }
+    const V3 sph_c = (bmax + bmin) * .5;
+    const double sph_r = length(bmax - sph_c) * 1.01;
+    const BDPT B{S, w, h, sph_r};
+    const int env_index = int(S.lights.size());
     const int M = P->max_verts;
     const int min_verts = P->min_verts > 0 ? P->min_verts : 2;
     // samples per pass, as the GPU renderer: min(w*h, 2^18)
@@ -1550,12 +1605,15 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
         const uint32_t idx = uint32_t(i % per_pass);
         const uint32_t pass = uint32_t(P->pass_offset + i / per_pass);
         // subpath random walk (sample_subpath_from_endpoint, bidir.h:489-535)
-        auto walk = [&](std::vector<BVert>& p, V3 o, V3 d, uint32_t blk) {
+        // `eye`: a miss on an environment-lit scene ends the walk with the
+        // environment's vertex (Scene_::intersect returns it, bidir.h:525-533);
+        // a light walk's such vertex could never be connected and is dropped
+        auto walk = [&](std::vector<BVert>& p, V3 o, V3 d, uint32_t blk, bool eye) {
             while (int(p.size()) < M) {
                 if (p.size() > 1) {
                     // path::sample_direction at the last vertex: (vertex, block)[2..3]
                     const BVert& v = p.back();
-                    const V3 wi = normalize(p[p.size() - 2].g.p - v.g.p);
+                    const V3 wi = BDPT::dir(v, p[p.size() - 2]);
                     const Rand4 r = rand4(idx, pass, uint32_t(p.size() - 1), blk, P->seed);
                     V3 sw;
                     if (!mat_sample_direction(v.mat, v.g, wi, v.comp, r.u[2], r.u[3], v.ul, d, sw)) break;
@@ -1563,6 +1621,14 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
                 }
                 n_ext++;
                 const Hit hit = scene_intersect(S, o, d, Eps, Inf);
+                if (eye && hit.valid && hit.env) {
+                    BVert v;
+                    v.kind = BVert::Surface;   // an eye-subpath vertex; the light endpoint once connected
+                    v.g = hit.g;
+                    v.light = env_index;
+                    p.push_back(v);
+                    break;
+                }
                 if (!hit.valid || hit.env) break;
                 n_vert++;
                 const Rand4 rc = rand4(idx, pass, uint32_t(p.size()), blk, P->seed);
@@ -1576,9 +1642,30 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
                 p.push_back(v);
             }
         };
-        // light subpath: light::area::sample_ray (light_area.cpp:78-100)
+        // light subpath: light::area::sample_ray (light_area.cpp:78-100) or
+        // light::envconst::sample_ray (light_envconst.cpp:47-68: a uniform
+        // direction and a point of the disk of the scene's bounding sphere)
         std::vector<BVert> L, E;
         {
+            const Rand4 r0 = rand4(idx, pass, 0, 0, P->seed);
+            const Rand4 r1 = rand4(idx, pass, 0, 1, P->seed);
+            const int li = std::clamp(int(r0.u[0] * S.num_lights), 0, S.num_lights - 1);
+            if (li == env_index) {
+                const V3 d = sample_uniform_sphere(r1.u[0], r1.u[1]);
+                const double r = safe_sqrt(r0.u[2]) * sph_r, ang = 2 * Pi * r0.u[3];
+                V3 u, vb;
+                onb(d, u, vb);
+                const V3 o = sph_c + sph_r * (-d) + (u * (r * std::cos(ang)) + vb * (r * std::sin(ang)));
+                BVert v0;
+                v0.kind = BVert::Light;
+                v0.g = make_infinite(d);
+                v0.g.p = o;
+                v0.light = env_index;
+                L.push_back(v0);
+                if (M > 1) walk(L, o, d, 0, false);
+            }
+        }
+        if (L.empty()) {
             const Rand4 r0 = rand4(idx, pass, 0, 0, P->seed);
             const Rand4 r1 = rand4(idx, pass, 0, 1, P->seed);
             const int li = std::clamp(int(r0.u[0] * S.num_lights), 0, S.num_lights - 1);
@@ -1597,7 +1684,7 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
             onb(gn, u, vb);
             const V3 l = sample_cosine_weighted(r1.u[0], r1.u[1]);
             L.push_back(v0);
-            if (M > 1) walk(L, v0.g.p, u * l.x + vb * l.y + gn * l.z, 0);
+            if (M > 1) walk(L, v0.g.p, u * l.x + vb * l.y + gn * l.z, 0, false);
         }
         // eye subpath: camera::pinhole::sample_ray (camera_pinhole.cpp:134-140)
         {
@@ -1606,7 +1693,7 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
             c.kind = BVert::Camera;
             c.g = make_degenerated(S.pos);
             E.push_back(c);
-            walk(E, S.pos, primary_dir(S, r.u[0], r.u[1]), 2);
+            walk(E, S.pos, primary_dir(S, r.u[0], r.u[1]), 2, true);
         }
         const int nL = int(L.size()), nE = int(E.size());
         for (int s = 0; s <= nL; s++) {
@@ -1619,6 +1706,8 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
                     for (int j = t - 1; j >= 0; j--) p.push_back(E[j]);
                     p[0].kind = BVert::Light;
                 } else {
+                    // no connection to a point at infinity (bidir.h:585-588)
+                    if (L[s - 1].g.infinite || E[t - 1].g.infinite) continue;
                     n_shadow++;
                     if (!visible(S, L[s - 1].g, E[t - 1].g)) continue;
                     for (int j = 0; j < s; j++) p.push_back(L[j]);

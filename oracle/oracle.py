@@ -83,6 +83,10 @@ class OrcPTStats(ctypes.Structure):
         ('extension_rays', ctypes.c_int64),
         ('shadow_rays', ctypes.c_int64),
         ('path_vertices', ctypes.c_int64),
+        ('extension_node_visits', ctypes.c_int64),
+        ('extension_tri_tests', ctypes.c_int64),
+        ('shadow_node_visits', ctypes.c_int64),
+        ('shadow_tri_tests', ctypes.c_int64),
     ]
 
 
@@ -283,7 +287,10 @@ def pt_render(flat, spp: int, max_verts: int, sampling_mode: int = 2, rr_depth: 
         raise RuntimeError(f'orc_pt_render failed ({rc})')
     out = {'sum': fsum.reshape(flat.h, flat.w, 3), 'image': fsum.reshape(flat.h, flat.w, 3) / spp,
            'stats': {'samples': st.samples, 'extension_rays': st.extension_rays,
-                     'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices}}
+                     'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices,
+                     'extension_node_visits': st.extension_node_visits,
+                     'extension_tri_tests': st.extension_tri_tests,
+                     'shadow_node_visits': st.shadow_node_visits, 'shadow_tri_tests': st.shadow_tri_tests}}
     if fsq is not None:
         out['sumsq'] = fsq.reshape(flat.h, flat.w, 3)
     return out
