@@ -95,13 +95,16 @@ def run(width=1920, height=1080, seconds=30.0, ref_spp=4096, max_verts=16, chunk
     lo, hi = sc.bounds()
 
     def guided_run(opts):
+        # a variant may set its own roulette start ('rr_depth'); the rest are lmgpu_guide_opts fields
+        opts = dict(opts) if opts else None
+        rr = opts.pop('rr_depth', rr_g) if opts else rr_g
         g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-        gs, _, gst = sc.render_raw(render_time=seconds, max_verts=max_verts, rr_depth=rr_g, seed=6, guide=g,
-                                   guide_opts=opts)
+        gs, _, gst = sc.render_raw(render_time=seconds, max_verts=max_verts, rr_depth=rr, seed=6, guide=g,
+                                   guide_opts=opts or None)
         img = gs / gst['spp']
         e = errors(img)
         steps, _ = g.stats()
-        d = entry(gst, e, rr_depth=rr_g, adam_steps=int(steps), training_records=int(gst['guide_records']),
+        d = entry(gst, e, rr_depth=rr, adam_steps=int(steps), training_records=int(gst['guide_records']),
                   options=opts or 'defaults',
                   rmse_ratio_plain_over_guided=plain_d['rmse_corrected'] / e[1],
                   rmse_ratio_control_over_guided=ctrl_d['rmse_corrected'] / e[1])
