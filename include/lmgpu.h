@@ -297,13 +297,16 @@ int lmgpu_lt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* f
 
 /* renderer::bdpt (src/renderer/renderer_bdpt.cpp:339-411, include/lm/bidir.h):
  * per sample an eye subpath (pinhole, raster uniform over the image) and a
- * light subpath (area lights) of at most max_verts vertices each, without
+ * light subpath (area lights, or light::envconst's disk-and-direction sample,
+ * light_envconst.cpp:47-74) of at most max_verts vertices each, without
  * Russian roulette; every connection strategy (s, t), 2 <= s + t <= max_verts,
  * weighted by the power heuristic over all strategies of its full path
  * (Path::eval_mis_weight) and splatted at its raster position. Same film and
  * scheduler contract as lmgpu_lt_render (film value = film * (w*h) /
  * stats->samples; stats->spp counts the passes, of at most 2^18 samples).
- * Scenes with an environment light are rejected (LMGPU_ERR_UNSUPPORTED).
+ * An environment light is a non-connectable light endpoint (bidir.h:245-258,
+ * 399-411, 585-588): light subpaths may start on it and eye subpaths may end
+ * on it (strategy s = 0), but no connection reaches it.
  * max_verts in [2, 32]. Synchronous. */
 int lmgpu_bdpt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* film, int film_on_device,
                       lmgpu_pt_stats* stats, void* stream);

@@ -321,8 +321,9 @@ struct alignas(16) LVert {
     float4 f;   // pL, pL of the previous vertex, G(previous, this), T(j - 1)
 };
 
-constexpr int kFlagSpec = 1 << 8;        // this vertex's component is specular
-constexpr int kFlagSpecPrev = 1 << 9;    // the previous vertex's component is specular
+constexpr int kFlagSpec = 1 << 8;        // this vertex's component is specular (or it cannot be connected)
+constexpr int kFlagSpecPrev = 1 << 9;    // the previous vertex's component is specular (or it cannot be connected)
+constexpr int kFlagInfPrev = 1 << 10;    // path state: the previous vertex is the environment, at infinity
 
 struct BDConsts {
     int max_verts;
@@ -331,6 +332,8 @@ struct BDConsts {
     uint32_t pass;
     uint32_t seed_lo, seed_hi;
     int lstride;      // LVert slots per path (max_verts - 1)
+    f3 sph_c;         // bounding sphere of the scene (light::envconst rays)
+    float sph_r;
 };
 
 __device__ __forceinline__ uint64_t bd_seed(const BDConsts& c) { return (uint64_t(c.seed_hi) << 32) | c.seed_lo; }
@@ -353,8 +356,20 @@ __device__ __forceinline__ Geom geom_of(const LVert& v) {
     return make_on_surface(mk(v.a.x, v.a.y, v.a.z), mk(v.b.x, v.b.y, v.b.z), mk(v.c.x, v.c.y, v.c.z));
 }
 
-// Light subpath start: light::area::sample_ray (light_area.cpp:78-100) with
-// the uniform light selection; y_0 is stored, the ray goes to the queue.
+// light::envconst's path::pdf_primary_ray: p_sel / (4 pi) / (pi r^2)
+// (light_envconst.cpp:70-74, path.h:154-160).
+__device__ __forceinline__ float env_pdf_ray(const SceneDev& sc, const BDConsts& c) {
+    return 1.f / (float(sc.num_lights) * 4.f * kPi * kPi * c.sph_r * c.sph_r);
+}
+
+// Light subpath start with the uniform light selection: light::area::sample_ray
+// (light_area.cpp:78-100), or light::envconst::sample_ray (light_envconst.cpp:47-68:
+// a uniform direction d and a point of the disk of the scene's bounding
+// sphere facing it). y_0 is stored, the ray goes to the queue. The
+// environment's y_0 lies at infinity and is not connectable
+// (Light_EnvConst::is_connectable): its (d, disk point) sample carries the
+// joint pdf p_ray of y_0 and the direction to y_1 (bidir.h:248-258, 399-411),
+// so pL(y_0) = 1 and pL(y_1) = p_ray |cos| at y_1.
 __global__ __launch_bounds__(kBlock) void k_bd_lgen(SceneDev sc, PathState ps, Queues q, BDConsts c,
                                                     LVert* __restrict__ lv, int32_t* __restrict__ lcount) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -362,7 +377,45 @@ __global__ __launch_bounds__(kBlock) void k_bd_lgen(SceneDev sc, PathState ps, Q
     const uint64_t seed = bd_seed(c);
     const rand4 r0 = rng(uint32_t(i), c.pass, 0u, 0u, seed);
     const rand4 r1 = rng(uint32_t(i), c.pass, 0u, 1u, seed);
-    const int li = min(max(int(r0.u[0] * sc.num_area_lights), 0), sc.num_area_lights - 1);
+    const int li = min(max(int(r0.u[0] * sc.num_lights), 0), sc.num_lights - 1);
+    if (i == 0) {
+        q.counts[0] = c.npaths;
+        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npaths);
+    }
+    if (li >= sc.num_area_lights) {
+        const f3 d = sample_uniform_sphere(r1.u[0], r1.u[1]);
+        const float rr = sqrtf(fmaxf(0.f, r0.u[2])) * c.sph_r;
+        float sn, cs;
+        sincosf(2.f * kPi * r0.u[3], &sn, &cs);
+        f3 u, v;
+        onb(d, u, v);
+        const f3 org = c.sph_c - d * c.sph_r + u * (rr * cs) + v * (rr * sn);
+        const float pray = env_pdf_ray(sc, c);
+        LVert y;
+        y.a = make_float4(org.x, org.y, org.z, __int_as_float(-1 - li));
+        y.b = make_float4(d.x, d.y, d.z, __int_as_float(-1));
+        y.c = make_float4(d.x, d.y, d.z, 0.f);
+        y.d = make_float4(0.f, 0.f, 0.f, 0.f);
+        y.e = make_float4(1.f, 1.f, 1.f, __int_as_float(kFlagSpec));   // never connected
+        y.f = make_float4(1.f, 0.f, 0.f, 0.f);
+        lv[int64_t(i) * c.lstride] = y;
+        lcount[i] = 1;
+        // weight toward y_1: Le / p_ray (bidir.h:254-256)
+        const f3 w = sc.env_Le * (1.f / pray);
+        ps.thr[i] = make_float4(w.x, w.y, w.z, 1.f);          // pL(y_0) = 1
+        ps.L[i] = make_float4(org.x, org.y, org.z, pray);     // pL(y_1) = p_ray G(y_0, y_1)
+        ps.prev[i] = make_float4(1.f, 1.f, 0.f, 0.f);         // cos 1 at infinity, T(0) = 1
+        // strategy s = 1 is not samplable: y_1 sees its predecessor as specular
+        ps.info[i] = make_int4(1, kFlagSpec | kFlagInfPrev, 0, 0);
+        lmgpu_ray ray;
+        ray.org[0] = org.x; ray.org[1] = org.y; ray.org[2] = org.z;
+        ray.tmin = kEps;
+        ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+        ray.tmax = kInf;
+        q.ext[0][i] = ray;
+        q.ext_path[0][i] = i;
+        return;
+    }
     const DevLight dl = sc.lights[li];
     f3 org, nL;
     sample_light_point(sc, li, r0.u[1], r0.u[2], r0.u[3], org, nL);
@@ -395,10 +448,6 @@ __global__ __launch_bounds__(kBlock) void k_bd_lgen(SceneDev sc, PathState ps, Q
     ray.tmax = kInf;
     q.ext[0][i] = ray;
     q.ext_path[0][i] = i;
-    if (i == 0) {
-        q.counts[0] = c.npaths;
-        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npaths);
-    }
     // strategy (s, t) = (1, 1): y_0 seen by the pinhole; the only other
     // strategy of this two-vertex path is the eye ray hitting the light (s = 0)
     const Camera& cam = sc.cam;
@@ -467,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_bd_lshade(SceneDev sc, PathState ps,
                 const f3 wi = -wdir;
                 const f3 pprev = mk(prv.x, prv.y, prv.z);
                 const f3 dv = pos - pprev;
-                const float d2 = dot(dv, dv);
+                const float d2 = (info.y & kFlagInfPrev) ? 1.f : dot(dv, dv);   // no distance to infinity
                 const float Gin = acc.x * fabsf(dot(nsh, wdir)) / d2;   // G(y_{j-1}, y_j)
                 const float pL = prv.w * Gin;                           // area pdf from the light side
                 const rand4 r0 = rng(uint32_t(p), c.pass, uint32_t(j), 0u, seed);
@@ -684,6 +733,28 @@ __global__ __launch_bounds__(kBlock) void k_bd_eshade(SceneDev sc, PathState ps,
                         }
                     }
                 }
+            } else if (tri == -1 && sc.has_env) {
+                // s = 0 on light::envconst: the eye subpath leaves the scene and
+                // ends on the environment (bidir.h:525-533), z_{t-1} at infinity.
+                // Strategy s = 1 is not samplable (the environment cannot be
+                // connected), so 1/w = 1 + u1^2 u2^2 V(t-2) with
+                // u1 = pL(z_{t-1}) / pE(z_{t-1}) = 1 / (pdf(z_{t-2}) |cos|) and
+                // u2 = pL(z_{t-2}) / pE(z_{t-2}) = p_ray |cos| / pE(z_{t-2}).
+                const float4 thr4 = ps.thr[p], prv = ps.L[p], acc = ps.prev[p];
+                const int4 info = ps.info[p];
+                const int te = info.x + 1;
+                if (te >= c.min_verts) {
+                    const float u12 = env_pdf_ray(sc, c) / (prv.w * thr4.w);
+                    const float sumE = acc.y > 0.f ? u12 * u12 * acc.y : 0.f;
+                    const float wgt = 1.f / (1.f + sumE);
+                    const f3 C = mk(thr4.x, thr4.y, thr4.z) * sc.env_Le * wgt;
+                    if (!is_zero(C) && wgt >= 0.f && wgt <= 1.f) {
+                        double* f = film + 3 * int64_t(info.z);
+                        atomicAdd(f + 0, double(C.x));
+                        atomicAdd(f + 1, double(C.y));
+                        atomicAdd(f + 2, double(C.z));
+                    }
+                }
             }
         }
         const int se = wave_append(q.counts + out_q, emit_ext);
@@ -865,9 +936,7 @@ int Scene::bdpt_render(const lmgpu_lt_params* p, double* film_in, bool on_device
     if (!(p->render_time > 0) && p->num_samples <= 0)
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "num_samples must be positive");
     if (p->pass_offset < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "pass_offset must be >= 0");
-    if (dev.has_env)
-        return fail(LMGPU_ERR_UNSUPPORTED, "bidirectional path tracing with an environment light is not supported");
-    if (dev.num_area_lights == 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bidirectional path tracing needs an area light");
+    if (dev.num_lights == 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bidirectional path tracing needs a light");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
     LMGPU_TRY(ensure_workspace(npix));
@@ -895,6 +964,11 @@ int Scene::bdpt_render(const lmgpu_lt_params* p, double* film_in, bool on_device
     c.seed_lo = uint32_t(p->seed);
     c.seed_hi = uint32_t(p->seed >> 32);
     c.lstride = lstride;
+    // bounding sphere of the scene's AABB, radius x1.01 (light_envconst.cpp:38-42)
+    const f3 blo = mk(accel.root_lo[0], accel.root_lo[1], accel.root_lo[2]);
+    const f3 bhi = mk(accel.root_hi[0], accel.root_hi[1], accel.root_hi[2]);
+    c.sph_c = (blo + bhi) * .5f;
+    c.sph_r = length(bhi - c.sph_c) * 1.01f;
     LVert* lv = reinterpret_cast<LVert*>(blverts.p);
     int64_t processed = 0, passes = 0;
     uint32_t pass = uint32_t(p->pass_offset);

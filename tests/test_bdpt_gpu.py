@@ -25,9 +25,28 @@ def bdpt_batches(sc, npix, K, per, M, seed, min_verts=2):
     return np.asarray(out)
 
 
-@pytest.mark.parametrize('name', ['config0', 'mixture_room'])
+def room_env():
+    sc = scenes.config0()
+    sc.env_Le = (0.6, 0.7, 0.8)   # the open front lets the environment in
+    return sc
+
+
+def mixture_env():
+    sc = mixture_room_scene()
+    sc.env_Le = (0.5, 0.5, 0.5)
+    return sc
+
+
+SAME_STREAM = {'config0': scenes.config0, 'mixture_room': mixture_room_scene, 'room_env': room_env,
+               'mixture_env': mixture_env}
+
+
+@pytest.mark.parametrize('name', list(SAME_STREAM))
 def test_bdpt_same_stream_matches_oracle(device, name):
-    flat = (scenes.config0() if name == 'config0' else mixture_room_scene()).flatten()
+    """Same Philox streams as the oracle; room_env and mixture_env add
+    light::envconst (light paths from the bounding-sphere disk, eye paths
+    ending on the environment)."""
+    flat = SAME_STREAM[name]().flatten()
     flat.w = flat.h = 48
     flat.camera.aspect = 1.0
     npix = flat.w * flat.h
@@ -56,7 +75,7 @@ def test_bdpt_furnace_closed_form(device):
 
 
 SCENES = {'config0': scenes.config0, 'glass_lamp': glass_lamp_scene, 'smooth_sphere': smooth_sphere_scene,
-          'mixture_room': mixture_room_scene}
+          'mixture_room': mixture_room_scene, 'room_env': room_env, 'env_cube': env_cube_scene}
 
 
 @pytest.mark.parametrize('name', list(SCENES))
@@ -108,5 +127,3 @@ def test_renderer_bdpt_api(device):
     assert t['processed'] % 1024 == 0 and t['processed'] >= 1024
     with pytest.raises(_lib.LmError):
         sc.render_bdpt(num_samples=10, max_verts=1)
-    with pytest.raises(_lib.LmError):
-        GpuScene(env_cube_scene().flatten()).render_bdpt(num_samples=10, max_verts=4)
