@@ -141,6 +141,23 @@ int lmgpu_accel_intersect_host(const lmgpu_accel* accel, const lmgpu_ray* rays, 
 int lmgpu_accel_occluded(const lmgpu_accel* accel, const lmgpu_ray* rays, uint8_t* occluded, int64_t n,
                          void* stream);
 
+/* Allocation-free form of lmgpu_accel_intersect_host. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (a
+ * process with HIP graph capture in progress, tight serving loops issuing
+ * small ray batches, harnesses that forbid hipMalloc). Host pointers in and
+ * out; the rays, hits, ray counter and overflow word are staged through one
+ * static device array per GPU (LMGPU_NOALLOC_SCRATCH_BYTES, held until the
+ * process ends), so nothing is allocated or freed. At most
+ * LMGPU_NOALLOC_MAX_RAYS rays per call: a larger n returns
+ * LMGPU_ERR_INVALID_ARGUMENT before any device work. Every call on one GPU
+ * shares that GPU's array; calls from threads of one process are serialized
+ * by a per-device lock (overlapping uses of the array would race).
+ * Synchronous; the results equal lmgpu_accel_intersect_host's. */
+#define LMGPU_NOALLOC_MAX_RAYS 65536
+#define LMGPU_NOALLOC_SCRATCH_BYTES (LMGPU_NOALLOC_MAX_RAYS * 48 + 256)
+int lmgpu_accel_intersect_host_noalloc(const lmgpu_accel* accel, const lmgpu_ray* rays, lmgpu_hit* hits,
+                                       int64_t n);
+
 /* Synchronizes `stream`; returns LMGPU_ERR_STACK_OVERFLOW if a traversal on
  * this accel exceeded its stack since the last call. Cannot happen when
  * depth * (width - 1) <= stack_capacity (see lmgpu_accel_info). */

@@ -7,6 +7,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -144,6 +145,77 @@ int lmgpu_accel_status(const lmgpu_accel* accel, void* stream) {
 }
 
 }  // extern "C"
+
+// ---- allocation-free host ray cast -------------------------------------------
+// lmgpu_accel_intersect_host_noalloc stages rays and hits through `scratch`,
+// a file-scope device array (one copy per GPU, held until the process ends):
+//   [0, 32*M)          rays  (lmgpu_ray, M = LMGPU_NOALLOC_MAX_RAYS)
+//   [32*M, 48*M)       hits  (lmgpu_hit)
+//   [48*M, 48*M + 8)   ray counter of the persistent traversal (zeroed by its launch)
+//   [48*M + 8, +4)     traversal overflow word (cleared before each call)
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that address is used from then on.
+namespace lmgpu {
+
+constexpr size_t kNoallocRayBytes = size_t(LMGPU_NOALLOC_MAX_RAYS) * sizeof(lmgpu_ray);
+constexpr size_t kNoallocHitBytes = size_t(LMGPU_NOALLOC_MAX_RAYS) * sizeof(lmgpu_hit);
+static_assert(kNoallocRayBytes + kNoallocHitBytes + 256 == size_t(LMGPU_NOALLOC_SCRATCH_BYTES),
+              "scratch layout and LMGPU_NOALLOC_SCRATCH_BYTES disagree");
+
+static __device__ __align__(256) unsigned char scratch[LMGPU_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoallocMaxDevices = 64;
+static std::mutex g_noalloc_mu[kNoallocMaxDevices];
+static unsigned char* g_noalloc_ptr[kNoallocMaxDevices];  // device address of `scratch` per device
+
+// Device address of `scratch` on `dev` (the current device); called with
+// g_noalloc_mu[dev] held.
+static int noalloc_scratch(int dev, unsigned char** out) {
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        LMGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(LMGPU_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return LMGPU_OK;
+}
+
+}  // namespace lmgpu
+
+extern "C" int lmgpu_accel_intersect_host_noalloc(const lmgpu_accel* accel, const lmgpu_ray* rays,
+                                                  lmgpu_hit* hits, int64_t n) {
+    using namespace lmgpu;
+    if (!accel) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel is NULL");
+    if (n < 0 || (n > 0 && (!rays || !hits))) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad ray buffers");
+    if (n > LMGPU_NOALLOC_MAX_RAYS)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT,
+                    "at most " + std::to_string(LMGPU_NOALLOC_MAX_RAYS) +
+                        " rays per call (static staging area); got " + std::to_string(n));
+    if (!accel->a.built) return fail(LMGPU_ERR_INVALID_ARGUMENT, "accel is not built");
+    if (n == 0) return LMGPU_OK;
+    int dev = 0;
+    LMGPU_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoallocMaxDevices) return fail(LMGPU_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu[dev]);
+    unsigned char* base = nullptr;
+    LMGPU_TRY(noalloc_scratch(dev, &base));
+    auto* drays = reinterpret_cast<lmgpu_ray*>(base);
+    auto* dhits = reinterpret_cast<lmgpu_hit*>(base + kNoallocRayBytes);
+    auto* next = reinterpret_cast<unsigned long long*>(base + kNoallocRayBytes + kNoallocHitBytes);
+    auto* status = reinterpret_cast<int32_t*>(next + 1);
+    hipStream_t s = nullptr;
+    LMGPU_HIP(hipMemsetAsync(status, 0, sizeof(int32_t), s));
+    LMGPU_HIP(hipMemcpyAsync(drays, rays, sizeof(lmgpu_ray) * size_t(n), hipMemcpyHostToDevice, s));
+    LMGPU_TRY(accel->a.trace_with(drays, dhits, nullptr, n, false, nullptr, next, status, s));
+    LMGPU_HIP(hipMemcpyAsync(hits, dhits, sizeof(lmgpu_hit) * size_t(n), hipMemcpyDeviceToHost, s));
+    int32_t overflow = 0;
+    LMGPU_HIP(hipMemcpyAsync(&overflow, status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LMGPU_HIP(hipStreamSynchronize(s));
+    if (overflow) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded");
+    return LMGPU_OK;
+}
 
 // ---- serialization ------------------------------------------------------------
 // A built accel as bytes: a 128-byte little-endian header followed by the

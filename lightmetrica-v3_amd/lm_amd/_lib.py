@@ -35,7 +35,8 @@ EXPORTED = [
     'lmgpu_last_error', 'lmgpu_version', 'lmgpu_device_count',
     'lmgpu_accel_create', 'lmgpu_accel_destroy', 'lmgpu_accel_build', 'lmgpu_accel_info_get',
     'lmgpu_accel_intersect', 'lmgpu_accel_intersect_host', 'lmgpu_accel_occluded', 'lmgpu_accel_status',
-    'lmgpu_accel_intersect_counted', 'lmgpu_accel_serialize', 'lmgpu_accel_deserialize',
+    'lmgpu_accel_intersect_counted', 'lmgpu_accel_intersect_host_noalloc', 'lmgpu_accel_serialize',
+    'lmgpu_accel_deserialize',
     'lmgpu_scene_create', 'lmgpu_scene_destroy', 'lmgpu_scene_accel', 'lmgpu_pt_render',
     'lmgpu_guide_create', 'lmgpu_guide_destroy', 'lmgpu_guide_params', 'lmgpu_guide_infer',
     'lmgpu_guide_sample', 'lmgpu_guide_pdf', 'lmgpu_guide_train', 'lmgpu_guide_stats', 'lmgpu_pt_render_guided',
@@ -193,6 +194,7 @@ def lib():
         L.lmgpu_accel_info_get.argtypes = [vp, ctypes.POINTER(AccelInfo)]
         L.lmgpu_accel_intersect.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_accel_intersect_host.argtypes = [vp, vp, vp, i64]
+        L.lmgpu_accel_intersect_host_noalloc.argtypes = [vp, vp, vp, i64]
         L.lmgpu_accel_occluded.argtypes = [vp, vp, vp, i64, vp]
         L.lmgpu_accel_status.argtypes = [vp, vp]
         L.lmgpu_accel_intersect_counted.argtypes = [vp, vp, vp, i64, ctypes.POINTER(i64), ctypes.POINTER(i64), vp]
