@@ -351,6 +351,10 @@ __global__ void k_b_reset(int32_t* counts, int out_q) {
 
 inline unsigned nblk(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
+// Graph replays (two iterations each) the host enqueues between looks at the
+// device's progress.
+constexpr int64_t kChunkReplays = 16;
+
 template <class K>
 int occupancy_grid(K kernel, int block) {
     int dev = 0, cus = 0, occ = 0;
@@ -403,6 +407,10 @@ struct Batch {
     DevBuf<lmgpu_hit> hit;
     DevBuf<float4> shc;
     DevBuf<BatchCtl> ctl;
+    BatchCtl* ctl_mirror = nullptr;   // pinned host copies of ctl, one per chunk parity
+    ~Batch() {
+        if (ctl_mirror) (void)hipHostFree(ctl_mirror);
+    }
     DevBuf<unsigned long long> next;
     // autotuned choice
     int tuned_width = 0, tuned_cap = 0, tuned_block = 0;
@@ -662,14 +670,42 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
     LMGPU_HIP(hipEventCreate(&e1));
     LMGPU_HIP(hipEventRecord(e0, s));
     hipLaunchKernelGGL(k_b_clock, dim3(1), dim3(64), 0, s, ctl.p);
-    // No polling: the replays up to the bound are enqueued back to back; once
-    // the device has accumulated every item its kernels return at once
-    // (k_b_regen), and the host synchronizes once, after the last replay.
+    // The host never stalls the device: replays go out in chunks and the host
+    // stays one chunk ahead. After each chunk the stream copies the control
+    // block into pinned memory and records an event; before deciding on the
+    // next chunk the host blocks on the event of the chunk before (no
+    // stream synchronization, no spinning) and stops once that copy shows no
+    // work left. Replays already queued when the work runs out find the
+    // remaining-work counter at 0 and return at entry (k_b_regen). The
+    // iteration bound turns a stall into an error.
     int rc = LMGPU_OK;
-    for (int64_t r = 0; r < (iter_cap + 1) / 2; r++) {
-        if ((e = hipGraphLaunch(ge, s)) != hipSuccess) {
-            rc = fail(LMGPU_ERR_DEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
+    if (!ctl_mirror) {
+        LMGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctl_mirror), 2 * sizeof(BatchCtl), hipHostMallocDefault));
+    }
+    hipEvent_t chunk_ev[2];
+    LMGPU_HIP(hipEventCreateWithFlags(&chunk_ev[0], hipEventBlockingSync | hipEventDisableTiming));
+    LMGPU_HIP(hipEventCreateWithFlags(&chunk_ev[1], hipEventBlockingSync | hipEventDisableTiming));
+    const int64_t max_replays = (iter_cap + 1) / 2;
+    int64_t enqueued = 0;
+    for (int64_t k = 0; enqueued < max_replays; k++) {
+        const int64_t nrep = std::min<int64_t>(kChunkReplays, max_replays - enqueued);
+        for (int64_t r = 0; r < nrep && rc == LMGPU_OK; r++)
+            if ((e = hipGraphLaunch(ge, s)) != hipSuccess)
+                rc = fail(LMGPU_ERR_DEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
+        if (rc != LMGPU_OK) break;
+        enqueued += nrep;
+        if ((e = hipMemcpyAsync(ctl_mirror + (k & 1), ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, s)) !=
+                hipSuccess ||
+            (e = hipEventRecord(chunk_ev[k & 1], s)) != hipSuccess) {
+            rc = fail(LMGPU_ERR_DEVICE, std::string("batch progress copy: ") + hipGetErrorString(e));
             break;
+        }
+        if (k >= 1) {
+            if ((e = hipEventSynchronize(chunk_ev[(k - 1) & 1])) != hipSuccess) {
+                rc = fail(LMGPU_ERR_DEVICE, std::string("batch progress: ") + hipGetErrorString(e));
+                break;
+            }
+            if (ctl_mirror[(k - 1) & 1].remaining == 0) break;
         }
     }
     BatchCtl hc{};
@@ -679,7 +715,11 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
             rc = fail(LMGPU_ERR_DEVICE, std::string("batch run: ") + hipGetErrorString(e));
         else if (hc.remaining != 0)
             rc = fail(LMGPU_ERR_FAILED_TO_RENDER, "batch did not finish within its iteration bound");
+    } else {
+        (void)hipStreamSynchronize(s);
     }
+    (void)hipEventDestroy(chunk_ev[0]);
+    (void)hipEventDestroy(chunk_ev[1]);
     const int64_t it = int64_t(hc.iterations);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
