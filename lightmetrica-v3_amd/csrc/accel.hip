@@ -37,6 +37,8 @@ namespace lmgpu {
 // as much as a wide-node visit in the SAH collapse.
 constexpr int kDefaultLeafSize = 2;
 constexpr float kDefaultSahTriCost = 1.0f;
+// Refit levels launched between two reads of the frontier size.
+constexpr int kRefitLevelsPerSync = 6;
 
 // ---------------------------------------------------------------------------
 // build kernels
@@ -194,6 +196,26 @@ __global__ __launch_bounds__(256) void k_lbvh(const uint64_t* __restrict__ k, in
     bn[i] = node;
 }
 
+// One atomic per wave: a lane with `want` gets a slot of *counter (-1
+// otherwise). Every lane of the wave must call it.
+__device__ __forceinline__ int32_t wave_alloc(int32_t* counter, bool want, int32_t count = 1) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(mask) - 1;
+    // inclusive prefix of the counts of the wanting lanes, in lane order
+    int32_t c = want ? count : 0, inc = c;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    const int32_t total = __shfl(inc, 63);
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    return want ? base + inc - c : -1;
+}
+
 // Bottom-up refit without device-scope fences (which write back the
 // per-XCD L2s): level-synchronous over a frontier. A node enters the
 // frontier of launch k+1 when its second child finished in launch k (atomic
@@ -206,13 +228,19 @@ __global__ __launch_bounds__(256) void k_refit_leaves(int64_t n, const int32_t* 
                                                       int32_t* __restrict__ flags, int32_t* __restrict__ frontier,
                                                       int32_t* __restrict__ fcount) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int32_t ti = order[j];
-    leaf_lo[j] = tlo[ti];
-    leaf_hi[j] = thi[ti];
-    if (n == 1) return;
-    const int32_t p = parent_leaf[j];
-    if (atomicAdd(&flags[p], 1) == 1) frontier[atomicAdd(fcount, 1)] = p;
+    bool push = false;
+    int32_t p = -1;
+    if (j < n) {
+        const int32_t ti = order[j];
+        leaf_lo[j] = tlo[ti];
+        leaf_hi[j] = thi[ti];
+        if (n > 1) {
+            p = parent_leaf[j];
+            push = atomicAdd(&flags[p], 1) == 1;
+        }
+    }
+    const int32_t k = wave_alloc(fcount, push);
+    if (push) frontier[k] = p;
 }
 
 __device__ __forceinline__ float half_area(float4 lo, float4 hi) {
@@ -298,22 +326,34 @@ __global__ __launch_bounds__(256) void k_refit_nodes(const int32_t* __restrict__
                                                      float4* __restrict__ node_hi, int32_t* __restrict__ flags,
                                                      int32_t* __restrict__ fout, int32_t* __restrict__ fout_count,
                                                      SahDP dp) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= *fin_count) return;
-    const int32_t p = fin[idx];
-    const BNode nd = bn[p];
-    const float4 l0 = nd.left < 0 ? leaf_lo[~nd.left] : node_lo[nd.left];
-    const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
-    const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
-    const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
-    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
-    node_lo[p] = lo;
-    node_hi[p] = hi;
-    if (dp.cost) sah_tables(dp, p, nd, half_area(lo, hi), l0, h0, l1, h1);
-    if (p == root) return;
-    const int32_t q = parent_int[p];
-    if (atomicAdd(&flags[q], 1) == 1) fout[atomicAdd(fout_count, 1)] = q;
+    const int cnt = *fin_count;
+    const int stride = gridDim.x * blockDim.x;
+    // whole waves run the same number of iterations (wave-level appends)
+    const int rounds = (cnt + stride - 1) / stride;
+    for (int r = 0; r < rounds; r++) {
+        const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool push = false;
+        int32_t q = -1;
+        if (idx < cnt) {
+            const int32_t p = fin[idx];
+            const BNode nd = bn[p];
+            const float4 l0 = nd.left < 0 ? leaf_lo[~nd.left] : node_lo[nd.left];
+            const float4 h0 = nd.left < 0 ? leaf_hi[~nd.left] : node_hi[nd.left];
+            const float4 l1 = nd.right < 0 ? leaf_lo[~nd.right] : node_lo[nd.right];
+            const float4 h1 = nd.right < 0 ? leaf_hi[~nd.right] : node_hi[nd.right];
+            const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+            const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+            node_lo[p] = lo;
+            node_hi[p] = hi;
+            if (dp.cost) sah_tables(dp, p, nd, half_area(lo, hi), l0, h0, l1, h1);
+            if (p != root) {
+                q = parent_int[p];
+                push = atomicAdd(&flags[q], 1) == 1;
+            }
+        }
+        const int32_t k = wave_alloc(fout_count, push);
+        if (push) fout[k] = q;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_reorder(int64_t n, const int32_t* __restrict__ order,
@@ -337,54 +377,63 @@ struct CollapseItem {
 // Writes wide node `it.wnode` from its nc <= W binary refs (each a leaf when
 // is_leaf(ref), else a wide child). The wide children of one node take
 // consecutive slots, so siblings share cache lines; they become next-level
-// items.
+// items. Every lane of the wave calls it (`active` false for lanes without an
+// item): slots and queue positions are taken with one atomic per wave and
+// the SAH and leaf statistics are summed over the wave first.
 template <int W, class IsLeaf>
-__device__ __forceinline__ void emit_wide_node(const CollapseItem& it, const int32_t (&ref)[W], int nc, IsLeaf is_leaf,
-                                               const BNode* __restrict__ bn, const float4* __restrict__ node_lo,
-                                               const float4* __restrict__ node_hi, const float4* __restrict__ leaf_lo,
-                                               const float4* __restrict__ leaf_hi, CollapseItem* __restrict__ out,
-                                               int32_t* __restrict__ n_out, int32_t* __restrict__ node_counter,
-                                               WNode<W>* __restrict__ wnodes, double* sah_acc, int32_t* leaf_counter) {
+__device__ __forceinline__ void emit_wide_node(bool active, const CollapseItem& it, const int32_t (&ref)[W], int nc,
+                                               IsLeaf is_leaf, const BNode* __restrict__ bn,
+                                               const float4* __restrict__ node_lo, const float4* __restrict__ node_hi,
+                                               const float4* __restrict__ leaf_lo, const float4* __restrict__ leaf_hi,
+                                               CollapseItem* __restrict__ out, int32_t* __restrict__ n_out,
+                                               int32_t* __restrict__ node_counter, WNode<W>* __restrict__ wnodes,
+                                               double* sah_acc, int32_t* leaf_counter) {
     int nint = 0;
-    for (int c = 0; c < nc; c++) nint += is_leaf(ref[c]) ? 0 : 1;
-    int32_t slot = 0, q = 0;
-    if (nint) {
-        slot = atomicAdd(node_counter, nint);
-        q = atomicAdd(n_out, nint);
-    }
-    WNode<W> node;
+    if (active)
+        for (int c = 0; c < nc; c++) nint += is_leaf(ref[c]) ? 0 : 1;
+    int32_t slot = wave_alloc(node_counter, nint > 0, nint);
+    int32_t q = wave_alloc(n_out, nint > 0, nint);
     double sah = 0.0;
     int nleaf = 0;
-    for (int c = 0; c < W; c++) {
-        if (c < nc) {
-            const int32_t r = ref[c];
-            const float4 lo = r < 0 ? leaf_lo[~r] : node_lo[r];
-            const float4 hi = r < 0 ? leaf_hi[~r] : node_hi[r];
-            node.lox[c] = lo.x; node.loy[c] = lo.y; node.loz[c] = lo.z;
-            node.hix[c] = hi.x; node.hiy[c] = hi.y; node.hiz[c] = hi.z;
-            const int cnt = r < 0 ? 1 : bn[r].last - bn[r].first + 1;
-            const double a = double(half_area(lo, hi));
-            if (is_leaf(r)) {
-                node.child[c] = r < 0 ? ~r : bn[r].first;
-                node.count[c] = cnt;
-                sah += a * cnt;
-                nleaf++;
+    if (active) {
+        WNode<W> node;
+        for (int c = 0; c < W; c++) {
+            if (c < nc) {
+                const int32_t r = ref[c];
+                const float4 lo = r < 0 ? leaf_lo[~r] : node_lo[r];
+                const float4 hi = r < 0 ? leaf_hi[~r] : node_hi[r];
+                node.lox[c] = lo.x; node.loy[c] = lo.y; node.loz[c] = lo.z;
+                node.hix[c] = hi.x; node.hiy[c] = hi.y; node.hiz[c] = hi.z;
+                const int cnt = r < 0 ? 1 : bn[r].last - bn[r].first + 1;
+                const double a = double(half_area(lo, hi));
+                if (is_leaf(r)) {
+                    node.child[c] = r < 0 ? ~r : bn[r].first;
+                    node.count[c] = cnt;
+                    sah += a * cnt;
+                    nleaf++;
+                } else {
+                    node.child[c] = slot;
+                    node.count[c] = 0;
+                    sah += a;
+                    out[q++] = CollapseItem{r, slot++};
+                }
             } else {
-                node.child[c] = slot;
-                node.count[c] = 0;
-                sah += a;
-                out[q++] = CollapseItem{r, slot++};
+                node.lox[c] = node.loy[c] = node.loz[c] = 1e30f;
+                node.hix[c] = node.hiy[c] = node.hiz[c] = -1e30f;
+                node.child[c] = 0;
+                node.count[c] = -1;
             }
-        } else {
-            node.lox[c] = node.loy[c] = node.loz[c] = 1e30f;
-            node.hix[c] = node.hiy[c] = node.hiz[c] = -1e30f;
-            node.child[c] = 0;
-            node.count[c] = -1;
         }
+        wnodes[it.wnode] = node;
     }
-    wnodes[it.wnode] = node;
-    atomicAdd(sah_acc, sah);
-    atomicAdd(leaf_counter, nleaf);
+    for (int off = 32; off > 0; off >>= 1) {
+        sah += __shfl_xor(sah, off);
+        nleaf += __shfl_xor(nleaf, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(sah_acc, sah);
+        atomicAdd(leaf_counter, nleaf);
+    }
 }
 
 // SAH collapse: one work item per wide node; its children are the roots the
@@ -401,12 +450,14 @@ __global__ __launch_bounds__(256) void k_collapse_sah(const CollapseItem* __rest
                                                       const uint8_t* __restrict__ choice, WNode<W>* __restrict__ wnodes,
                                                       double* sah_acc, int32_t* leaf_counter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= *n_in) return;
-    const CollapseItem it = in[idx];
+    const bool active = idx < *n_in;
+    const CollapseItem it = active ? in[idx] : CollapseItem{~0, 0};
     auto is_leaf = [&](int32_t r) { return r < 0 || choice[int64_t(r) * W] == 0xFF; };
     int32_t ref[W];
     int nc = 0;
-    if (is_leaf(it.bref)) {
+    if (!active) {
+        // no item: only takes part in the wave's allocations
+    } else if (is_leaf(it.bref)) {
         ref[nc++] = it.bref;   // the whole tree is one leaf (tiny scenes)
     } else {
         int32_t sr[W];
@@ -432,8 +483,8 @@ __global__ __launch_bounds__(256) void k_collapse_sah(const CollapseItem* __rest
             sr[sp] = bm.left; sj[sp++] = c;
         }
     }
-    emit_wide_node<W>(it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter, wnodes,
-                      sah_acc, leaf_counter);
+    emit_wide_node<W>(active, it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter,
+                      wnodes, sah_acc, leaf_counter);
 }
 
 // Greedy collapse: one work item per wide node; open the largest-area child
@@ -448,14 +499,16 @@ __global__ __launch_bounds__(256) void k_collapse(const CollapseItem* __restrict
                                                   WNode<W>* __restrict__ wnodes, int leaf_size, double* sah_acc,
                                                   int32_t* leaf_counter) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= *n_in) return;
-    const CollapseItem it = in[idx];
+    const bool active = idx < *n_in;
+    const CollapseItem it = active ? in[idx] : CollapseItem{~0, 0};
     int32_t ref[W];
     int nc = 0;
     auto cnt_of = [&](int32_t r) -> int { return r < 0 ? 1 : bn[r].last - bn[r].first + 1; };
     auto lo_of = [&](int32_t r) { return r < 0 ? leaf_lo[~r] : node_lo[r]; };
     auto hi_of = [&](int32_t r) { return r < 0 ? leaf_hi[~r] : node_hi[r]; };
-    if (it.bref < 0 || cnt_of(it.bref) <= leaf_size) {
+    if (!active) {
+        // no item: only takes part in the wave's allocations
+    } else if (it.bref < 0 || cnt_of(it.bref) <= leaf_size) {
         ref[nc++] = it.bref;   // whole range is one leaf (tiny scenes)
     } else {
         ref[nc++] = bn[it.bref].left;
@@ -476,8 +529,8 @@ __global__ __launch_bounds__(256) void k_collapse(const CollapseItem* __restrict
         }
     }
     auto is_leaf = [&](int32_t r) { return r < 0 || cnt_of(r) <= leaf_size; };
-    emit_wide_node<W>(it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter, wnodes,
-                      sah_acc, leaf_counter);
+    emit_wide_node<W>(active, it, ref, nc, is_leaf, bn, node_lo, node_hi, leaf_lo, leaf_hi, out, n_out, node_counter,
+                      wnodes, sah_acc, leaf_counter);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,6 +580,10 @@ __global__ __launch_bounds__(256) void k_quantise(const WNode<W>* __restrict__ i
                                                   int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i < n) out[i] = quantise_node<W>(in[i]);
+}
+
+__global__ void k_zero_i32_1(int32_t* c) {
+    if (threadIdx.x == 0) *c = 0;
 }
 
 __global__ void k_zero_counter(unsigned long long* c) {
@@ -665,6 +722,10 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
         LMGPU_HIP(hipMemsetAsync(counters.p, 0, 2 * sizeof(int32_t), s));
         hipLaunchKernelGGL(k_refit_leaves, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, tlo.p, thi.p,
                            parent_leaf.p, leaf_lo.p, leaf_hi.p, flags.p, fr[0], counters.p);
+        // A frontier is never larger than the one before it (each parent
+        // enters from exactly one child of the previous frontier), so the
+        // grid sized for the last count read back serves the next
+        // kRefitLevelsPerSync levels, each launch reading its own count.
         int cur = 0;
         int levels_refit = 0;
         while (n > 1) {
@@ -672,12 +733,15 @@ int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_b
             LMGPU_HIP(hipMemcpyAsync(&cnt, counters.p + cur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             LMGPU_HIP(hipStreamSynchronize(s));
             if (cnt == 0) break;
-            LMGPU_HIP(hipMemsetAsync(counters.p + (cur ^ 1), 0, sizeof(int32_t), s));
-            hipLaunchKernelGGL(k_refit_nodes, dim3(nblocks(cnt, 256)), dim3(256), 0, s, fr[cur], counters.p + cur,
-                               bn.p, root_ref, parent_int.p, leaf_lo.p, leaf_hi.p, node_lo.p, node_hi.p, flags.p,
-                               fr[cur ^ 1], counters.p + (cur ^ 1), dp);
-            cur ^= 1;
-            if (++levels_refit > 4096) return fail(LMGPU_ERR_DEVICE, "BVH refit did not terminate");
+            for (int l = 0; l < kRefitLevelsPerSync; l++) {
+                hipLaunchKernelGGL(k_zero_i32_1, dim3(1), dim3(64), 0, s, counters.p + (cur ^ 1));
+                hipLaunchKernelGGL(k_refit_nodes, dim3(std::min(nblocks(cnt, 256), 2048u)), dim3(256), 0, s, fr[cur],
+                                   counters.p + cur, bn.p, root_ref, parent_int.p, leaf_lo.p, leaf_hi.p, node_lo.p,
+                                   node_hi.p, flags.p, fr[cur ^ 1], counters.p + (cur ^ 1), dp);
+                cur ^= 1;
+            }
+            levels_refit += kRefitLevelsPerSync;
+            if (levels_refit > 4096) return fail(LMGPU_ERR_DEVICE, "BVH refit did not terminate");
         }
     }
     LMGPU_TRY(tris.alloc(n));

@@ -265,10 +265,17 @@ LMGPU_HD QNode<W> quantise_node(const WNode<W>& n) {
             }
         if (!(mn <= mx)) mn = mx = 0.f;   // no child (not produced by the builder)
         org[a] = mn;
-        // smallest e with fma(255, 2^e, mn) >= mx; steps of at least 2^-126
+        // smallest e with fma(255, 2^e, mn) >= mx; steps of at least 2^-126.
+        // Start from ceil(log2(span / 255)), read off the binary exponent:
+        // x = m 2^k with m in [0.5, 1) gives ceil(log2 x) = k, or k - 1 when
+        // m = 0.5 exactly.
         int e = -126;
         const double span = double(mx) - double(mn);
-        if (span > 0.0) e = std::max(-126, int(std::ceil(std::log2(span / 255.0))));
+        if (span > 0.0) {
+            int k = 0;
+            const double m = std::frexp(span / 255.0, &k);
+            e = std::max(-126, m == 0.5 ? k - 1 : k);
+        }
         while (e < 127 && std::fma(255.f, std::ldexp(1.f, e), mn) < mx) e++;
         ebits |= uint32_t(e + 127) << (8 * a);
         uint32_t ql[W / 4] = {}, qh[W / 4] = {};
@@ -276,10 +283,11 @@ LMGPU_HD QNode<W> quantise_node(const WNode<W>& n) {
         for (int c = 0; c < W; c++) {
             int l = 0, h = 0;
             if (n.count[c] >= 0) {
-                l = int(std::floor((double(lo[a][c]) - double(mn)) / double(step)));
+                // division by the power-of-two step is an exact scaling
+                l = int(std::floor(std::ldexp(double(lo[a][c]) - double(mn), -e)));
                 l = std::min(std::max(l, 0), 255);
                 while (l > 0 && std::fma(float(l), step, mn) > lo[a][c]) l--;
-                h = int(std::ceil((double(hi[a][c]) - double(mn)) / double(step)));
+                h = int(std::ceil(std::ldexp(double(hi[a][c]) - double(mn), -e)));
                 h = std::min(std::max(h, 0), 255);
                 while (h < 255 && std::fma(float(h), step, mn) < hi[a][c]) h++;
             }

@@ -3,8 +3,9 @@
 //
 // Scan: reduce-then-scan over tiles of 4096 elements (256 threads x 16),
 // each tile scanned in 16 rounds of a 256-wide block scan (wave shuffles,
-// one LDS exchange of the 4 wave totals); the tile sums are scanned
-// recursively.
+// one LDS exchange of the 4 wave totals); each block adds the sum of the
+// tile sums before its own (up to 4096 tiles; beyond, the tile sums are
+// scanned recursively).
 //
 // Sort: one pass per 8-bit digit, three launches per pass and no spin-waits:
 //   upsweep    per-tile digit histogram (LDS atomics) -> hist[digit][tile],
@@ -85,6 +86,26 @@ __global__ __launch_bounds__(kT) void k_scan_down(const T* in, int64_t m, const 
     }
 }
 
+// As k_scan_down, with each tile's offset summed by its own block from the
+// tile sums before it (for up to kTile tiles; saves the scan of the sums).
+template <class T>
+__global__ __launch_bounds__(kT) void k_scan_down_sums(const T* in, int64_t m, const T* __restrict__ sums, T* out) {
+    __shared__ T lds[4];
+    T part = 0;
+    for (unsigned b = threadIdx.x; b < blockIdx.x; b += kT) part += sums[b];
+    T carry;
+    (void)block_scan_256<T>(part, carry, lds);
+    const int64_t base = int64_t(blockIdx.x) * kTile;
+    for (int i = 0; i < kItems; i++) {
+        const int64_t idx = base + int64_t(i) * kT + threadIdx.x;
+        const T x = idx < m ? in[idx] : T(0);
+        T total;
+        const T ex = block_scan_256<T>(x, total, lds);
+        if (idx < m) out[idx] = carry + ex;
+        carry += total;
+    }
+}
+
 // Elements of temporary storage scan_rec needs for m elements.
 int64_t scan_temp(int64_t m) {
     const int64_t nb = (m + kTile - 1) / kTile;
@@ -103,6 +124,10 @@ void scan_rec(const T* in, T* out, int64_t m, T* temp, hipStream_t s) {
     }
     T* sums = temp;
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kT), 0, s, in, m, sums);
+    if (nb <= unsigned(kTile)) {
+        hipLaunchKernelGGL(k_scan_down_sums<T>, dim3(nb), dim3(kT), 0, s, in, m, static_cast<const T*>(sums), out);
+        return;
+    }
     scan_rec<T>(sums, sums, int64_t(nb), temp + nb, s);
     hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kT), 0, s, in, m, static_cast<const T*>(sums), out);
 }
