@@ -368,8 +368,8 @@ int occupancy_grid(K kernel, int block) {
 
 // Host copy of one job (the caller's buffers are only valid during add_job).
 struct BatchJob {
-    std::vector<float> verts, normals, mat_param, prim_Ke, mat_ks;
-    std::vector<int32_t> tri_prim, mat_type, light_prims;
+    std::vector<float> verts, normals, mat_param, prim_Ke, mat_ks, uvs, tex_data;
+    std::vector<int32_t> tri_prim, mat_type, light_prims, mat_tex, tex_info;
     lmgpu_scene_desc desc{};
     int spp = 1, max_verts = 2, mode = 2, rr_depth = 5;
     uint64_t seed = 0;
@@ -387,6 +387,9 @@ struct Batch {
     DevBuf<DevLight> lights;
     DevBuf<float> cdf;
     DevBuf<int32_t> ltri;
+    DevBuf<float2> uvs;       // textured jobs: texture coordinates per triangle corner
+    DevBuf<float4> texels;    // textured jobs: texels (RGBA)
+    DevBuf<int4> tex_info;    // textured jobs: width, height, channels, first texel within the job's texels
     std::vector<JobDev> jobdev;   // per job, film_off filled per run
     const void* trav_nodes() const { return qnodes.p ? static_cast<const void*>(qnodes.p) : nodes.p; }
     // run state
@@ -425,9 +428,11 @@ struct Batch {
 int Batch::build_arena(int width, int leaf, hipStream_t s) {
     if (arena_width == width && arena_leaf == leaf) return LMGPU_OK;
     arena_width = 0;
-    // element counts per job: nodes, tris, shading tris, materials, lights, cdf entries, light tris
+    // element counts per job: nodes, tris, shading tris, materials, lights, cdf entries, light tris,
+    // texture coordinates, texels, textures
+    constexpr int kParts = 10;
     struct Part {
-        int64_t v[7];
+        int64_t v[kParts];
     };
     std::vector<Part> parts;
     std::vector<Scene*> scenes;
@@ -439,7 +444,7 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
     o.width = width;
     o.leaf_size = leaf;
     o.builder = LMGPU_BUILDER_LBVH;
-    int64_t tot[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t tot[kParts] = {};
     for (size_t j = 0; j < jobs.size(); j++) {
         Scene* sc = new (std::nothrow) Scene();
         if (!sc) {
@@ -453,9 +458,10 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
             return fail(rc, "job " + std::to_string(j) + ": " + lmgpu_last_error());
         }
         Part p{{sc->accel.num_nodes, int64_t(sc->accel.tris.n), int64_t(sc->stri.n), int64_t(sc->mats.n),
-                int64_t(sc->lights.n), int64_t(sc->light_cdf.n), int64_t(sc->light_tri.n)}};
+                int64_t(sc->lights.n), int64_t(sc->light_cdf.n), int64_t(sc->light_tri.n), int64_t(sc->uvs.n),
+                int64_t(sc->texels.n), int64_t(sc->tex_info.n)}};
         parts.push_back(p);
-        for (int k = 0; k < 7; k++) tot[k] += p.v[k];
+        for (int k = 0; k < kParts; k++) tot[k] += p.v[k];
         if (tot[0] >= (int64_t(1) << 31) || tot[1] >= (int64_t(1) << 27)) {
             cleanup();
             return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch too large for one arena (nodes or triangles)");
@@ -469,8 +475,15 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
         cleanup();
         return rc;
     }
+    uvs.reset();
+    texels.reset();
+    tex_info.reset();
+    if (tot[7] > 0 && ((rc = uvs.alloc(tot[7])) || (rc = texels.alloc(tot[8])) || (rc = tex_info.alloc(tot[9])))) {
+        cleanup();
+        return rc;
+    }
     jobdev.assign(jobs.size(), JobDev{});
-    int64_t off[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t off[kParts] = {};
     for (size_t j = 0; j < jobs.size(); j++) {
         Scene* sc = scenes[j];
         const Part& p = parts[j];
@@ -485,6 +498,9 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
         if (e == hipSuccess) e = cp(lights.p + off[4], sc->lights.p, sizeof(DevLight) * p.v[4]);
         if (e == hipSuccess) e = cp(cdf.p + off[5], sc->light_cdf.p, sizeof(float) * p.v[5]);
         if (e == hipSuccess) e = cp(ltri.p + off[6], sc->light_tri.p, sizeof(int32_t) * p.v[6]);
+        if (e == hipSuccess && p.v[7] > 0) e = cp(uvs.p + off[7], sc->uvs.p, sizeof(float2) * p.v[7]);
+        if (e == hipSuccess && p.v[7] > 0) e = cp(texels.p + off[8], sc->texels.p, sizeof(float4) * p.v[8]);
+        if (e == hipSuccess && p.v[7] > 0) e = cp(tex_info.p + off[9], sc->tex_info.p, sizeof(int4) * p.v[9]);
         if (e != hipSuccess) {
             cleanup();
             return fail(LMGPU_ERR_DEVICE, std::string("arena copy: ") + hipGetErrorString(e));
@@ -510,8 +526,12 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
         J.sc.lights = lights.p + off[4];
         J.sc.light_cdf = cdf.p + off[5];
         J.sc.light_tri = ltri.p + off[6];
+        // a job's texture indices and first texels stay job-local: its pointers start at its own tables
+        J.sc.uvs = p.v[7] > 0 ? uvs.p + off[7] : nullptr;
+        J.sc.texels = p.v[7] > 0 ? texels.p + off[8] : nullptr;
+        J.sc.tex_info = p.v[7] > 0 ? tex_info.p + off[9] : nullptr;
         J.root = int(off[0]);
-        for (int k = 0; k < 7; k++) off[k] += p.v[k];
+        for (int k = 0; k < kParts; k++) off[k] += p.v[k];
         // the per-job scene is no longer needed once its data is in the arena
         LMGPU_HIP(hipStreamSynchronize(s));
         delete sc;
@@ -786,12 +806,9 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     if (job->sampling_mode < 0 || job->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
     if (d.num_area_lights < 0 || (d.num_area_lights > 0 && !d.light_prims))
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad light list");
-    if (d.num_textures != 0 || d.uvs || d.tex_info || d.tex_data)
-        return fail(LMGPU_ERR_UNSUPPORTED, "textured scenes are not supported by the batched renderer");
-    if (d.mat_tex)
-        for (int i = 0; i < d.num_prims; i++)
-            if (d.mat_tex[i] >= 0)
-                return fail(LMGPU_ERR_UNSUPPORTED, "textured scenes are not supported by the batched renderer");
+    if (d.num_textures < 0 || (d.num_textures > 0 && (!d.tex_info || !d.tex_data || d.tex_data_len <= 0)))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad texture table");
+    // (texture indices, sizes and coordinates are checked in full when the job's scene is built)
     lmgpu::BatchJob J;
     const int64_t N = d.num_tris;
     const int P = d.num_prims;
@@ -804,6 +821,12 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     else J.prim_Ke.assign(3 * P, 0.f);
     if (d.num_area_lights > 0) J.light_prims.assign(d.light_prims, d.light_prims + d.num_area_lights);
     if (d.mat_ks) J.mat_ks.assign(d.mat_ks, d.mat_ks + 3 * P);
+    if (d.mat_tex) J.mat_tex.assign(d.mat_tex, d.mat_tex + P);
+    if (d.uvs) J.uvs.assign(d.uvs, d.uvs + 6 * N);
+    if (d.num_textures > 0) {
+        J.tex_info.assign(d.tex_info, d.tex_info + 4 * int64_t(d.num_textures));
+        J.tex_data.assign(d.tex_data, d.tex_data + d.tex_data_len);
+    }
     J.desc = d;
     J.spp = job->spp;
     J.max_verts = job->max_verts;
@@ -821,7 +844,10 @@ int lmgpu_batch_add_job(lmgpu_batch* b, const lmgpu_job_desc* job, int32_t* job_
     S.desc.prim_Ke = S.prim_Ke.data();
     S.desc.light_prims = S.light_prims.empty() ? nullptr : S.light_prims.data();
     S.desc.mat_ks = S.mat_ks.empty() ? nullptr : S.mat_ks.data();
-    S.desc.mat_tex = nullptr;
+    S.desc.mat_tex = S.mat_tex.empty() ? nullptr : S.mat_tex.data();
+    S.desc.uvs = S.uvs.empty() ? nullptr : S.uvs.data();
+    S.desc.tex_info = S.tex_info.empty() ? nullptr : S.tex_info.data();
+    S.desc.tex_data = S.tex_data.empty() ? nullptr : S.tex_data.data();
     b->b.arena_width = 0;   // the arena must be rebuilt
     b->b.have_results = false;
     if (job_id) *job_id = int32_t(b->b.jobs.size() - 1);

@@ -1,7 +1,7 @@
 """GPU material::mixture_wavefrontobj and texture::bitmap (the OBJ material
-path of model_wavefrontobj.cpp:155-215, 340-513) in renderer::pt and
-renderer::lt: closed form with a binary alpha mask, same-stream parity with
-the FP64 oracle, and the LT = PT - direct identity."""
+path of model_wavefrontobj.cpp:155-215, 340-513) in renderer::pt,
+renderer::lt and the batched renderer: closed form with a binary alpha mask,
+same-stream parity with the FP64 oracle, and the LT = PT - direct identity."""
 import numpy as np
 import pytest
 
@@ -92,19 +92,33 @@ def test_raycast_shows_texture_reflectance(device):
     assert np.allclose(film.data[solid], tx[solid], rtol=1e-5)
 
 
-def test_batch_rejects_textured_jobs(device):
-    flat = masked_quad_scene()[0].flatten()
+def test_batch_textured_jobs_equal_single_renders(device):
+    """Textured jobs in one batch (texture::bitmap Kd and an alpha mask):
+    each job's texture tables sit in the batch's arenas behind the job's own
+    base pointers, so every job renders as lmgpu_pt_render of its scene
+    alone (the same Philox streams and per-vertex code: equal films). An
+    untextured job between the two checks the offsets."""
+    from lm_amd import scenes
+    jobs = [mixture_room_scene(40, 32).flatten(), scenes.config0(seed=3).flatten(),
+            masked_quad_scene(24, 24)[0].flatten()]
+    jobs[1].w, jobs[1].h = 32, 32
+    jobs[1].camera.aspect = 1.0
+    params = [dict(spp=6, max_verts=8, seed=4), dict(spp=4, max_verts=6, seed=5), dict(spp=5, max_verts=6, seed=6)]
     b = Batch()
+    for flat, p in zip(jobs, params):
+        b.add_job(flat, **p)
+    b.run(width=4, path_capacity=4096, block=256)
+    for j, (flat, p) in enumerate(zip(jobs, params)):
+        ref, _, _ = GpuScene(flat, width=4).render_raw(spp=p['spp'], max_verts=p['max_verts'], seed=p['seed'])
+        got = b.film(j)
+        got = got[0] if isinstance(got, tuple) else got
+        err = np.abs(got - ref).max() / max(1.0, np.abs(ref).max())
+        print(f'textured batch job {j}: max rel diff {err:.2e}')
+        assert np.allclose(got, ref, rtol=1e-9, atol=1e-12)
+    # a texture index beyond the job's table is an error when the arena is built
+    bad = masked_quad_scene()[0].flatten()
+    bad.mat_tex[:] = 3
+    b2 = Batch()
+    b2.add_job(bad, spp=1, max_verts=4, seed=1)
     with pytest.raises(_lib.LmError):
-        b.add_job(flat, spp=1, max_verts=4, seed=1)
-    # an untextured mixture job is accepted and renders as lmgpu_pt_render does
-    flat = mixture_room_scene(32, 32).flatten()
-    flat.mat_tex[:] = -1
-    flat.textures = []
-    flat.uvs = None
-    b.add_job(flat, spp=8, max_verts=6, seed=4)
-    b.run(width=4)
-    ref, _, _ = GpuScene(flat, width=4).render_raw(spp=8, max_verts=6, seed=4)
-    got = b.film(0)
-    got = got[0] if isinstance(got, tuple) else got
-    assert np.allclose(got, ref, rtol=1e-9, atol=1e-12)
+        b2.run(width=4)
