@@ -246,8 +246,26 @@ typedef struct lmgpu_pt_params {
     int32_t sample_offset;    /* index of the first sample (progressive rendering) */
     int32_t guiding;          /* must be 0: guided rendering is lmgpu_pt_render_guided */
     double render_time;       /* > 0: scheduler::spp::time, render 1-spp passes for this many seconds */
-    int32_t reserved[8];
+    int32_t primary_mode;     /* LMGPU_PRIMARY_PIXEL or LMGPU_PRIMARY_IMAGE (primary_ray_sampling_mode) */
+    int32_t reserved0;
+    int64_t num_samples;      /* image mode, scheduler::spi::sample: samples over the whole image */
+    int32_t reserved[4];
 } lmgpu_pt_params;
+
+/* primary_ray_sampling_mode of renderer::pt (renderer_pt.cpp:60-76, 96-107,
+ * 133-136, 307-312).
+ * PIXEL (scheduler::spp::*): sample s of pixel p jitters its raster position
+ * inside pixel p and adds its estimate to pixel p; film value = film_sum / spp.
+ * IMAGE (scheduler::spi::*): `num_samples` samples (or passes of w*h samples for
+ * `render_time` seconds) in passes of min(w*h, remaining); sample i of pass k
+ * (k counted from sample_offset) draws the Philox stream (i, k) and takes a
+ * raster position uniform over the whole image; its estimate lands in the
+ * pixel of that position. The camera vertex also samples a light (NEE from
+ * the camera), whose contribution lands in the pixel of the light's raster
+ * position. Film value = film_sum * w*h / stats.samples. IMAGE takes no
+ * film_sumsq (a sample adds to two pixels) and no guide; spp is ignored. */
+#define LMGPU_PRIMARY_PIXEL 0
+#define LMGPU_PRIMARY_IMAGE 1
 
 typedef struct lmgpu_pt_stats {
     int64_t samples;          /* path samples traced */

@@ -338,20 +338,6 @@ struct BDConsts {
 
 __device__ __forceinline__ uint64_t bd_seed(const BDConsts& c) { return (uint64_t(c.seed_hi) << 32) | c.seed_lo; }
 
-// camera::pinhole: raster position of the direction `woc` (camera -> point)
-// and its importance J (camera_pinhole.cpp:102-109, 207-226).
-__device__ __forceinline__ bool cam_raster(const Camera& cam, f3 woc, float& rx, float& ry) {
-    const float ez = dot(woc, cam.w);
-    if (!(ez < 0.f)) return false;
-    rx = -dot(woc, cam.u) / ez / cam.tf / cam.aspect * .5f + .5f;
-    ry = -dot(woc, cam.v) / ez / cam.tf * .5f + .5f;
-    return rx >= 0.f && rx <= 1.f && ry >= 0.f && ry <= 1.f;
-}
-__device__ __forceinline__ float cam_J(const Camera& cam, f3 woc) {
-    const float ct = -dot(woc, cam.w);
-    return 1.f / (ct * ct * ct * (4.f * cam.tf * cam.tf * cam.aspect));
-}
-
 __device__ __forceinline__ Geom geom_of(const LVert& v) {
     return make_on_surface(mk(v.a.x, v.a.y, v.a.z), mk(v.b.x, v.b.y, v.b.z), mk(v.c.x, v.c.y, v.c.z));
 }

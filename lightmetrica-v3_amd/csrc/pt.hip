@@ -39,19 +39,105 @@
 
 namespace lmgpu {
 
+// IMAGE is a template parameter so that the pixel-mode camera ray is the
+// same code (and the same FP32 rounding) as in the batched renderer.
+template <bool IMAGE>
 __global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathState ps, Queues q, PTConsts c) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= c.npix) return;
     const uint32_t sample = uint32_t(q.counts[3]);
+    const uint64_t seed = (uint64_t(c.seed_hi) << 32) | c.seed_lo;
+    const lmgpu_ray ray = camera_ray(sc, p, sample, seed, IMAGE);
     ps.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.prev[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.info[p] = make_int4(1, 0, 2, 0);   // nv = 1, comp 0, prev degenerate (camera), not NEE-samplable
-    q.ext[0][p] = camera_ray(sc, p, sample, (uint64_t(c.seed_hi) << 32) | c.seed_lo);
+    if (IMAGE && c.mode != LMGPU_SAMPLING_NAIVE) {
+        // image mode: the camera vertex is NEE-samplable; the pdf of the
+        // sampled direction is Camera_Pinhole::pdf_direction = J(wo)
+        // (renderer_pt.cpp:133-136, 258-261)
+        ps.prev[p] = make_float4(0.f, 0.f, 0.f, cam_J(sc.cam, mk(ray.dir[0], ray.dir[1], ray.dir[2])));
+        ps.info[p] = make_int4(1, 0, 3, 0);   // nv = 1, comp 0, prev NEE-samplable and degenerate
+    } else {
+        ps.prev[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.info[p] = make_int4(1, 0, 2, 0);   // nv = 1, comp 0, prev degenerate (camera), not NEE-samplable
+    }
+    q.ext[0][p] = ray;
     q.ext_path[0][p] = p;
     if (p == 0) {
         q.counts[0] = c.npix;
         atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npix);
+    }
+}
+
+// Image mode: NEE at the camera vertex (renderer_pt.cpp:138-188 with sp the
+// camera): a light point sampled as at every other vertex (Philox block 1 of
+// vertex 0), Camera_Pinhole::eval = J(wo) inside the frustum, MIS against
+// reaching the light from the camera (balance heuristic with J), queued as a
+// shadow ray whose contribution goes to the pixel of the light's raster
+// position (contribution w = -(pixel + 1)).
+__global__ __launch_bounds__(kBlock) void k_camera_nee(SceneDev sc, Queues q, PTConsts c) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    bool emit = false;
+    lmgpu_ray ray;
+    float4 C;
+    if (p < c.npix) {
+        const uint32_t sample = uint32_t(q.counts[3]);
+        const uint64_t seed = (uint64_t(c.seed_hi) << 32) | c.seed_lo;
+        const rand4 r1 = rng(uint32_t(p), sample, 0u, 1u, seed);
+        const int li = min(max(int(r1.u[0] * sc.num_lights), 0), sc.num_lights - 1);
+        const float psel = 1.f / sc.num_lights;
+        const f3 pos = sc.cam.pos;
+        f3 dirL = mk(0, 0, 0);   // from the camera toward the light
+        float tmaxL = 0.f, p_light = 0.f;
+        f3 weight = mk(0, 0, 0);
+        bool ok = false;
+        if (li < sc.num_area_lights) {
+            // Light_Area::sample_direct / pdf_direct from a degenerate point: G = |cos_L| / d^2
+            const DevLight dl = sc.lights[li];
+            f3 pL, nL;
+            sample_light_point(sc, li, r1.u[1], r1.u[2], r1.u[3], pL, nL);
+            const f3 wL = normalize(pos - pL);
+            if (dot(wL, nL) > 0.f) {
+                const f3 dd = pL - pos;
+                const float d2 = dot(dd, dd);
+                const float G = fabsf(dot(nL, -wL)) / d2;
+                if (G != 0.f) {
+                    const float pA = dl.Ke_invA.w / G;
+                    weight = mk(dl.Ke_invA.x, dl.Ke_invA.y, dl.Ke_invA.z) / pA / psel;
+                    p_light = psel * pA;
+                    dirL = normalize(pL - pos);
+                    tmaxL = sqrtf(d2) * (1.f - kEps);
+                    ok = true;
+                }
+            }
+        } else {
+            // Light_EnvConst::sample_direct: uniform sphere, solid-angle pdf 1 / (4 pi)
+            const f3 wL = sample_uniform_sphere(r1.u[2], r1.u[3]);
+            const float pL = 1.f / (4.f * kPi);
+            weight = sc.env_Le / pL / psel;
+            p_light = psel * pL;
+            dirL = -wL;
+            tmaxL = kInf - 1.f;
+            ok = true;
+        }
+        float rx, ry;
+        if (ok && cam_raster(sc.cam, dirL, rx, ry)) {
+            const float J = cam_J(sc.cam, dirL);
+            const float mis = c.mode == LMGPU_SAMPLING_MIS ? balance_heuristic(p_light, J) : 1.f;
+            const f3 Cv = weight * (J * mis);
+            if (!is_zero(Cv)) {
+                emit = true;
+                ray.org[0] = pos.x; ray.org[1] = pos.y; ray.org[2] = pos.z;
+                ray.tmin = kEps;
+                ray.dir[0] = dirL.x; ray.dir[1] = dirL.y; ray.dir[2] = dirL.z;
+                ray.tmax = tmaxL;
+                C = make_float4(Cv.x, Cv.y, Cv.z, __int_as_float(-(raster_pixel(sc.w, sc.h, rx, ry) + 1)));
+            }
+        }
+    }
+    const int ss = wave_append(q.counts + 2, emit);
+    if (emit) {
+        q.sh[ss] = ray;
+        q.sh_contrib[ss] = C;
     }
 }
 
@@ -69,7 +155,8 @@ __global__ void k_reset_counts(int32_t* counts, int out_q) {
 // incident radiance).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Queues q, unsigned long long* next,
-                                                   int refill, GRec* __restrict__ rec, const int32_t* __restrict__ sh_rec) {
+                                                   int refill, GRec* __restrict__ rec, const int32_t* __restrict__ sh_rec,
+                                                   double* __restrict__ splat) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
@@ -85,6 +172,14 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Q
         if (r.prim < 0 && !r.overflow) {
             const float4 c = q.sh_contrib[id];
             const int p = __float_as_int(c.w);
+            if (p < 0) {
+                // a camera-vertex light sample (image mode): straight into the film at its pixel
+                double* f = splat + 3 * int64_t(-p - 1);
+                atomicAdd(f + 0, double(c.x));
+                atomicAdd(f + 1, double(c.y));
+                atomicAdd(f + 2, double(c.z));
+                return;
+            }
             float4 L = ps.L[p];
             L.x += c.x;
             L.y += c.y;
@@ -324,11 +419,23 @@ __global__ void k_zero_i32(int32_t* p, int n) {
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_accumulate(PathState ps, double* __restrict__ sum,
-                                                       double* __restrict__ sumsq, int npix) {
+// Adds each path's estimate to the film. Pixel mode: to its own pixel, with
+// its square. Image mode: to the pixel of its raster position (the numbers
+// of camera_ray), with FP64 atomics since several paths may land there.
+__global__ __launch_bounds__(kBlock) void k_accumulate(SceneDev sc, PathState ps, Queues q, PTConsts c,
+                                                       double* __restrict__ sum, double* __restrict__ sumsq) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
+    if (p >= c.npix) return;
     const float4 L = ps.L[p];
+    if (c.image) {
+        if (L.x == 0.f && L.y == 0.f && L.z == 0.f) return;
+        const rand4 r = rng(uint32_t(p), uint32_t(q.counts[3]), 0u, 0u, (uint64_t(c.seed_hi) << 32) | c.seed_lo);
+        double* f = sum + 3 * int64_t(raster_pixel(sc.w, sc.h, r.u[2], r.u[3]));
+        atomicAdd(f + 0, double(L.x));
+        atomicAdd(f + 1, double(L.y));
+        atomicAdd(f + 2, double(L.z));
+        return;
+    }
     sum[3 * p + 0] += double(L.x);
     sum[3 * p + 1] += double(L.y);
     sum[3 * p + 2] += double(L.z);
@@ -647,10 +754,27 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
     const int32_t* cshrec = crec ? gshrec.p : nullptr;
     const int npix = c.npix;
     const unsigned gb = unsigned((npix + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_generate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
+    if (c.image) hipLaunchKernelGGL(k_generate<true>, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
+    else hipLaunchKernelGGL(k_generate<false>, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c);
     if (gp && gp->rec_depth > 0) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, gcount.p + 1, 1);
     const int sg = shade_grid(npix);
     const int tg = accel.width == 4 ? pt_trace_grid<4>() : (accel.width == 8 ? pt_trace_grid<8>() : pt_trace_grid<2>());
+    if (c.image && c.mode != LMGPU_SAMPLING_NAIVE) {
+        // the camera vertex's light samples, traced before the first bounce
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, counts.p + 2, 1);
+        hipLaunchKernelGGL(k_camera_nee, dim3(gb), dim3(kBlock), 0, s, dev, q, c);
+        hipLaunchKernelGGL(k_bounce_begin, dim3(1), dim3(64), 0, s, next.p);
+        unsigned long long* nx = next.p + 1;
+        if (accel.width == 4)
+            hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, refill_primary, nullptr,
+                               nullptr, sum);
+        else if (accel.width == 8)
+            hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, refill_primary, nullptr,
+                               nullptr, sum);
+        else
+            hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, refill_primary, nullptr,
+                               nullptr, sum);
+    }
     for (int b = 0; b < c.max_verts - 1; b++) {
         const int in_q = b & 1;
         // camera rays are coherent: keep 64 consecutive rays per wave
@@ -668,13 +792,13 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         if (c.mode != LMGPU_SAMPLING_NAIVE) {
             unsigned long long* nx = next.p + 1;
             if (accel.width == 4)
-                hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
+                hipLaunchKernelGGL(k_connect<4>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec, sum);
             else if (accel.width == 8)
-                hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
-            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec);
+                hipLaunchKernelGGL(k_connect<8>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec, sum);
+            else hipLaunchKernelGGL(k_connect<2>, dim3(tg), dim3(kBlock), 0, s, dev, ps, q, nx, pm, crec, cshrec, sum);
         }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(gb), dim3(kBlock), 0, s, ps, sum, sumsq, npix);
+    hipLaunchKernelGGL(k_accumulate, dim3(gb), dim3(kBlock), 0, s, dev, ps, q, c, sum, sumsq);
     if (gp && gp->steps > 0) {
         // online training on this pass's records
         const int64_t total = int64_t(gp->rec_depth) * npix;
@@ -698,8 +822,17 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (!sum_in) return fail(LMGPU_ERR_INVALID_ARGUMENT, "film_sum is NULL");
     if (p->max_verts < 2 || p->max_verts > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "max_verts must be in [2, 64]");
     if (p->sampling_mode < 0 || p->sampling_mode > 2) return fail(LMGPU_ERR_INVALID_ARGUMENT, "bad sampling_mode");
-    if (p->render_time <= 0 && p->spp <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "spp must be positive");
     if (p->guiding) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering goes through lmgpu_pt_render_guided");
+    if (p->primary_mode != LMGPU_PRIMARY_PIXEL && p->primary_mode != LMGPU_PRIMARY_IMAGE)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "primary_mode must be LMGPU_PRIMARY_PIXEL or LMGPU_PRIMARY_IMAGE");
+    const bool image = p->primary_mode == LMGPU_PRIMARY_IMAGE;
+    if (!image && p->render_time <= 0 && p->spp <= 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "spp must be positive");
+    if (image && p->render_time <= 0 && p->num_samples <= 0)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "image sampling mode needs num_samples or render_time");
+    if (p->sample_offset < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "sample_offset must be >= 0");
+    if (image && sumsq_in)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "image sampling mode has no per-pixel sum of squares (pass NULL)");
+    if (image && gp) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering uses the pixel sampling mode");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
     LMGPU_TRY(ensure_workspace(npix));
@@ -732,6 +865,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     c.seed_lo = uint32_t(p->seed);
     c.seed_hi = uint32_t(p->seed >> 32);
     c.npix = npix;
+    c.image = image ? 1 : 0;
     // stats counters [8..15] as 64-bit: [8] samples, [10] extension rays, [12] shadow rays,
     // [14] guide training records
     LMGPU_HIP(hipMemsetAsync(counts.p + 8, 0, 8 * sizeof(int32_t), s));
@@ -744,6 +878,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
                             graph_guide.batch == gk.batch && graph_guide.steps == gk.steps &&
                             graph_guide.rec_depth == gk.rec_depth;
     const bool key_same = graph && graph_key[0] == c.max_verts && graph_key[1] == c.mode && graph_key[2] == c.rr_depth &&
+                          graph_key[3] == c.image &&
                           graph_seed == p->seed && graph_sum_ptr == sum && graph_sq_ptr == sumsq && guide_same;
     if (!key_same) {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
@@ -759,7 +894,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
         (void)hipGraphDestroy(g);
         (void)hipStreamDestroy(cs);
         if (e != hipSuccess) { graph = nullptr; return fail(LMGPU_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
-        graph_key[0] = c.max_verts; graph_key[1] = c.mode; graph_key[2] = c.rr_depth;
+        graph_key[0] = c.max_verts; graph_key[1] = c.mode; graph_key[2] = c.rr_depth; graph_key[3] = c.image;
         graph_seed = p->seed;
         graph_guide = gk;
         graph_sum_ptr = sum;
@@ -767,13 +902,27 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     }
     int64_t passes = 0;
     if (p->render_time > 0) {
-        // scheduler::spp::time: 1-spp passes until the budget is spent
+        // scheduler::spp::time (spi::time in image mode): passes of w*h samples until the budget is spent
         while (true) {
             LMGPU_HIP(hipGraphLaunch(graph, s));
             passes++;
             LMGPU_HIP(hipStreamSynchronize(s));
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (el > p->render_time) break;
+        }
+    } else if (image) {
+        // scheduler::spi::sample: full passes from the graph, then the remainder as one shorter pass
+        const int64_t full = p->num_samples / npix;
+        const int rem = int(p->num_samples % npix);
+        for (int64_t i = 0; i < full; i++) {
+            LMGPU_HIP(hipGraphLaunch(graph, s));
+            passes++;
+        }
+        if (rem > 0) {
+            PTConsts cr = c;
+            cr.npix = rem;
+            LMGPU_TRY(enqueue_pass(cr, sum, nullptr, nullptr, s));
+            passes++;
         }
     } else {
         for (int i = 0; i < p->spp; i++) {

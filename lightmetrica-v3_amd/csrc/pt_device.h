@@ -74,6 +74,7 @@ struct PTConsts {
     int rr_depth;
     uint32_t seed_lo, seed_hi;
     int npix;
+    int image = 0;   // primary_ray_sampling_mode = image
 };
 
 // Training record of one guided vertex of one path (slot v * npix + p for
@@ -207,12 +208,15 @@ __device__ __forceinline__ float pdf_light_env(const SceneDev& sc, f3 np, bool p
 
 // Camera ray of `pixel` for `sample` (camera::pinhole primary ray through a
 // jittered raster position, src/camera/camera_pinhole.cpp:128-132; the
-// jitter is (pixel, sample, 0, 0)[2..3] of the Philox stream).
-__device__ __forceinline__ lmgpu_ray camera_ray(const SceneDev& sc, int pixel, uint32_t sample, uint64_t seed) {
+// jitter is (pixel, sample, 0, 0)[2..3] of the Philox stream). With `image`
+// the same two numbers are the raster position over the whole image.
+__device__ __forceinline__ lmgpu_ray camera_ray(const SceneDev& sc, int pixel, uint32_t sample, uint64_t seed,
+                                                bool image = false) {
     const int x = pixel % sc.w, y = pixel / sc.w;
     const rand4 r = rng(uint32_t(pixel), sample, 0u, 0u, seed);
     const float dx = 1.f / sc.w, dy = 1.f / sc.h;
-    const f3 d = primary_dir(sc.cam, dx * x + dx * r.u[2], dy * y + dy * r.u[3]);
+    const f3 d = image ? primary_dir(sc.cam, r.u[2], r.u[3])
+                       : primary_dir(sc.cam, dx * x + dx * r.u[2], dy * y + dy * r.u[3]);
     lmgpu_ray ray;
     ray.org[0] = sc.cam.pos.x; ray.org[1] = sc.cam.pos.y; ray.org[2] = sc.cam.pos.z;
     ray.tmin = kEps;

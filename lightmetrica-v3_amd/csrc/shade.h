@@ -335,4 +335,25 @@ __device__ __forceinline__ f3 primary_dir(const Camera& c, float rx, float ry) {
     return c.u * d.x + c.v * d.y + c.w * d.z;
 }
 
+// camera::pinhole: raster position of the direction `woc` (camera -> point)
+// and its importance J (camera_pinhole.cpp:102-109, 207-226).
+__device__ __forceinline__ bool cam_raster(const Camera& cam, f3 woc, float& rx, float& ry) {
+    const float ez = dot(woc, cam.w);
+    if (!(ez < 0.f)) return false;
+    rx = -dot(woc, cam.u) / ez / cam.tf / cam.aspect * .5f + .5f;
+    ry = -dot(woc, cam.v) / ez / cam.tf * .5f + .5f;
+    return rx >= 0.f && rx <= 1.f && ry >= 0.f && ry <= 1.f;
+}
+__device__ __forceinline__ float cam_J(const Camera& cam, f3 woc) {
+    const float ct = -dot(woc, cam.w);
+    return 1.f / (ct * ct * ct * (4.f * cam.tf * cam.tf * cam.aspect));
+}
+
+// Film::raster_to_pixel (film.h:166-171): pixel index y * w + x.
+__device__ __forceinline__ int raster_pixel(int w, int h, float rx, float ry) {
+    const int x = min(max(int(rx * float(w)), 0), w - 1);
+    const int y = min(max(int(ry * float(h)), 0), h - 1);
+    return y * w + x;
+}
+
 }  // namespace lmgpu

@@ -452,12 +452,24 @@ Renderer_PT_MI355X::Renderer_PT_MI355X(const Scene* scene, Film* film, const Pro
     // (paths end at max_verts); plain rendering keeps renderer_pt.cpp:281.
     rr_depth_ = prop_or<int>(p, "rr_depth", guiding_ ? std::max(max_verts_, 5) : 5);
     seed_ = uint64_t(prop_or<int>(p, "seed", 0));
+    const std::string prm = prop_or<std::string>(p, "primary_ray_sampling_mode", "pixel");
+    if (prm != "pixel" && prm != "image")
+        throw Exception(Error::InvalidArgument, "invalid primary_ray_sampling_mode '" + prm + "'");
+    image_ = prm == "image";
+    if (image_ && guiding_) throw Exception(Error::Unsupported, "guided rendering uses the pixel sampling mode");
     const std::string sched = prop_or<std::string>(p, "scheduler", "sample");
     spp_ = 0;
+    num_samples_ = 0;
     render_time_ = 0;
-    if (sched == "sample") {
+    if (sched == "sample" && !image_) {
         spp_ = prop_req<int>(p, "spp");
         if (spp_ <= 0) throw Exception(Error::InvalidArgument, "spp must be positive");
+    } else if (sched == "sample") {
+        // an int, or a whole double for counts beyond 2^31
+        const double ns = prop_req<double>(p, "num_samples");
+        if (!(ns >= 1) || ns != std::floor(ns) || ns > 9.0e15)
+            throw Exception(Error::InvalidArgument, "num_samples must be a positive whole number");
+        num_samples_ = int64_t(ns);
     } else if (sched == "time") {
         render_time_ = prop_req<double>(p, "render_time");
         if (!(render_time_ > 0)) throw Exception(Error::InvalidArgument, "render_time must be positive");
@@ -511,9 +523,12 @@ RenderResult Renderer_PT_MI355X::render() {
     pp.rr_depth = rr_depth_;
     pp.seed = seed_;
     pp.render_time = render_time_;
+    pp.primary_mode = image_ ? LMGPU_PRIMARY_IMAGE : LMGPU_PRIMARY_PIXEL;
+    pp.num_samples = num_samples_;
     const size_t n = size_t(size.w) * size_t(size.h) * 3;
     sum_.assign(n, 0.0);
-    sumsq_.assign(n, 0.0);
+    if (image_) sumsq_.clear();
+    else sumsq_.assign(n, 0.0);
     lmgpu_pt_stats st{};
     if (guiding_) {
         lmgpu_guide_opts go{};
@@ -525,15 +540,18 @@ RenderResult Renderer_PT_MI355X::render() {
         check(lmgpu_pt_render_guided(gs_, &pp, guide_, &go, sum_.data(), sumsq_.data(), 0, &st, nullptr),
               "lmgpu_pt_render_guided");
     } else {
-        check(lmgpu_pt_render(gs_, &pp, sum_.data(), sumsq_.data(), 0, &st, nullptr), "lmgpu_pt_render");
+        check(lmgpu_pt_render(gs_, &pp, sum_.data(), image_ ? nullptr : sumsq_.data(), 0, &st, nullptr),
+              "lmgpu_pt_render");
     }
-    // film value = sum / spp (src/renderer/renderer_pt.cpp:307-308)
+    // film value = sum / spp in pixel mode, sum * w*h / processed in image mode
+    // (src/renderer/renderer_pt.cpp:307-312)
     film_->clear();
     auto& data = film_->data();
-    const double inv = st.spp > 0 ? 1.0 / double(st.spp) : 0.0;
-    for (size_t i = 0; i < n; i++) data[i] = sum_[i] * inv;
+    const int64_t processed = image_ ? st.samples : st.spp;
+    const double scale = processed <= 0 ? 0.0 : (image_ ? double(size.w) * size.h : 1.0) / double(processed);
+    for (size_t i = 0; i < n; i++) data[i] = sum_[i] * scale;
     RenderResult r;
-    r.processed = st.spp;
+    r.processed = processed;
     r.elapsed = st.elapsed_s;
     r.extension_rays = st.extension_rays;
     r.shadow_rays = st.shadow_rays;

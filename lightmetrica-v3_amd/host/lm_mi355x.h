@@ -300,7 +300,9 @@ public:
 
 // renderer::pt_mi355x. Props (src/renderer/renderer_pt.cpp:51-77):
 //   max_verts (int, required), sampling_mode ("mis" | "nee" | "naive"),
-//   scheduler ("sample" | "time"), spp (int) or render_time (seconds),
+//   primary_ray_sampling_mode ("pixel" | "image"), scheduler ("sample" |
+//   "time"): spp (pixel mode, scheduler::spp::sample), num_samples (image
+//   mode, scheduler::spi::sample) or render_time (seconds),
 //   seed (int), rr_depth (int; 5, or max_verts when guiding), guiding
 //   (bool) with guide_alpha (0.65), guide_lr (1e-2), guide_batch (2^16),
 //   guide_steps (8), guide_depth (8);
@@ -311,6 +313,7 @@ public:
     ~Renderer_PT_MI355X() override;
     RenderResult render() override;
     // per-pixel sum and sum of squares of the per-sample estimate (FP64) of the last render
+    // (no sums of squares in image mode: film_sumsq() is empty)
     const std::vector<double>& film_sum() const { return sum_; }
     const std::vector<double>& film_sumsq() const { return sumsq_; }
 
@@ -318,6 +321,8 @@ private:
     const Scene* scene_;
     Film* film_;
     int max_verts_, mode_, rr_depth_, spp_;
+    bool image_;
+    int64_t num_samples_;
     double render_time_;
     uint64_t seed_;
     bool guiding_;

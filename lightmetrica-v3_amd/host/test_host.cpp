@@ -285,6 +285,24 @@ int main() {
             CHECK(std::abs(z) < 5);
             if (guided) CHECK(res.guide_records > 0);
         }
+        // primary_ray_sampling_mode = "image" (scheduler::spi::sample, a last partial pass):
+        // film value sum * w*h / num_samples, the same furnace constant
+        for (const char* mode : {"mis", "nee", "naive"}) {
+            Film film(32, 32);
+            const int64_t ns = int64_t(film.num_pixels()) * spp + 100;
+            Props p{{"max_verts", M}, {"seed", 6}, {"scheduler", std::string("sample")},
+                    {"primary_ray_sampling_mode", std::string("image")}, {"num_samples", double(ns)},
+                    {"sampling_mode", std::string(mode)}};
+            Renderer_PT_MI355X r(&s, &film, p);
+            const RenderResult res = r.render();
+            CHECK(res.processed == ns && r.film_sumsq().empty());
+            double m = 0;
+            for (int y = 0; y < 32; y++)
+                for (int x = 0; x < 32; x++) m += film.pixel(x, y).x;
+            m /= film.num_pixels();
+            std::printf("  image mode %s: film mean %.6f expected %.6f\n", mode, m, expected);
+            CHECK(std::abs(m - expected) < 0.03 * expected);
+        }
     });
 
     run("renderer::pt_mi355x properties and errors", [] {
@@ -305,6 +323,12 @@ int main() {
         expect({{"max_verts", 8}, {"scheduler", std::string("time")}}, Error::InvalidArgument);   // no render_time
         expect({{"max_verts", 1}, {"spp", 4}}, Error::InvalidArgument);                      // lmgpu rejects
         expect({{"max_verts", std::string("8")}, {"spp", 4}}, Error::InvalidArgument);      // wrong type
+        expect({{"max_verts", 8}, {"spp", 4}, {"primary_ray_sampling_mode", std::string("film")}},
+               Error::InvalidArgument);
+        expect({{"max_verts", 8}, {"spp", 4}, {"primary_ray_sampling_mode", std::string("image")}},
+               Error::InvalidArgument);                                                     // no num_samples
+        expect({{"max_verts", 8}, {"num_samples", 100}, {"primary_ray_sampling_mode", std::string("image")},
+                {"guiding", true}}, Error::Unsupported);
         // time scheduler
         Renderer_PT_MI355X r(&s, &film, {{"max_verts", 8}, {"scheduler", std::string("time")}, {"render_time", 0.2}});
         const RenderResult res = r.render();

@@ -109,7 +109,8 @@ class PTParams(ctypes.Structure):
     _fields_ = [
         ('spp', ctypes.c_int32), ('max_verts', ctypes.c_int32), ('sampling_mode', ctypes.c_int32),
         ('rr_depth', ctypes.c_int32), ('seed', ctypes.c_uint64), ('sample_offset', ctypes.c_int32),
-        ('guiding', ctypes.c_int32), ('render_time', ctypes.c_double), ('reserved', ctypes.c_int32 * 8),
+        ('guiding', ctypes.c_int32), ('render_time', ctypes.c_double), ('primary_mode', ctypes.c_int32),
+        ('reserved0', ctypes.c_int32), ('num_samples', ctypes.c_int64), ('reserved', ctypes.c_int32 * 4),
     ]
 
 

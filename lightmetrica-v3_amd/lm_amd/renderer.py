@@ -20,6 +20,7 @@ from . import _lib
 from .accel import _stream_ptr
 
 _MODES = {'naive': 0, 'nee': 1, 'mis': 2}
+_PRIMARY = {'pixel': 0, 'image': 1}   # primary_ray_sampling_mode
 
 
 class Film:
@@ -167,11 +168,16 @@ class GpuScene:
 
     def render_raw(self, spp: int = 1, max_verts: int = 8, sampling_mode: int = 2, rr_depth: Optional[int] = None,
                    seed: int = 0, sample_offset: int = 0, render_time: float = 0.0,
-                   film_sum=None, film_sumsq=None, stream=None, guide=None, guide_opts: Optional[dict] = None):
+                   film_sum=None, film_sumsq=None, stream=None, guide=None, guide_opts: Optional[dict] = None,
+                   primary_mode: str = 'pixel', num_samples: int = 0):
         """lmgpu_pt_render (lmgpu_pt_render_guided with a `guide`). With
         torch CUDA tensors for film_sum / film_sumsq ([h,w,3] float64) the
         film stays on the device and is accumulated into; otherwise host
-        arrays are returned (fresh sums). guide_opts keys: alpha,
+        arrays are returned (fresh sums). primary_mode 'image' is
+        primary_ray_sampling_mode = "image" with scheduler::spi::sample over
+        `num_samples` samples (or spi::time with render_time); spp is then
+        ignored, the film value is sum * w*h / stats['samples'] and there are
+        no sums of squares (q is None; film_sumsq must be None). guide_opts keys: alpha,
         learning_rate, batch, steps_per_pass, record_depth, weight_clamp.
         rr_depth: Russian roulette once a path has more vertices than this;
         default 5 (renderer_pt.cpp:281) without a guide and max_verts (no
@@ -188,6 +194,13 @@ class GpuScene:
         p.sample_offset = int(sample_offset)
         p.guiding = 0
         p.render_time = float(render_time)
+        if primary_mode not in _PRIMARY:
+            raise ValueError(f'invalid primary_mode {primary_mode!r}')
+        p.primary_mode = _PRIMARY[primary_mode]
+        p.num_samples = int(num_samples)
+        image = primary_mode == 'image'
+        if image and film_sumsq is not None:
+            raise ValueError('image sampling mode has no sums of squares: pass film_sumsq=None')
         st = _lib.PTStats()
         if guide is not None:
             o = _lib.GuideOpts(*GUIDE_DEFAULTS)
@@ -208,8 +221,9 @@ class GpuScene:
             s, q = film_sum, film_sumsq
         else:
             s = np.zeros((self.h, self.w, 3), dtype=np.float64)
-            q = np.zeros((self.h, self.w, 3), dtype=np.float64)
-            _lib.call(fn, *pre, s.ctypes.data_as(ctypes.c_void_p), q.ctypes.data_as(ctypes.c_void_p), 0,
+            q = None if image else np.zeros((self.h, self.w, 3), dtype=np.float64)
+            _lib.call(fn, *pre, s.ctypes.data_as(ctypes.c_void_p),
+                      q.ctypes.data_as(ctypes.c_void_p) if q is not None else None, 0,
                       ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
         stats = {'samples': st.samples, 'spp': st.spp, 'extension_rays': st.extension_rays,
                  'shadow_rays': st.shadow_rays, 'elapsed': st.elapsed_s, 'guide_records': st.guide_records}
@@ -301,18 +315,23 @@ GUIDE_DEFAULTS = (0.65, 1e-2, 1 << 16, 8, 8)
 
 
 class RendererPT:
-    """renderer::pt_mi355x: drop-in for renderer::pt (pixel sampling mode)."""
+    """renderer::pt_mi355x: drop-in for renderer::pt (renderer_pt.cpp:22-315)
+    with both primary_ray_sampling_mode values ('pixel', 'image')."""
 
     def __init__(self, scene: GpuScene, output: Film, max_verts: int, seed: Optional[int] = None,
                  sampling_mode: str = 'mis', scheduler: str = 'sample', spp: Optional[int] = None,
                  render_time: Optional[float] = None, rr_depth: int = 5,
-                 primary_ray_sampling_mode: str = 'pixel'):
+                 primary_ray_sampling_mode: str = 'pixel', num_samples: Optional[int] = None):
         if sampling_mode not in _MODES:
             raise ValueError(f'invalid sampling_mode {sampling_mode!r}')
-        if primary_ray_sampling_mode != 'pixel':
-            raise NotImplementedError('only primary_ray_sampling_mode = "pixel" is implemented')
-        if scheduler == 'sample' and not spp:
+        if primary_ray_sampling_mode not in _PRIMARY:
+            raise ValueError(f'invalid primary_ray_sampling_mode {primary_ray_sampling_mode!r}')
+        image = primary_ray_sampling_mode == 'image'
+        # pixel mode runs scheduler::spp::*, image mode scheduler::spi::* (renderer_pt.cpp:60-76)
+        if scheduler == 'sample' and not image and not spp:
             raise ValueError('scheduler "sample" needs spp')
+        if scheduler == 'sample' and image and not num_samples:
+            raise ValueError('scheduler "sample" in image mode needs num_samples')
         if scheduler == 'time' and not render_time:
             raise ValueError('scheduler "time" needs render_time')
         if (output.w, output.h) != (scene.w, scene.h):
@@ -326,6 +345,8 @@ class RendererPT:
         self.spp = int(spp or 0)
         self.render_time = float(render_time or 0.0)
         self.rr_depth = int(rr_depth)
+        self.primary_mode = primary_ray_sampling_mode
+        self.num_samples = int(num_samples or 0)
 
     def render(self) -> dict:
         film = self.film
@@ -333,10 +354,17 @@ class RendererPT:
         s, q, st = self.scene.render_raw(
             spp=self.spp if self.scheduler == 'sample' else 0, max_verts=self.max_verts,
             sampling_mode=self.mode, rr_depth=self.rr_depth, seed=self.seed,
-            render_time=self.render_time if self.scheduler == 'time' else 0.0)
-        processed = st['spp']
-        film.sum, film.sumsq, film.spp = s, q, processed
-        film.data[:] = s / processed
+            render_time=self.render_time if self.scheduler == 'time' else 0.0, primary_mode=self.primary_mode,
+            num_samples=self.num_samples if self.scheduler == 'sample' else 0)
+        if self.primary_mode == 'image':
+            # Film::rescale(w*h / processed) (renderer_pt.cpp:310-312)
+            processed = st['samples']
+            film.sum, film.sumsq, film.spp = s, None, st['spp']
+            film.data[:] = s * (film.num_pixels() / processed)
+        else:
+            processed = st['spp']
+            film.sum, film.sumsq, film.spp = s, q, processed
+            film.data[:] = s / processed
         return {'processed': processed, 'elapsed': st['elapsed'], 'stats': st}
 
 

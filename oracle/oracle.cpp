@@ -342,6 +342,9 @@ struct OrcPTParams {
     uint64_t seed;
     int32_t nthreads;
     int32_t sample_offset;   // first sample index (for progressive passes)
+    int32_t primary_mode;    // 0 pixel, 1 image (primary_ray_sampling_mode)
+    int32_t pad;
+    int64_t num_samples;     // image mode (scheduler::spi::sample): samples over the image
 };
 
 struct OrcLTParams {
@@ -801,6 +804,30 @@ inline V3 primary_dir(const Scene& S, double rx, double ry) {
     return S.cu * d.x + S.cv * d.y + S.cw * d.z;
 }
 
+// camera::pinhole raster_position (camera_pinhole.cpp:207-226): false when wo
+// points away from the camera or leaves the view frustum.
+inline bool cam_raster(const Scene& S, V3 wo, double& rx, double& ry) {
+    const double ez = dot(wo, S.cw);
+    if (ez >= 0) return false;
+    rx = -dot(wo, S.cu) / ez / S.tf / S.aspect * .5 + .5;
+    ry = -dot(wo, S.cv) / ez / S.tf * .5 + .5;
+    return !(rx < 0 || rx > 1 || ry < 0 || ry > 1);
+}
+
+// camera::pinhole J(wo) (camera_pinhole.cpp:100-109): the density per solid
+// angle of directions whose raster position is uniform over the image.
+inline double cam_J(const Scene& S, V3 wo) {
+    const double c = -dot(wo, S.cw);
+    return 1.0 / (c * c * c) / (S.tf * S.tf * S.aspect * 4);
+}
+
+// Film::raster_to_pixel (film.h:166-171).
+inline int64_t raster_to_pixel(int w, int h, double rx, double ry) {
+    const int x = std::clamp(int(rx * double(w)), 0, w - 1);
+    const int y = std::clamp(int(ry * double(h)), 0, h - 1);
+    return int64_t(y) * w + x;
+}
+
 // Light selection pdf and light pdf_direct for a light endpoint hit at `gL`
 // seen from `g` along wo (from the light toward g).
 inline double pdf_direct_light(const Scene& S, const Geom& g, const Geom& gL, int light, V3 wo) {
@@ -1037,30 +1064,57 @@ void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) 
 
 // ---- renderer::pt -------------------------------------------------------------
 
-// Restates Renderer_PT::render (src/renderer/renderer_pt.cpp:89-302) in pixel
-// sampling mode with scheduler::spp::sample: for every pixel and sample, one
-// random walk; every contribution of the walk is splatted into the walk's own
-// pixel (the reference splats at raster_to_pixel(raster_pos), whose pixel is
-// the sampled one). Accumulates per-pixel sum and sum of squares of the
-// per-sample estimate, in FP64, unscaled (divide by spp for the film value).
+// Restates Renderer_PT::render (src/renderer/renderer_pt.cpp:89-302) with
+// scheduler::spp::sample: for every pixel index and sample, one random walk.
+// Pixel sampling mode: the raster position is jittered inside the pixel, and
+// every contribution of the walk lands in that pixel (the reference splats at
+// raster_to_pixel(raster_pos), whose pixel is the sampled one); the film
+// holds per-pixel sums and sums of squares of the per-sample estimate.
+// Image sampling mode (renderer_pt.cpp:96-107, 130-136, 147-152, 307-312)
+// with scheduler::spi::sample: num_samples walks, walk j drawing the Philox
+// stream (j mod w*h, sample_offset + j / (w*h)); the raster position is
+// uniform over the whole image and the walk's contributions land in the
+// pixel of raster_position(primary direction); the camera vertex is
+// NEE-samplable, so a light sampled from the camera splats
+// J(wo) * weight * MIS at the light's own raster position. No sums of squares
+// then (a sample touches two pixels). Sums are FP64 and unscaled: the film
+// value is sum / spp in pixel mode, sum * w*h / num_samples in image mode.
 int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, double* film_sum,
                   double* film_sumsq, OrcPTStats* stats) {
     Scene S;
     init_scene(S, scene, static_cast<Accel*>(accel));
     if (S.num_lights == 0) return -1;
+    const bool image = P->primary_mode == 1;
+    if (P->primary_mode < 0 || P->primary_mode > 1 || (image && (film_sumsq || P->num_samples < 0))) return -2;
     const int w = scene->w, h = scene->h;
     const int64_t npix = int64_t(w) * h;
     if (P->nthreads > 0) omp_set_num_threads(P->nthreads);
     int64_t n_ext = 0, n_shadow = 0, n_vert = 0, e_nv = 0, e_nt = 0, s_nv = 0, s_nt = 0;
     const int mode = P->sampling_mode;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_ext, n_shadow, n_vert, e_nv, e_nt, s_nv, s_nt)
-    for (int64_t pix = 0; pix < npix; pix++) {
+    // pixel mode: one item per pixel, its spp samples; image mode: one item per sample
+    const int64_t n_items = image ? P->num_samples : npix;
+    const int chunk = image ? 64 : 1;
+#pragma omp parallel for schedule(dynamic, chunk) reduction(+ : n_ext, n_shadow, n_vert, e_nv, e_nt, s_nv, s_nt)
+    for (int64_t item = 0; item < n_items; item++) {
+        const int64_t pix = image ? item % npix : item;
+        const int s_lo = image ? int(item / npix) : 0;
+        const int s_hi = image ? s_lo + 1 : P->spp;
         const int x = int(pix % w), y = int(pix / w);
         const double dx = 1.0 / w, dy = 1.0 / h;
         double sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
-        for (int sidx = 0; sidx < P->spp; sidx++) {
+        for (int sidx = s_lo; sidx < s_hi; sidx++) {
             const uint32_t sample = uint32_t(sidx + P->sample_offset);
             V3 L(0);
+            int64_t tgt = pix;   // pixel of the walk's own contributions
+            auto splat = [&](int64_t px, V3 C) {
+                double* f = film_sum + 3 * px;
+#pragma omp atomic
+                f[0] += C.x;
+#pragma omp atomic
+                f[1] += C.y;
+#pragma omp atomic
+                f[2] += C.z;
+            };
             // camera endpoint (path::sample_position -> Camera_Pinhole::sample_position)
             Geom sp = make_degenerated(S.pos);
             bool sp_camera = true;
@@ -1073,7 +1127,7 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                 const Rand4 ra = rand4(uint32_t(pix), sample, uint32_t(nv - 1), 0, P->seed);
                 const Mat* m = sp_camera ? nullptr : &sp_mat;
                 const bool spec = m ? is_specular(*m, comp) : false;
-                const bool samplable_by_nee = mode != 0 && nv > 1 && !spec;
+                const bool samplable_by_nee = mode != 0 && (image || nv > 1) && !spec;
                 if (samplable_by_nee) {
                     const Rand4 rb = rand4(uint32_t(pix), sample, uint32_t(nv - 1), 1, P->seed);
                     // path::sample_direct (path.h:509-528)
@@ -1115,7 +1169,18 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     }
                     if (ok) {
                         n_shadow++;
-                        if (visible(S, sp, gL, &s_nv, &s_nt)) {
+                        if (sp_camera && visible(S, sp, gL, &s_nv, &s_nt)) {
+                            // the camera vertex (image mode): Camera_Pinhole::eval(wo) = J(wo)
+                            // inside the frustum, splatted at the light's raster position
+                            double rx, ry;
+                            const V3 wo = -wL;
+                            if (cam_raster(S, wo, rx, ry)) {
+                                const double J = cam_J(S, wo);
+                                double mis_w = 1;
+                                if (mode == 2) mis_w = balance_heuristic(pdf_direct_light(S, sp, gL, li, wL), J);
+                                splat(raster_to_pixel(w, h, rx, ry), thr * V3(J) * weight * mis_w);
+                            }
+                        } else if (!sp_camera && visible(S, sp, gL, &s_nv, &s_nt)) {
                             const V3 wo = -wL;
                             const V3 fs = mat_eval_delta(*m, sp, wi, wo, comp) * shading_normal_correction(sp, wi, wo);
                             if (!is_zero(fs)) {
@@ -1133,10 +1198,16 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                 // path::sample_direction
                 V3 wo, sw;
                 if (sp_camera) {
-                    const double ux = x * dx + dx * ra.u[2];
-                    const double uy = y * dy + dy * ra.u[3];
+                    const double ux = image ? ra.u[2] : x * dx + dx * ra.u[2];
+                    const double uy = image ? ra.u[3] : y * dy + dy * ra.u[3];
                     wo = primary_dir(S, ux, uy);
                     sw = V3(1);
+                    if (image) {
+                        // raster_pos = raster_position(wo) (renderer_pt.cpp:209-212)
+                        double rx = ux, ry = uy;
+                        (void)cam_raster(S, wo, rx, ry);
+                        tgt = raster_to_pixel(w, h, rx, ry);
+                    }
                 } else {
                     if (!mat_sample_direction(*m, sp, wi, comp, ra.u[2], ra.u[3], ul, wo, sw)) break;
                     sw = sw * shading_normal_correction(sp, wi, wo);
@@ -1155,7 +1226,10 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                     else fs = dot(woL, hit.g.n) <= 0 ? V3(0) : S.lights[hit_light].Ke;
                     double mis_w = 1;
                     if (mode != 0 && samplable_by_nee) {
-                        const double pdf_bsdf = mat_pdf_delta(*m, sp, wi, wo, comp);
+                        // Camera_Pinhole::pdf_direction at the camera vertex (image mode)
+                        double rx, ry;
+                        const double pdf_bsdf = sp_camera ? (cam_raster(S, wo, rx, ry) ? cam_J(S, wo) : 0.0)
+                                                          : mat_pdf_delta(*m, sp, wi, wo, comp);
                         const double pdf_light = pdf_direct_light(S, sp, hit.g, hit_light, woL);
                         mis_w = balance_heuristic(pdf_bsdf, pdf_light);
                     }
@@ -1180,16 +1254,21 @@ int orc_pt_render(const OrcScene* scene, void* accel, const OrcPTParams* P, doub
                 sp_mat = hit.mat;
                 comp = c;
             }
+            if (image) {
+                if (!is_zero(L)) splat(tgt, L);
+                continue;
+            }
             sx += L.x; sy += L.y; sz += L.z;
             qx += L.x * L.x; qy += L.y * L.y; qz += L.z * L.z;
         }
+        if (image) continue;
         film_sum[3 * pix + 0] += sx; film_sum[3 * pix + 1] += sy; film_sum[3 * pix + 2] += sz;
         if (film_sumsq) {
             film_sumsq[3 * pix + 0] += qx; film_sumsq[3 * pix + 1] += qy; film_sumsq[3 * pix + 2] += qz;
         }
     }
     if (stats) {
-        stats->samples += npix * P->spp;
+        stats->samples += image ? P->num_samples : npix * P->spp;
         stats->extension_rays += n_ext;
         stats->shadow_rays += n_shadow;
         stats->path_vertices += n_vert;

@@ -63,6 +63,9 @@ class OrcPTParams(ctypes.Structure):
         ('seed', ctypes.c_uint64),
         ('nthreads', ctypes.c_int32),
         ('sample_offset', ctypes.c_int32),
+        ('primary_mode', ctypes.c_int32),
+        ('pad', ctypes.c_int32),
+        ('num_samples', ctypes.c_int64),
     ]
 
 
@@ -270,13 +273,26 @@ def make_orc_scene(flat) -> tuple:
 
 def pt_render(flat, spp: int, max_verts: int, sampling_mode: int = 2, rr_depth: int = 5,
               seed: int = 0, nthreads: int = 0, accel: Accel = None, sample_offset: int = 0,
-              sumsq: bool = True):
+              sumsq: bool = True, primary_mode: str = 'pixel', num_samples: int = None):
     """renderer::pt restated in FP64. Returns dict with 'sum', 'sumsq'
-    ([h,w,3] float64 unscaled), 'image' (= sum/spp) and 'stats'."""
+    ([h,w,3] float64 unscaled), 'image' (the film value) and 'stats'.
+    primary_mode is the reference's primary_ray_sampling_mode, 'pixel' or
+    'image'. Pixel mode renders spp samples per pixel (image = sum / spp).
+    Image mode renders num_samples samples over the image (default
+    spp * w * h; image = sum * w*h / num_samples) and has no 'sumsq' (a
+    sample splats into two pixels)."""
+    if primary_mode not in ('pixel', 'image'):
+        raise ValueError(f'primary_mode {primary_mode!r}')
+    image = primary_mode == 'image'
+    sumsq = sumsq and not image
     if accel is None:
         accel = Accel(flat.verts, nthreads)
     s, keep = make_orc_scene(flat)
-    P = OrcPTParams(spp, max_verts, sampling_mode, rr_depth, seed, nthreads, sample_offset)
+    npix = flat.w * flat.h
+    if num_samples is None:
+        num_samples = spp * npix
+    P = OrcPTParams(spp, max_verts, sampling_mode, rr_depth, seed, nthreads, sample_offset, int(image), 0,
+                    int(num_samples) if image else 0)
     npix = flat.w * flat.h
     fsum = np.zeros(npix * 3, dtype=np.float64)
     fsq = np.zeros(npix * 3, dtype=np.float64) if sumsq else None
@@ -285,7 +301,8 @@ def pt_render(flat, spp: int, max_verts: int, sampling_mode: int = 2, rr_depth: 
                              fsq.ctypes.data if fsq is not None else None, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f'orc_pt_render failed ({rc})')
-    out = {'sum': fsum.reshape(flat.h, flat.w, 3), 'image': fsum.reshape(flat.h, flat.w, 3) / spp,
+    scale = npix / num_samples if image else 1.0 / spp
+    out = {'sum': fsum.reshape(flat.h, flat.w, 3), 'image': fsum.reshape(flat.h, flat.w, 3) * scale,
            'stats': {'samples': st.samples, 'extension_rays': st.extension_rays,
                      'shadow_rays': st.shadow_rays, 'path_vertices': st.path_vertices,
                      'extension_node_visits': st.extension_node_visits,
