@@ -224,23 +224,73 @@ LMGPU_HD float qplane(uint32_t q, int c, float step, float origin) {
     return std::fma(float((q >> (8 * c)) & 0xffu), step, origin);
 }
 
+// Slab keys of the W children of a quantised node. The slab distance of
+// grid plane q on axis x is computed as one fma, t = q * ax + bx, with
+//   ax = step_x * inv_x   (exact: the step is a power of two),
+//   bx = (ox - o_x) * inv_x,
+// and no decoded plane: the planes are the exact reals ox + q * step. Each
+// axis picks its near and far plane by the sign of inv_x, so one fma gives
+// the entry and one the exit distance (no per-axis min / max). Bounds, with
+// u = 2^-24: bx is within 2u |bx| of (ox - o_x) * inv_x; the exact plane
+// ox + q step lies within u (|ox| + 255 step) of the FP32 plane the builder
+// rounded outward (quantise_node); the fma and inv_x round relatively, which
+// the 1 + 2 gamma(3) factor on the exit distance covers (box_near). The entry
+// offset is lowered and the exit offset raised by
+//   Ex = 16u (|bx| + |inv_x| |ox| + 255 |ax|),
+// which exceeds the absolute terms with their own rounding, so every child
+// box the FP32 test of the decoded planes would hit is hit here too.
 template <int W>
-__device__ __forceinline__ void child_keys(const QNode<W>& nd, const RayPre& r, float tmin, float tmax,
-                                           float (&key)[W], int (&ent)[W]) {
-    const float sx = __uint_as_float((nd.e & 0xffu) << 23);
-    const float sy = __uint_as_float(((nd.e >> 8) & 0xffu) << 23);
-    const float sz = __uint_as_float(((nd.e >> 16) & 0xffu) << 23);
+LMGPU_HD void child_keys(const QNode<W>& nd, const RayPre& r, float tmin, float tmax, float (&key)[W],
+                         int (&ent)[W]) {
+    constexpr float kSlack = 0x1p-20f;   // 16u
+    const float org[3] = {nd.ox, nd.oy, nd.oz};
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    const float inv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    float a[3], bn[3], bf[3];
+    uint32_t wn[3][W / 4], wf[3][W / 4];
+    // both plane words of every axis as loaded (whole 16-B loads of the node);
+    // the empty asm keeps the compiler from turning the sign choice below into
+    // per-lane dword loads from chosen addresses
+    uint32_t ql[3][W / 4], qh[3][W / 4];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int w = 0; w < W / 4; w++) {
+            ql[k][w] = nd.qlo[k][w];
+            qh[k][w] = nd.qhi[k][w];
+#if defined(__HIP_DEVICE_COMPILE__)
+            asm("" : "+v"(ql[k][w]), "+v"(qh[k][w]));
+#endif
+        }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float step = __builtin_bit_cast(float, ((nd.e >> (8 * k)) & 0xffu) << 23);
+        a[k] = step * inv[k];
+        const float b = (org[k] - o[k]) * inv[k];
+        const float E = (fabsf(b) + fmaf(fabsf(org[k]), fabsf(inv[k]), 255.f * fabsf(a[k]))) * kSlack;
+        bn[k] = b - E;
+        bf[k] = b + E;
+        const bool pos = inv[k] >= 0.f;
+#pragma unroll
+        for (int w = 0; w < W / 4; w++) {
+            wn[k][w] = pos ? ql[k][w] : qh[k][w];
+            wf[k][w] = pos ? qh[k][w] : ql[k][w];
+        }
+    }
 #pragma unroll
     for (int c = 0; c < W; c++) {
-        bool hit;
-        const int w = c >> 2, b = c & 3;
-        const float tn = box_near(r, qplane(nd.qlo[0][w], b, sx, nd.ox), qplane(nd.qlo[1][w], b, sy, nd.oy),
-                                  qplane(nd.qlo[2][w], b, sz, nd.oz), qplane(nd.qhi[0][w], b, sx, nd.ox),
-                                  qplane(nd.qhi[1][w], b, sy, nd.oy), qplane(nd.qhi[2][w], b, sz, nd.oz), tmin, tmax,
-                                  hit);
+        const int w = c >> 2, sh = 8 * (c & 3);
+        float tn[3], tf[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            tn[k] = fmaf(float((wn[k][w] >> sh) & 0xffu), a[k], bn[k]);
+            tf[k] = fmaf(float((wf[k][w] >> sh) & 0xffu), a[k], bf[k]);
+        }
+        const float n = fmaxf(fmaxf(tn[0], tn[1]), fmaxf(tn[2], tmin));
+        const float f = fminf(fminf(tf[0], tf[1]), fminf(tf[2], tmax));
         const int e = nd.ent[c];
-        hit = hit && e != kQEmpty;
-        key[c] = hit ? tn : __builtin_inff();
+        const bool hit = n <= f * 1.00000036f && e != kQEmpty;
+        key[c] = hit ? n : __builtin_inff();
         ent[c] = e;
     }
 }

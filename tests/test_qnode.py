@@ -3,8 +3,13 @@ qplane, compiled with hipcc, decodes 4- and 8-wide boxes that contain the
 exact child boxes on 200,000 random nodes of each width (1e-7..1e3 extents,
 offsets up to 5e4, flat boxes, partly empty nodes), stays within one grid
 step of them, and encodes stack entries and empty slots as the traversal
-reads them. GPU hit parity
-with the quantised traversal is covered by the accel tests."""
+reads them. The traversal's slab test of a quantised node (child_keys: one
+fma per plane from the exact grid, the near and far plane picked by the
+direction's sign, offsets widened by a rounding bound) never misses a child
+whose exact box the ray meets: 2.3 million rays aimed at faces, edges and
+corners, starting on the box's planes or parallel to them, decided in exact
+arithmetic (with the widening removed, the first node already fails). GPU
+hit parity with the quantised traversal is covered by the accel tests."""
 import os
 import shutil
 import subprocess
