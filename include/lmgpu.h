@@ -77,7 +77,8 @@ typedef struct lmgpu_build_opts {
     int32_t width;       /* BVH branching factor: 2, 4 or 8 (0 = 4) */
     int32_t leaf_size;   /* max triangles per leaf, 1..15 (0 = 2) */
     int32_t refill;      /* traversal: a wave takes new rays once this many of its
-                            64 lanes are idle, 1..64 (0 = 32; 64 suits coherent rays) */
+                            64 lanes are idle, 1..64 (0 = 8 for lmgpu_accel_*, 32 for a
+                            scene's renderers; 64 suits coherent rays) */
     int32_t builder;     /* LMGPU_BUILDER_LBVH (1) or LMGPU_BUILDER_PLOC (0); with opts == NULL: LBVH */
     int32_t ploc_radius; /* PLOC nearest-neighbour window, 1..64 (0 = 16) */
     int32_t collapse;    /* LMGPU_COLLAPSE_SAH (0) or LMGPU_COLLAPSE_GREEDY (1) */

@@ -615,7 +615,7 @@ static void launch_collapse(bool sah_collapse, const CollapseItem* in, const int
 int Accel::build(const float* verts_in, int64_t n, bool on_device, const lmgpu_build_opts* opts, hipStream_t s) {
     int W = opts && opts->width ? opts->width : 4;
     int leaf = opts && opts->leaf_size ? opts->leaf_size : kDefaultLeafSize;
-    const int rf = opts && opts->refill ? opts->refill : 32;
+    const int rf = opts && opts->refill ? opts->refill : kDefaultRefill;
     if (rf < 1 || rf > 64) return fail(LMGPU_ERR_INVALID_ARGUMENT, "refill must be in [1, 64]");
     const int builder = opts ? opts->builder : LMGPU_BUILDER_LBVH;
     if (builder != LMGPU_BUILDER_PLOC && builder != LMGPU_BUILDER_LBVH)

@@ -351,7 +351,7 @@ int lmgpu_accel_deserialize(lmgpu_accel* accel, const void* buf, size_t size, co
     a.num_nodes = h.num_nodes;
     LMGPU_TRY(a.make_trav_nodes(nullptr));
     a.leaf_size = int(h.leaf_size);
-    a.refill = (opts && opts->refill > 0) ? std::min(opts->refill, 64) : 32;
+    a.refill = (opts && opts->refill > 0) ? std::min(opts->refill, 64) : lmgpu::kDefaultRefill;
     a.num_tris = h.num_tris;
     a.num_nodes = h.num_nodes;
     a.num_leaves = h.num_leaves;

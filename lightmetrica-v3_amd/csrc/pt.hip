@@ -510,6 +510,7 @@ int Scene::create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts) {
         if (d->mat_type[i] == LMGPU_MAT_MIXTURE && !d->mat_ks)
             return fail(LMGPU_ERR_INVALID_ARGUMENT, "mixture materials need mat_ks");
     LMGPU_TRY(accel.build(d->verts, N, false, opts, nullptr));
+    if (!(opts && opts->refill)) accel.refill = kDefaultSceneRefill;
     refill_secondary = accel.refill;
     w = d->w;
     h = d->h;

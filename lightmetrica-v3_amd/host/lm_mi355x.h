@@ -227,7 +227,7 @@ public:
 struct AccelOptions {
     int width = 4;        // 2, 4 or 8
     int leaf_size = 2;    // 1..15
-    int refill = 32;      // traversal wave refill threshold
+    int refill = 0;       // traversal wave refill threshold (0: the library default, 8)
     bool ploc = false;    // PLOC instead of LBVH
     int ploc_radius = 16;
 };

@@ -65,9 +65,10 @@ def _stream_ptr(stream) -> Optional[int]:
 
 
 class Accel:
-    """accel::mi355x: GPU LBVH collapsed to a W-wide BVH + ordered traversal."""
+    """accel::mi355x: GPU LBVH collapsed to a W-wide BVH + ordered traversal.
+    refill 0 takes the library's default traversal refill threshold (8)."""
 
-    def __init__(self, width: int = 4, leaf_size: int = 2, refill: int = 32, builder: int = 1, radius: int = 16,
+    def __init__(self, width: int = 4, leaf_size: int = 2, refill: int = 0, builder: int = 1, radius: int = 16,
                  collapse: int = 0, sah_tri_cost: float = 0.0):
         self.width = width
         self.leaf_size = leaf_size
