@@ -38,8 +38,6 @@
 
 namespace lmgpu {
 
-namespace {
-
 struct JobDev {
     SceneDev sc;          // job-local pointers into the shading arenas
     int root;             // root of the job's BVH in the node arena
@@ -79,6 +77,8 @@ struct BatchCtl {
     unsigned long long t_start;     // device clock at the start of the run
     unsigned long long iterations;  // iterations that found work left
 };
+
+namespace {
 
 __device__ __forceinline__ uint64_t job_seed(const JobDev& J) { return (uint64_t(J.seed_hi) << 32) | J.seed_lo; }
 
@@ -375,6 +375,37 @@ struct BatchJob {
     uint64_t seed = 0;
 };
 
+// Path pool, work queue and the loop that drains it (regenerate -> extend ->
+// shade -> connect, two iterations per graph replay), shared by the batch
+// and by lmgpu_pt_render's regenerating path.
+struct Wavefront {
+    int cap = 0;
+    DevBuf<int32_t> pjob, ppix, psample, pstate;
+    DevBuf<float4> pthr, pL, pprev;
+    DevBuf<int4> pinfo;
+    DevBuf<lmgpu_ray> ext0, ext1, sh;
+    DevBuf<int32_t> path0, path1, counts;
+    DevBuf<lmgpu_hit> hit;
+    DevBuf<float4> shc;
+    DevBuf<BatchCtl> ctl;
+    BatchCtl* ctl_mirror = nullptr;   // pinned host copies of ctl, one per chunk parity
+    DevBuf<unsigned long long> next;
+    // per-run tables: jobs, work segments, per-job completion counters and clocks
+    DevBuf<JobDev> djobs;
+    DevBuf<WorkSeg> dsegs;
+    DevBuf<unsigned long long> job_done, job_time;
+    ~Wavefront() {
+        if (ctl_mirror) (void)hipHostFree(ctl_mirror);
+    }
+    int ensure_pool(int cap);
+    // Uploads the jobs and the work segments (queue order) of one run.
+    int set_work(const std::vector<JobDev>& jobs, const std::vector<WorkSeg>& segs, hipStream_t s);
+    // Renders every work item of the tables set last into film_sum / film_sq
+    // (FP64 atomics; film_sq may be null) and waits for the end.
+    int run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
+            double* film_sum, double* film_sq, double* elapsed, int64_t* iters, hipStream_t s);
+};
+
 struct Batch {
     std::vector<BatchJob> jobs;
     // arena (valid for arena_width / arena_leaf)
@@ -393,33 +424,15 @@ struct Batch {
     std::vector<JobDev> jobdev;   // per job, film_off filled per run
     const void* trav_nodes() const { return qnodes.p ? static_cast<const void*>(qnodes.p) : nodes.p; }
     // run state
-    DevBuf<JobDev> djobs;
-    DevBuf<WorkSeg> dsegs;
-    DevBuf<unsigned long long> job_done, job_time;
+    Wavefront wf;
     DevBuf<double> film, film_sq;
     int64_t film_doubles = 0;
     std::vector<double> latency_ms;
     bool have_results = false;
-    // path pool
-    int cap = 0;
-    DevBuf<int32_t> pjob, ppix, psample, pstate;
-    DevBuf<float4> pthr, pL, pprev;
-    DevBuf<int4> pinfo;
-    DevBuf<lmgpu_ray> ext0, ext1, sh;
-    DevBuf<int32_t> path0, path1, counts;
-    DevBuf<lmgpu_hit> hit;
-    DevBuf<float4> shc;
-    DevBuf<BatchCtl> ctl;
-    BatchCtl* ctl_mirror = nullptr;   // pinned host copies of ctl, one per chunk parity
-    ~Batch() {
-        if (ctl_mirror) (void)hipHostFree(ctl_mirror);
-    }
-    DevBuf<unsigned long long> next;
     // autotuned choice
     int tuned_width = 0, tuned_cap = 0, tuned_block = 0;
 
     int build_arena(int width, int leaf, hipStream_t s);
-    int ensure_pool(int cap);
     bool sumsq = false;   // the last run accumulated sums of squares
     int run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, int queue_order,
                  double* elapsed, int64_t* iters, hipStream_t s);
@@ -554,7 +567,7 @@ int Batch::build_arena(int width, int leaf, hipStream_t s) {
     return LMGPU_OK;
 }
 
-int Batch::ensure_pool(int c) {
+int Wavefront::ensure_pool(int c) {
     if (cap == c) return LMGPU_OK;
     cap = 0;
     LMGPU_TRY(pjob.alloc(c)); LMGPU_TRY(ppix.alloc(c)); LMGPU_TRY(psample.alloc(c)); LMGPU_TRY(pstate.alloc(c));
@@ -568,65 +581,26 @@ int Batch::ensure_pool(int c) {
     return LMGPU_OK;
 }
 
-// Renders every job once with the current arena and pool: at most spp_cap
-// samples per pixel per job when spp_cap > 0 (autotuning trials).
-int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, int queue_order,
-                    double* elapsed, int64_t* iters, hipStream_t s) {
+int Wavefront::set_work(const std::vector<JobDev>& jobs, const std::vector<WorkSeg>& segs, hipStream_t s) {
     const int nj = int(jobs.size());
-    int64_t total = 0, film_n = 0;
-    int max_spp = 0;
-    for (int j = 0; j < nj; j++) {
-        JobDev& J = jobdev[j];
-        const BatchJob& B = jobs[j];
-        J.npix = B.desc.w * B.desc.h;
-        J.spp = spp_cap > 0 ? std::min(B.spp, spp_cap) : B.spp;
-        J.max_verts = B.max_verts;
-        J.mode = B.mode;
-        J.rr_depth = B.rr_depth;
-        J.seed_lo = uint32_t(B.seed);
-        J.seed_hi = uint32_t(B.seed >> 32);
-        J.film_off = film_n;
-        total += int64_t(J.npix) * J.spp;
-        film_n += int64_t(J.npix) * 3;
-        max_spp = std::max(max_spp, J.spp);
-    }
-    // the work queue as segments of one (job, sample) each, in queue order
-    std::vector<WorkSeg> segs;
-    int64_t cursor = 0;
-    auto add_seg = [&](int j, int smp) {
-        segs.push_back(WorkSeg{cursor, j, smp});
-        cursor += jobdev[j].npix;
-    };
-    if (queue_order == LMGPU_QUEUE_INTERLEAVED) {
-        for (int smp = 0; smp < max_spp; smp++)
-            for (int j = 0; j < nj; j++)
-                if (smp < jobdev[j].spp) add_seg(j, smp);
-    } else {
-        for (int j = 0; j < nj; j++)
-            for (int smp = 0; smp < jobdev[j].spp; smp++) add_seg(j, smp);
-    }
-    if (cursor != total || segs.size() >= size_t(1) << 31)
-        return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch work queue too large");
-    const int nsegs = int(segs.size());
     if (djobs.n != size_t(nj)) LMGPU_TRY(djobs.alloc(nj));
     if (dsegs.n != segs.size()) LMGPU_TRY(dsegs.alloc(segs.size()));
     if (job_done.n != size_t(nj)) {
         LMGPU_TRY(job_done.alloc(nj));
         LMGPU_TRY(job_time.alloc(nj));
     }
-    if (film_doubles != film_n) {
-        LMGPU_TRY(film.alloc(film_n));
-        LMGPU_TRY(film_sq.alloc(film_n));
-        film_doubles = film_n;
-    }
     LMGPU_HIP(hipStreamSynchronize(s));
-    LMGPU_HIP(hipMemcpy(djobs.p, jobdev.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice));
+    LMGPU_HIP(hipMemcpy(djobs.p, jobs.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice));
     LMGPU_HIP(hipMemcpy(dsegs.p, segs.data(), sizeof(WorkSeg) * segs.size(), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_done.p, int64_t(nj));
     hipLaunchKernelGGL(k_b_fill_u64, dim3(nblk(nj, 256)), dim3(256), 0, s, job_time.p, int64_t(nj));
-    hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film.p, film_n);
-    if (with_sumsq) hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film_sq.p, film_n);
-    double* fsq = with_sumsq ? film_sq.p : nullptr;
+    LMGPU_HIP(hipGetLastError());
+    return LMGPU_OK;
+}
+
+int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
+                   double* film_sum, double* fsq, double* elapsed, int64_t* iters, hipStream_t s) {
+    const int nsegs = int(dsegs.n);
     hipLaunchKernelGGL(k_b_fill_i32, dim3(nblk(cap, 256)), dim3(256), 0, s, pstate.p, int64_t(cap), kFree);
     hipLaunchKernelGGL(k_b_init, dim3(1), dim3(64), 0, s, ctl.p, total, counts.p);
 
@@ -643,33 +617,33 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
         const unsigned rg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)));
         if (block == 128)
             hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dsegs.p, nsegs, total, P, q, in_q, cap,
-                               ctl.p, job_done.p, job_time.p, film.p, fsq);
+                               ctl.p, job_done.p, job_time.p, film_sum, fsq);
         else
             hipLaunchKernelGGL(k_b_regen<256>, dim3(rg), dim3(256), 0, cs, djobs.p, dsegs.p, nsegs, total, P, q, in_q, cap,
-                               ctl.p, job_done.p, job_time.p, film.p, fsq);
+                               ctl.p, job_done.p, job_time.p, film_sum, fsq);
         hipLaunchKernelGGL(k_b_zero2, dim3(1), dim3(64), 0, cs, next.p);
         if (width == 2)
             hipLaunchKernelGGL(k_b_extend<2>, dim3(occupancy_grid(k_b_extend<2>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<2>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<2>::type*>(trav_nodes), tris, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         else if (width == 4)
             hipLaunchKernelGGL(k_b_extend<4>, dim3(occupancy_grid(k_b_extend<4>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<4>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<4>::type*>(trav_nodes), tris, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         else
             hipLaunchKernelGGL(k_b_extend<8>, dim3(occupancy_grid(k_b_extend<8>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<8>::type*>(trav_nodes()), tris.p, djobs.p, P, q, in_q, next.p, refill, ctl.p);
+                               static_cast<const TravNode<8>::type*>(trav_nodes), tris, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, cs, counts.p, in_q ^ 1);
         const unsigned sg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 2048 * 256 / block)));
         if (block == 128) hipLaunchKernelGGL(k_b_shade<128>, dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q);
         else hipLaunchKernelGGL(k_b_shade<256>, dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q);
         if (width == 2)
             hipLaunchKernelGGL(k_b_connect<2>, dim3(occupancy_grid(k_b_connect<2>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<2>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<2>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
         else if (width == 4)
             hipLaunchKernelGGL(k_b_connect<4>, dim3(occupancy_grid(k_b_connect<4>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<4>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<4>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
         else
             hipLaunchKernelGGL(k_b_connect<8>, dim3(occupancy_grid(k_b_connect<8>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<8>::type*>(trav_nodes()), tris.p, djobs.p, P, q, next.p + 1, refill, ctl.p);
+                               static_cast<const TravNode<8>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
     }
     hipError_t e = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {
@@ -682,8 +656,7 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
     (void)hipStreamDestroy(cs);
     if (e != hipSuccess) return fail(LMGPU_ERR_DEVICE, std::string("batch graph instantiate: ") + hipGetErrorString(e));
     // every work item needs at most max_verts + 1 iterations in its slot
-    int max_v = 2;
-    for (const BatchJob& B : jobs) max_v = std::max(max_v, B.max_verts);
+    const int max_v = std::max(2, max_verts);
     const int64_t iter_cap = (total / cap + 2) * (max_v + 2) + 64;
     hipEvent_t e0, e1;
     LMGPU_HIP(hipEventCreate(&e0));
@@ -754,12 +727,65 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
     if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during rendering");
     *elapsed = ms * 1e-3;
     *iters = it;
+    return LMGPU_OK;
+}
+
+// Renders every job once with the current arena and pool: at most spp_cap
+// samples per pixel per job when spp_cap > 0 (autotuning trials).
+int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool with_sumsq, int queue_order,
+                    double* elapsed, int64_t* iters, hipStream_t s) {
+    const int nj = int(jobs.size());
+    int64_t total = 0, film_n = 0;
+    int max_spp = 0, max_v = 2;
+    for (int j = 0; j < nj; j++) {
+        JobDev& J = jobdev[j];
+        const BatchJob& B = jobs[j];
+        J.npix = B.desc.w * B.desc.h;
+        J.spp = spp_cap > 0 ? std::min(B.spp, spp_cap) : B.spp;
+        J.max_verts = B.max_verts;
+        J.mode = B.mode;
+        J.rr_depth = B.rr_depth;
+        J.seed_lo = uint32_t(B.seed);
+        J.seed_hi = uint32_t(B.seed >> 32);
+        J.film_off = film_n;
+        total += int64_t(J.npix) * J.spp;
+        film_n += int64_t(J.npix) * 3;
+        max_spp = std::max(max_spp, J.spp);
+        max_v = std::max(max_v, B.max_verts);
+    }
+    // the work queue as segments of one (job, sample) each, in queue order
+    std::vector<WorkSeg> segs;
+    int64_t cursor = 0;
+    auto add_seg = [&](int j, int smp) {
+        segs.push_back(WorkSeg{cursor, j, smp});
+        cursor += jobdev[j].npix;
+    };
+    if (queue_order == LMGPU_QUEUE_INTERLEAVED) {
+        for (int smp = 0; smp < max_spp; smp++)
+            for (int j = 0; j < nj; j++)
+                if (smp < jobdev[j].spp) add_seg(j, smp);
+    } else {
+        for (int j = 0; j < nj; j++)
+            for (int smp = 0; smp < jobdev[j].spp; smp++) add_seg(j, smp);
+    }
+    if (cursor != total || segs.size() >= size_t(1) << 31)
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "batch work queue too large");
+    if (film_doubles != film_n) {
+        LMGPU_TRY(film.alloc(film_n));
+        LMGPU_TRY(film_sq.alloc(film_n));
+        film_doubles = film_n;
+    }
+    LMGPU_TRY(wf.set_work(jobdev, segs, s));
+    hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film.p, film_n);
+    if (with_sumsq) hipLaunchKernelGGL(k_b_fill_f64, dim3(nblk(film_n, 256)), dim3(256), 0, s, film_sq.p, film_n);
+    LMGPU_TRY(wf.run(trav_nodes(), tris.p, width, block, total, max_v, film.p, with_sumsq ? film_sq.p : nullptr,
+                     elapsed, iters, s));
     if (keep_results) {
         // job latency = device clock at the job's last sample - clock at the start
         std::vector<unsigned long long> t(nj);
         BatchCtl hc;
-        LMGPU_HIP(hipMemcpy(t.data(), job_time.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost));
-        LMGPU_HIP(hipMemcpy(&hc, ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(t.data(), wf.job_time.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&hc, wf.ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
         int dev = 0, khz = 0;
         LMGPU_HIP(hipGetDevice(&dev));
         LMGPU_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -769,6 +795,47 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
         have_results = true;
         sumsq = with_sumsq;
     }
+    return LMGPU_OK;
+}
+
+void wavefront_destroy(Wavefront* w) { delete w; }
+
+int Scene::render_regen(const PTConsts& c, int spp, int sample_offset, double* sum, double* sumsq, int64_t* ext_rays,
+                        int64_t* shadow_rays, hipStream_t s) {
+    if (!wf) {
+        wf = new (std::nothrow) Wavefront();
+        if (!wf) return fail(LMGPU_ERR_OUT_OF_MEMORY, "wavefront");
+    }
+    JobDev J{};
+    J.sc = dev;
+    J.root = 0;
+    J.npix = c.npix;
+    J.spp = spp;
+    J.max_verts = c.max_verts;
+    J.mode = c.mode;
+    J.rr_depth = c.rr_depth;
+    J.seed_lo = c.seed_lo;
+    J.seed_hi = c.seed_hi;
+    J.film_off = 0;
+    const int64_t total = int64_t(c.npix) * spp;
+    std::vector<WorkSeg> segs(static_cast<size_t>(spp));
+    for (int k = 0; k < spp; k++) segs[size_t(k)] = WorkSeg{int64_t(k) * c.npix, 0, sample_offset + k};
+    // pool: two paths per pixel (the slots of a sample's short paths take the
+    // next sample's pixels), at most 2^22 slots, at least 4096
+    int64_t want = std::min<int64_t>(total, 2 * int64_t(c.npix));
+    int cap = 4096;
+    while (cap < want && cap < (1 << 22)) cap <<= 1;
+    LMGPU_TRY(wf->ensure_pool(cap));
+    LMGPU_TRY(wf->set_work(std::vector<JobDev>{J}, segs, s));
+    const void* tn = accel.width == 4 ? static_cast<const void*>(dev.nodes4)
+                     : accel.width == 8 ? static_cast<const void*>(dev.nodes8) : static_cast<const void*>(dev.nodes2);
+    double sec = 0;
+    int64_t iters = 0;
+    LMGPU_TRY(wf->run(tn, accel.tris.p, accel.width, 128, total, c.max_verts, sum, sumsq, &sec, &iters, s));
+    BatchCtl hc;
+    LMGPU_HIP(hipMemcpy(&hc, wf->ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+    *ext_rays = int64_t(hc.ext_rays);
+    *shadow_rays = int64_t(hc.shadow_rays);
     return LMGPU_OK;
 }
 
@@ -888,7 +955,7 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
                 const auto tb0 = now();
                 LMGPU_TRY(b.build_arena(w, leaf, s));
                 tb += std::chrono::duration<double>(now() - tb0).count();
-                LMGPU_TRY(b.ensure_pool(c));
+                LMGPU_TRY(b.wf.ensure_pool(c));
                 int64_t it;
                 return b.run_once(w, bl, 1, false, o.film_sumsq != 0, o.queue_order, sec, &it, s);
             };
@@ -923,14 +990,14 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
     const auto tb0 = now();
     LMGPU_TRY(b.build_arena(width, leaf, s));
     build_s += std::chrono::duration<double>(now() - tb0).count();
-    LMGPU_TRY(b.ensure_pool(capv));
+    LMGPU_TRY(b.wf.ensure_pool(capv));
     double sec = 0;
     int64_t iters = 0;
     LMGPU_TRY(b.run_once(width, block, 0, true, o.film_sumsq != 0, o.queue_order, &sec, &iters, s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         lmgpu::BatchCtl hc;
-        LMGPU_HIP(hipMemcpy(&hc, b.ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        LMGPU_HIP(hipMemcpy(&hc, b.wf.ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
         int64_t samples = 0;
         for (const auto& J : b.jobs) samples += int64_t(J.desc.w) * J.desc.h * J.spp;
         stats->jobs = int64_t(b.jobs.size());

@@ -836,7 +836,11 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (image && gp) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering uses the pixel sampling mode");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
-    LMGPU_TRY(ensure_workspace(npix));
+    // fixed-spp pixel-mode renders of 2 or more samples without a guide go
+    // through the regenerating wavefront; 1-spp calls, the time scheduler,
+    // the image mode and guided rendering replay the one-sample pass graph
+    const bool regen = !image && !gp && p->render_time <= 0 && p->spp >= 2;
+    if (!regen) LMGPU_TRY(ensure_workspace(npix));
     GuidedPass gpv;
     if (gp) {
         LMGPU_TRY(ensure_guided(npix, *gp));
@@ -867,6 +871,24 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     c.seed_hi = uint32_t(p->seed >> 32);
     c.npix = npix;
     c.image = image ? 1 : 0;
+    if (regen) {
+        int64_t ext = 0, shd = 0;
+        LMGPU_TRY(render_regen(c, p->spp, p->sample_offset, sum, sumsq, &ext, &shd, s));
+        if (!on_device) {
+            LMGPU_HIP(hipMemcpyAsync(sum_in, sum, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+            if (sumsq_in) LMGPU_HIP(hipMemcpyAsync(sumsq_in, sumsq, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+        }
+        LMGPU_HIP(hipStreamSynchronize(s));
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->samples = int64_t(npix) * p->spp;
+            st->spp = p->spp;
+            st->extension_rays = ext;
+            st->shadow_rays = shd;
+            st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return LMGPU_OK;
+    }
     // stats counters [8..15] as 64-bit: [8] samples, [10] extension rays, [12] shadow rays,
     // [14] guide training records
     LMGPU_HIP(hipMemsetAsync(counts.p + 8, 0, 8 * sizeof(int32_t), s));

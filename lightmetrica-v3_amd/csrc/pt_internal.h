@@ -23,6 +23,10 @@ struct GuidedPass {
     int batch = 0, steps = 0, rec_depth = 0;
 };
 
+// Path pool and work-queue loop of the regenerating wavefront (batch.hip).
+struct Wavefront;
+void wavefront_destroy(Wavefront* w);
+
 struct Scene {
     Accel accel;
     DevBuf<ShadeTri> stri;
@@ -59,8 +63,11 @@ struct Scene {
     double* graph_sum_ptr = nullptr;
     double* graph_sq_ptr = nullptr;
 
+    Wavefront* wf = nullptr;   // regenerating path of fixed-spp renders, created on first use
+
     ~Scene() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        wavefront_destroy(wf);
     }
 
     int create(const lmgpu_scene_desc* d, const lmgpu_build_opts* opts);
@@ -70,6 +77,13 @@ struct Scene {
     int render(const lmgpu_pt_params* p, double* sum, double* sumsq, bool on_device, const GuidedPass* gp,
                lmgpu_pt_stats* st, hipStream_t s);
     int raycast(const lmgpu_raycast_params* p, double* film, bool on_device, hipStream_t s);
+    // A fixed-spp pixel-mode render through the regenerating wavefront: a pool of
+    // path slots that take the render's (sample, pixel) items in order and add a
+    // finished sample to the film at once (FP64 atomics), so paths of later samples
+    // fill the slots that short paths free instead of each bounce waiting for a
+    // pass's longest paths. Same Philox streams and per-vertex code as the passes.
+    int render_regen(const PTConsts& c, int spp, int sample_offset, double* sum, double* sumsq, int64_t* ext_rays,
+                     int64_t* shadow_rays, hipStream_t s);
     // renderer::lt and renderer::bdpt (lt.hip)
     DevBuf<float4> blverts;            // BDPT light vertices, [samples per pass][max_verts - 1]
     DevBuf<lmgpu_ray> bsh;             // BDPT connection queue
