@@ -82,30 +82,96 @@ namespace {
 
 __device__ __forceinline__ uint64_t job_seed(const JobDev& J) { return (uint64_t(J.seed_hi) << 32) | J.seed_lo; }
 
+// Most grid-stride iterations of k_b_regen per block (the host sizes the grid
+// so that a block covers at most this many BLOCK-slot strides of the pool).
+constexpr int kRegenMaxIter = 32;
+// Entries of k_b_regen's per-block table of finished-sample counts per job.
+constexpr int kRegenJobSlots = 64;
+
 // Frees finishing paths (their sample goes to the job film) and hands the
 // free slots new work items, appending their camera rays to queue in_q.
+// The shared counters are updated once per block, not once per wave: a
+// first sweep over the block's slots counts the slots that will take work
+// (per stride and wave, in LDS), one atomic on work_next then reserves the
+// block's items and one on the queue count its queue entries; finished
+// samples are counted per job in an LDS table and added to the job's and
+// the run's counters at the end of the block.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jobs, const WorkSeg* __restrict__ segs,
                                                    int nsegs, int64_t total, PoolDev P, BatchQ q, int in_q, int cap,
                                                    BatchCtl* ctl, unsigned long long* job_done,
                                                    unsigned long long* job_time, double* film_sum, double* film_sq) {
+    constexpr int kWaves = BLOCK / 64;
+    __shared__ int s_want[kRegenMaxIter][kWaves];   // slots taking work, then their exclusive prefix
+    __shared__ unsigned long long s_base;           // first work item of the block
+    __shared__ int s_qbase;                         // first queue entry of the block
+    __shared__ int s_emit;                          // queue entries of the block
+    __shared__ int s_key[kRegenJobSlots];
+    __shared__ unsigned int s_cnt[kRegenJobSlots];
+    __shared__ unsigned int s_fin;
+    __shared__ int s_idle;
     // Every item accumulated: no slot is finishing and nothing is left to
     // hand out, so this and the rest of the iteration's kernels are no-ops
-    // (the host enqueues iterations up to a bound without watching).
-    if (__hip_atomic_load(&ctl->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    // (the host enqueues iterations up to a bound without watching). One
+    // thread reads the counter, so the whole block takes the same branch
+    // (other blocks of this launch may lower it meanwhile).
+    if (threadIdx.x == 0)
+        s_idle = __hip_atomic_load(&ctl->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __syncthreads();
+    if (s_idle) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->iterations, 1ull);
     const int stride = gridDim.x * BLOCK;
-    const int n_iter = (cap + stride - 1) / stride;
+    const int n_iter = (cap + stride - 1) / stride;   // <= kRegenMaxIter (host)
     const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < kRegenJobSlots; k += BLOCK) {
+        s_key[k] = -1;
+        s_cnt[k] = 0u;
+    }
+    if (threadIdx.x == 0) s_fin = 0u;
+    // sweep 1: slots that take work (free now or freed below)
     for (int it = 0; it < n_iter; it++) {
         const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool want = false, fin = false;
-        int fj = -1;
+        bool want = false;
+        if (s < cap) {
+            const int32_t st = P.state[s];
+            want = st == kFinishing || st == kFree;
+        }
+        const unsigned long long m = __ballot(want);
+        if (lane == 0) s_want[it][wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int it = 0; it < n_iter; it++)
+            for (int w = 0; w < kWaves; w++) {
+                const int c = s_want[it][w];
+                s_want[it][w] = acc;
+                acc += c;
+            }
+        unsigned long long base = 0;
+        int emit = 0, qb = 0;
+        if (acc > 0) {
+            base = atomicAdd(&ctl->work_next, (unsigned long long)acc);
+            if (base < (unsigned long long)total)
+                emit = int(min((unsigned long long)acc, (unsigned long long)total - base));
+            if (emit > 0) qb = atomicAdd(q.counts + in_q, emit);
+        }
+        s_base = base;
+        s_emit = emit;
+        s_qbase = qb;
+    }
+    __syncthreads();
+    const unsigned long long base = s_base;
+    const int emit_n = s_emit, qbase = s_qbase;
+    // sweep 2: finish, then hand out
+    for (int it = 0; it < n_iter; it++) {
+        const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        bool want = false;
         if (s < cap) {
             int32_t st = P.state[s];
             if (st == kFinishing) {
-                fin = true;
-                fj = P.job[s];
+                const int fj = P.job[s];
                 const JobDev& J = jobs[fj];
                 const float4 L = P.L[s];
                 const int64_t f = J.film_off + 3 * int64_t(P.pixel[s]);
@@ -117,44 +183,27 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
                     atomicAdd(film_sq + f + 1, double(L.y) * double(L.y));
                     atomicAdd(film_sq + f + 2, double(L.z) * double(L.z));
                 }
+                // count the finished sample in the block's job table, or
+                // straight into the job's counter when its entry is taken
+                atomicAdd(&s_fin, 1u);
+                const int h = fj & (kRegenJobSlots - 1);
+                const int prev = atomicCAS(&s_key[h], -1, fj);
+                if (prev == -1 || prev == fj) {
+                    atomicAdd(&s_cnt[h], 1u);
+                } else {
+                    const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
+                    if (atomicAdd(job_done + fj, 1ull) + 1ull == need) job_time[fj] = wall_clock64();
+                }
                 st = kFree;
                 P.state[s] = kFree;
             }
             want = st == kFree;
         }
-        // completion counters, one atomic per wave (remaining) and per job
-        // present in the wave (job_done); the job whose count reaches its
-        // sample total records the device clock
-        const unsigned long long fmask = __ballot(fin);
-        if (fmask) {
-            if (lane == __ffsll(fmask) - 1)
-                atomicAdd(&ctl->remaining, (unsigned long long)(-(long long)__popcll(fmask)));
-            unsigned long long todo = fmask;
-            while (todo) {
-                const int l0 = __ffsll(todo) - 1;
-                const int jj = __shfl(fj, l0);
-                const unsigned long long same = __ballot(fin && fj == jj);
-                if (lane == l0) {
-                    const JobDev& J = jobs[jj];
-                    const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
-                    const unsigned long long k = (unsigned long long)__popcll(same);
-                    if (atomicAdd(job_done + jj, k) + k == need) job_time[jj] = wall_clock64();
-                }
-                todo &= ~same;
-            }
-        }
         const unsigned long long mask = __ballot(want);
-        unsigned long long base = 0;
-        if (mask) {
-            const int leader = __ffsll(mask) - 1;
-            if (lane == leader) base = atomicAdd(&ctl->work_next, (unsigned long long)__popcll(mask));
-            base = (unsigned long long)__shfl((long long)base, leader);
-        }
-        bool emit = false;
-        lmgpu_ray ray;
         if (want) {
-            const unsigned long long w = base + __popcll(mask & ((1ull << lane) - 1ull));
-            if (w < (unsigned long long)total) {
+            const int r = s_want[it][wave] + __popcll(mask & ((1ull << lane) - 1ull));   // rank in the block
+            if (r < emit_n) {
+                const unsigned long long w = base + (unsigned long long)r;
                 // segment of item w: the last one starting at or before w
                 int lo = 0, hi = nsegs - 1;
                 while (lo < hi) {
@@ -173,14 +222,22 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
                 P.L[s] = make_float4(0.f, 0.f, 0.f, 0.f);
                 P.prev[s] = make_float4(0.f, 0.f, 0.f, 0.f);
                 P.info[s] = make_int4(1, 0, 2, 0);   // camera vertex: degenerate, not NEE-samplable
-                ray = camera_ray(J.sc, pix, uint32_t(sample), job_seed(J));
-                emit = true;
+                q.ext[in_q][qbase + r] = camera_ray(J.sc, pix, uint32_t(sample), job_seed(J));
+                q.ext_path[in_q][qbase + r] = s;
             }
         }
-        const int k = wave_append(q.counts + in_q, emit);
-        if (emit) {
-            q.ext[in_q][k] = ray;
-            q.ext_path[in_q][k] = s;
+    }
+    // completion counters: once per block for the run, once per job in the table
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fin > 0u)
+        atomicAdd(&ctl->remaining, (unsigned long long)(-(long long)s_fin));
+    for (int k = threadIdx.x; k < kRegenJobSlots; k += BLOCK) {
+        const int j = s_key[k];
+        if (j >= 0 && s_cnt[k] > 0u) {
+            const JobDev& J = jobs[j];
+            const unsigned long long need = (unsigned long long)J.npix * (unsigned long long)J.spp;
+            const unsigned long long c = s_cnt[k];
+            if (atomicAdd(job_done + j, c) + c == need) job_time[j] = wall_clock64();
         }
     }
 }
@@ -614,7 +671,9 @@ int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int bloc
     LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     for (int par = 0; par < 2; par++) {
         const int in_q = par;
-        const unsigned rg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)));
+        // at most kRegenMaxIter strides of the pool per block
+        const unsigned rg = unsigned(std::max<int64_t>(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)),
+                                                       nblk(cap, block * kRegenMaxIter)));
         if (block == 128)
             hipLaunchKernelGGL(k_b_regen<128>, dim3(rg), dim3(128), 0, cs, djobs.p, dsegs.p, nsegs, total, P, q, in_q, cap,
                                ctl.p, job_done.p, job_time.p, film_sum, fsq);
