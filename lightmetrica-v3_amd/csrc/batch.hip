@@ -36,6 +36,13 @@
 #include "pt_device.h"
 #include "pt_internal.h"
 
+// Rays a wave takes per atomic on the traversal's ray counter in the
+// regenerating loop, whose queues stay long (the one-sample passes of the
+// path tracer use LMGPU_RAY_CHUNK: their late bounces queue few rays).
+#ifndef LMGPU_REGEN_RAY_CHUNK
+#define LMGPU_REGEN_RAY_CHUNK 128
+#endif
+
 namespace lmgpu {
 
 struct JobDev {
@@ -242,6 +249,41 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
     }
 }
 
+// Appends to two queues for the whole block: one atomic per block and queue
+// on the queue counts (which every block of the launch shares) instead of
+// one per wave. i0 / i1 are the entries of the lanes whose flag is set.
+// Every thread of the block calls it, the same number of times.
+template <int BLOCK>
+__device__ __forceinline__ void block_append2(int32_t* c0, int32_t* c1, bool f0, bool f1, int& i0, int& i1) {
+    constexpr int kW = BLOCK / 64;
+    __shared__ int cnt[2][kW];
+    __shared__ int base[2];
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+    if (lane == 0) {
+        cnt[0][wave] = __popcll(m0);
+        cnt[1][wave] = __popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int w = 0; w < kW; w++) {
+            const int x = cnt[0][w], y = cnt[1][w];
+            cnt[0][w] = a;
+            cnt[1][w] = b;
+            a += x;
+            b += y;
+        }
+        base[0] = a ? atomicAdd(c0, a) : 0;
+        base[1] = b ? atomicAdd(c1, b) : 0;
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    i0 = f0 ? base[0] + cnt[0][wave] + __popcll(m0 & lt) : -1;
+    i1 = f1 ? base[1] + cnt[1][wave] + __popcll(m1 & lt) : -1;
+    __syncthreads();   // the next call rewrites cnt and base
+}
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>::type* __restrict__ nodes,
                                                      const TriV* __restrict__ tris,
@@ -249,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>:
                                                      unsigned long long* next, int refill, BatchCtl* ctl) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[in_q];
+    if (n == 0) return;   // an idle iteration after the last work item (whole grid)
     const lmgpu_ray* rays = q.ext[in_q];
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(rays + id);
@@ -263,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>:
         reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
     };
     trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes, tris, n, next, lds, fetch, emit, q.counts + 5,
-                                                           nullptr, refill);
+                                                           nullptr, refill, LMGPU_REGEN_RAY_CHUNK);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->ext_rays, (unsigned long long)n);
 }
 
@@ -306,12 +349,12 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
             }
             P.L[s] = make_float4(R.L.x, R.L.y, R.L.z, 0.f);
         }
-        const int se = wave_append(q.counts + out_q, out.ext);
+        int se, ss;
+        block_append2<BLOCK>(q.counts + out_q, q.counts + 2, out.ext, out.sh, se, ss);
         if (out.ext) {
             q.ext[out_q][se] = out.ext_ray;
             q.ext_path[out_q][se] = s;
         }
-        const int ss = wave_append(q.counts + 2, out.sh);
         if (out.sh) {
             q.sh[ss] = out.sh_ray;
             q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(s));
@@ -326,6 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_b_connect(const typename TravNode<W>
                                                       unsigned long long* next, int refill, BatchCtl* ctl) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
+    if (n == 0) return;   // no shadow rays this iteration (whole grid)
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
         const float4 a = rp[0], b = rp[1];
@@ -347,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_b_connect(const typename TravNode<W>
         }
     };
     trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes, tris, n, next, lds, fetch, emit, q.counts + 5, nullptr,
-                                                          refill);
+                                                          refill, LMGPU_REGEN_RAY_CHUNK);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
 }
 

@@ -462,7 +462,7 @@ __device__ __forceinline__ void sort_children(float (&k)[W], int (&e)[W]) {
 // Persistent traversal: every lane owns one ray at a time; when at least
 // `refill` lanes of a wave are idle (or the whole wave is), the idle lanes
 // take new rays from the wave's pool, which the wave refills by chunks of
-// LMGPU_RAY_CHUNK from one global counter with one atomic (Aila & Laine,
+// chunk_rays (default LMGPU_RAY_CHUNK) from one global counter with one atomic (Aila & Laine,
 // HPG 2009, "persistent threads"), so lanes do not sit idle behind a wave's
 // slowest ray and the counter sees few atomics. refill = 64 keeps whole
 // waves of consecutive rays together (best for coherent rays), smaller
@@ -478,7 +478,8 @@ template <int W, bool ANY, bool COUNT, int LDS_DEPTH, int PRIV_DEPTH, int BLOCK,
 __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes, const TriV* __restrict__ tris,
                                                  int64_t n, unsigned long long* next, int (*lds)[BLOCK],
                                                  Fetch fetch, Emit emit, int32_t* status,
-                                                 unsigned long long* counters, int refill) {
+                                                 unsigned long long* counters, int refill,
+                                                 int chunk_rays = LMGPU_RAY_CHUNK) {
     const int lane = threadIdx.x & 63;
     int priv[PRIV_DEPTH];
     Stack<LDS_DEPTH, PRIV_DEPTH, BLOCK> st;
@@ -508,7 +509,7 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
             if (avail < nidle) {
                 const int leader = __ffsll(idle_mask) - 1;
                 unsigned long long b = 0;
-                if (lane == leader) b = atomicAdd(next, (unsigned long long)LMGPU_RAY_CHUNK);
+                if (lane == leader) b = atomicAdd(next, (unsigned long long)chunk_rays);
                 b = (unsigned long long)__shfl((long long)b, leader);
                 // wave-uniform from here on: scalar registers for the pool
                 chunk = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(b >> 32)))) << 32) |
@@ -535,7 +536,7 @@ __device__ __forceinline__ void trace_persistent(const Node* __restrict__ nodes,
             }
             if (avail < nidle) {
                 pool_next = chunk + (nidle - avail);
-                pool_end = chunk + LMGPU_RAY_CHUNK;
+                pool_end = chunk + chunk_rays;
             } else {
                 pool_next += nidle;
             }
