@@ -249,41 +249,6 @@ __global__ __launch_bounds__(BLOCK) void k_b_regen(const JobDev* __restrict__ jo
     }
 }
 
-// Appends to two queues for the whole block: one atomic per block and queue
-// on the queue counts (which every block of the launch shares) instead of
-// one per wave. i0 / i1 are the entries of the lanes whose flag is set.
-// Every thread of the block calls it, the same number of times.
-template <int BLOCK>
-__device__ __forceinline__ void block_append2(int32_t* c0, int32_t* c1, bool f0, bool f1, int& i0, int& i1) {
-    constexpr int kW = BLOCK / 64;
-    __shared__ int cnt[2][kW];
-    __shared__ int base[2];
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
-    if (lane == 0) {
-        cnt[0][wave] = __popcll(m0);
-        cnt[1][wave] = __popcll(m1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int a = 0, b = 0;
-        for (int w = 0; w < kW; w++) {
-            const int x = cnt[0][w], y = cnt[1][w];
-            cnt[0][w] = a;
-            cnt[1][w] = b;
-            a += x;
-            b += y;
-        }
-        base[0] = a ? atomicAdd(c0, a) : 0;
-        base[1] = b ? atomicAdd(c1, b) : 0;
-    }
-    __syncthreads();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    i0 = f0 ? base[0] + cnt[0][wave] + __popcll(m0 & lt) : -1;
-    i1 = f1 ? base[1] + cnt[1][wave] + __popcll(m1 & lt) : -1;
-    __syncthreads();   // the next call rewrites cnt and base
-}
-
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>::type* __restrict__ nodes,
                                                      const TriV* __restrict__ tris,

@@ -114,8 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
     const int out_q = in_q ^ 1;
     const uint64_t seed = lt_seed(c);
     const int stride = gridDim.x * kBlock;
-    // every lane of a wave runs the same number of iterations so the
-    // wave-level appends see the whole wave
+    // every thread of the grid runs the same number of iterations: the
+    // appends are block-wide
     const int n_iter = (n + stride - 1) / stride;
     for (int it = 0; it < n_iter; it++) {
         const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
@@ -211,12 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                 }
             }
         }
-        const int se = wave_append(q.counts + out_q, emit_ext);
+        int se, ss;
+        block_append2<kBlock>(q.counts + out_q, q.counts + 2, emit_ext, emit_sh, se, ss);
         if (emit_ext) {
             q.ext[out_q][se] = ext_ray;
             q.ext_path[out_q][se] = p;
         }
-        const int ss = wave_append(q.counts + 2, emit_sh);
         if (emit_sh) {
             q.sh[ss] = sh_ray;
             q.sh_contrib[ss] = sh_c;
@@ -582,12 +582,12 @@ __global__ __launch_bounds__(kBlock) void k_bd_lshade(SceneDev sc, PathState ps,
                 }
             }
         }
-        const int se = wave_append(q.counts + out_q, emit_ext);
+        int se, ss;
+        block_append2<kBlock>(q.counts + out_q, q.counts + 2, emit_ext, emit_sh, se, ss);
         if (emit_ext) {
             q.ext[out_q][se] = ext_ray;
             q.ext_path[out_q][se] = p;
         }
-        const int ss = wave_append(q.counts + 2, emit_sh);
         if (emit_sh) {
             q.sh[ss] = sh_ray;
             q.sh_contrib[ss] = sh_c;
@@ -743,13 +743,13 @@ __global__ __launch_bounds__(kBlock) void k_bd_eshade(SceneDev sc, PathState ps,
                 }
             }
         }
-        const int se = wave_append(q.counts + out_q, emit_ext);
+        const int se = block_append1<kBlock>(q.counts + out_q, emit_ext);
         if (emit_ext) {
             q.ext[out_q][se] = ext_ray;
             q.ext_path[out_q][se] = p;
         }
-        // connections to light vertices y_j, s = j + 1 <= max_verts - t (every lane runs
-        // the same loop so the wave-level appends see the whole wave)
+        // connections to light vertices y_j, s = j + 1 <= max_verts - t (every thread
+        // runs the same loop: the appends are block-wide)
         const bool can = active && !spec;
         for (int j = 0; j < c.lstride; j++) {
             bool emit_sh = false;
@@ -812,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_bd_eshade(SceneDev sc, PathState ps,
                     }
                 }
             }
-            const int ss = wave_append(q.counts + 2, emit_sh);
+            const int ss = block_append1<kBlock>(q.counts + 2, emit_sh);
             if (emit_sh) {
                 q.sh[ss] = sh_ray;
                 q.sh_contrib[ss] = sh_c;

@@ -217,8 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
     const uint32_t sample = uint32_t(q.counts[3]);
     const uint64_t seed = (uint64_t(c.seed_hi) << 32) | c.seed_lo;
     const int stride = gridDim.x * kBlock;
-    // every lane of a wave runs the same number of iterations so the
-    // wave-level appends see the whole wave
+    // every thread of the grid runs the same number of iterations: the
+    // appends are block-wide (wave-wide for the guide's records)
     const int n_iter = (n + stride - 1) / stride;
     for (int it = 0; it < n_iter; it++) {
         const int i = it * stride + blockIdx.x * kBlock + threadIdx.x;
@@ -241,19 +241,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
             }
             ps.L[p] = make_float4(P.L.x, P.L.y, P.L.z, 0.f);
         }
-        const int se = wave_append(q.counts + out_q, out.ext);
+        int se, ss;
+        block_append2<kBlock>(q.counts + out_q, q.counts + 2, out.ext, out.sh, se, ss);
         if (out.ext) {
             q.ext[out_q][se] = out.ext_ray;
             q.ext_path[out_q][se] = p;
         }
-        const int ss = wave_append(q.counts + 2, out.sh);
         if (out.sh) {
             q.sh[ss] = out.sh_ray;
             q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(p));
             if (GUIDED) gd.sh_rec[ss] = out.sh_slot;
         }
         if (GUIDED) {
-            const int rs = wave_append(gd.rec_count, out.rec_slot >= 0);
+            const int rs = block_append1<kBlock>(gd.rec_count, out.rec_slot >= 0);
             if (out.rec_slot >= 0) gd.rec_list[rs] = out.rec_slot;
         }
     }

@@ -153,6 +153,66 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     return want ? base + __popcll(lower) : -1;
 }
 
+// Appends to two queues for the whole block: one atomic per block and queue
+// on the queue counts (which every block of the launch shares) instead of
+// one per wave. i0 / i1 are the entries of the lanes whose flag is set.
+// Every thread of the block calls it, the same number of times.
+template <int BLOCK>
+__device__ __forceinline__ void block_append2(int32_t* c0, int32_t* c1, bool f0, bool f1, int& i0, int& i1) {
+    constexpr int kW = BLOCK / 64;
+    __shared__ int cnt[2][kW];
+    __shared__ int base[2];
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+    if (lane == 0) {
+        cnt[0][wave] = __popcll(m0);
+        cnt[1][wave] = __popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int w = 0; w < kW; w++) {
+            const int x = cnt[0][w], y = cnt[1][w];
+            cnt[0][w] = a;
+            cnt[1][w] = b;
+            a += x;
+            b += y;
+        }
+        base[0] = a ? atomicAdd(c0, a) : 0;
+        base[1] = b ? atomicAdd(c1, b) : 0;
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    i0 = f0 ? base[0] + cnt[0][wave] + __popcll(m0 & lt) : -1;
+    i1 = f1 ? base[1] + cnt[1][wave] + __popcll(m1 & lt) : -1;
+    __syncthreads();   // the next call rewrites cnt and base
+}
+
+// As block_append2, for one queue.
+template <int BLOCK>
+__device__ __forceinline__ int block_append1(int32_t* c0, bool f0) {
+    constexpr int kW = BLOCK / 64;
+    __shared__ int cnt[kW];
+    __shared__ int base;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long m0 = __ballot(f0);
+    if (lane == 0) cnt[wave] = __popcll(m0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0;
+        for (int w = 0; w < kW; w++) {
+            const int x = cnt[w];
+            cnt[w] = a;
+            a += x;
+        }
+        base = a ? atomicAdd(c0, a) : 0;
+    }
+    __syncthreads();
+    const int i0 = f0 ? base + cnt[wave] + __popcll(m0 & ((1ull << lane) - 1ull)) : -1;
+    __syncthreads();   // the next call rewrites cnt and base
+    return i0;
+}
+
 template <int W>
 __device__ __forceinline__ const typename TravNode<W>::type* nodes_of(const SceneDev& sc);
 template <>
