@@ -31,6 +31,12 @@
 #include "bvh.h"
 #include "lmgpu_internal.h"
 
+// Rays a wave of the ray-cast kernel takes per atomic on the ray counter
+// (128: coherent config-1 rays +10% over 64, incoherent level).
+#ifndef LMGPU_TRACE_RAY_CHUNK
+#define LMGPU_TRACE_RAY_CHUNK 128
+#endif
+
 namespace lmgpu {
 
 // Build defaults: leaves of at most 2 triangles, and a triangle test costing
@@ -571,7 +577,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Node* __restrict__ 
         }
     };
     trace_persistent<W, ANY, COUNT, kLdsDepth, kPrivDepth, kTraceBlock>(nodes, tris, n, next, lds, fetch, emit,
-                                                                         status, counters, refill);
+                                                                         status, counters, refill, LMGPU_TRACE_RAY_CHUNK);
 }
 
 // Quantised copy of a built W-wide node array (bvh.h, quantise_node).
