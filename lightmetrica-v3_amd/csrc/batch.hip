@@ -19,7 +19,7 @@
 //
 // BVH width, path-pool capacity (queue capacity) and the block size of the
 // regenerate / shade kernels are chosen by an autotuner when the caller
-// leaves them 0: a coordinate search over {2,4,8} x {2^19,2^20,2^21} x
+// leaves them 0: a coordinate search over {2,4,8} x {2^21,2^22,2^23} x
 // {128,256} timing one sample per pixel of every job.
 
 #include <hip/hip_runtime.h>
@@ -39,6 +39,15 @@
 // Rays a wave takes per atomic on the traversal's ray counter in the
 // regenerating loop, whose queues stay long (the one-sample passes of the
 // path tracer use LMGPU_RAY_CHUNK: their late bounces queue few rays).
+// Path pool of a fixed-spp render: slots per pixel and at most (1080p
+// config 2: 2 per pixel 431, 4: 503, 8: 547, 16: 558 M samples/s; 2^24 slots
+// take 3.6 GB).
+#ifndef LMGPU_REGEN_SLOTS_PER_PIXEL
+#define LMGPU_REGEN_SLOTS_PER_PIXEL 8
+#endif
+#ifndef LMGPU_REGEN_MAX_SLOTS
+#define LMGPU_REGEN_MAX_SLOTS (1 << 24)
+#endif
 #ifndef LMGPU_REGEN_RAY_CHUNK
 #define LMGPU_REGEN_RAY_CHUNK 128
 #endif
@@ -888,11 +897,12 @@ int Scene::render_regen(const PTConsts& c, int spp, int sample_offset, double* s
     const int64_t total = int64_t(c.npix) * spp;
     std::vector<WorkSeg> segs(static_cast<size_t>(spp));
     for (int k = 0; k < spp; k++) segs[size_t(k)] = WorkSeg{int64_t(k) * c.npix, 0, sample_offset + k};
-    // pool: two paths per pixel (the slots of a sample's short paths take the
-    // next sample's pixels), at most 2^22 slots, at least 4096
-    int64_t want = std::min<int64_t>(total, 2 * int64_t(c.npix));
+    // pool: LMGPU_REGEN_SLOTS_PER_PIXEL paths per pixel (the slots of a
+    // sample's short paths take the next sample's pixels), at most
+    // LMGPU_REGEN_MAX_SLOTS, at least 4096
+    int64_t want = std::min<int64_t>(total, int64_t(LMGPU_REGEN_SLOTS_PER_PIXEL) * c.npix);
     int cap = 4096;
-    while (cap < want && cap < (1 << 22)) cap <<= 1;
+    while (cap < want && cap < LMGPU_REGEN_MAX_SLOTS) cap <<= 1;
     LMGPU_TRY(wf->ensure_pool(cap));
     LMGPU_TRY(wf->set_work(std::vector<JobDev>{J}, segs, s));
     const void* tn = accel.width == 4 ? static_cast<const void*>(dev.nodes4)
@@ -1027,8 +1037,8 @@ int lmgpu_batch_run(lmgpu_batch* bh, const lmgpu_batch_opts* opts, lmgpu_batch_s
                 int64_t it;
                 return b.run_once(w, bl, 1, false, o.film_sumsq != 0, o.queue_order, sec, &it, s);
             };
-            const int widths[3] = {4, 8, 2}, caps[3] = {1 << 20, 1 << 19, 1 << 21}, blocks[2] = {256, 128};
-            int bw = width ? width : 4, bc = capv ? capv : (1 << 20), bb = block ? block : 256;
+            const int widths[3] = {4, 8, 2}, caps[3] = {1 << 22, 1 << 21, 1 << 23}, blocks[2] = {256, 128};
+            int bw = width ? width : 4, bc = capv ? capv : (1 << 22), bb = block ? block : 256;
             double best = 1e30, sec;
             if (!width)
                 for (int w : widths) {
