@@ -36,63 +36,9 @@
 #include "pt_device.h"
 #include "pt_internal.h"
 
-// Rays a wave takes per atomic on the traversal's ray counter in the
-// regenerating loop, whose queues stay long (the one-sample passes of the
-// path tracer use LMGPU_RAY_CHUNK: their late bounces queue few rays).
-// Path pool of a fixed-spp render: slots per pixel and at most (1080p
-// config 2: 2 per pixel 431, 4: 503, 8: 547, 16: 558 M samples/s; 2^24 slots
-// take 3.6 GB).
-#ifndef LMGPU_REGEN_SLOTS_PER_PIXEL
-#define LMGPU_REGEN_SLOTS_PER_PIXEL 8
-#endif
-#ifndef LMGPU_REGEN_MAX_SLOTS
-#define LMGPU_REGEN_MAX_SLOTS (1 << 24)
-#endif
-#ifndef LMGPU_REGEN_RAY_CHUNK
-#define LMGPU_REGEN_RAY_CHUNK 128
-#endif
+#include "wavefront.h"
 
 namespace lmgpu {
-
-struct JobDev {
-    SceneDev sc;          // job-local pointers into the shading arenas
-    int root;             // root of the job's BVH in the node arena
-    int npix, spp, max_verts, mode, rr_depth;
-    uint32_t seed_lo, seed_hi;
-    int64_t film_off;     // first double of the job's film (3 per pixel)
-};
-
-// A run of consecutive work items: the pixels 0.. of one sample of one job,
-// starting at queue position `start`.
-struct WorkSeg {
-    int64_t start;
-    int32_t job, sample;
-};
-
-constexpr int32_t kFree = 0, kActive = 1, kFinishing = 2;
-
-struct PoolDev {
-    int32_t *job, *pixel, *sample, *state;
-    float4 *thr, *L, *prev;
-    int4* info;
-};
-
-struct BatchQ {
-    lmgpu_ray* ext[2];
-    int32_t* ext_path[2];
-    lmgpu_hit* hit;
-    lmgpu_ray* sh;
-    float4* sh_contrib;   // xyz contribution, w = path slot bits
-    int32_t* counts;      // [0],[1] extension queues, [2] shadow queue, [5] stack overflow flag
-};
-
-struct BatchCtl {
-    unsigned long long work_next;   // next work item to hand out
-    unsigned long long remaining;   // work items not yet accumulated
-    unsigned long long ext_rays, shadow_rays;
-    unsigned long long t_start;     // device clock at the start of the run
-    unsigned long long iterations;  // iterations that found work left
-};
 
 namespace {
 
@@ -450,37 +396,6 @@ struct BatchJob {
     uint64_t seed = 0;
 };
 
-// Path pool, work queue and the loop that drains it (regenerate -> extend ->
-// shade -> connect, two iterations per graph replay), shared by the batch
-// and by lmgpu_pt_render's regenerating path.
-struct Wavefront {
-    int cap = 0;
-    DevBuf<int32_t> pjob, ppix, psample, pstate;
-    DevBuf<float4> pthr, pL, pprev;
-    DevBuf<int4> pinfo;
-    DevBuf<lmgpu_ray> ext0, ext1, sh;
-    DevBuf<int32_t> path0, path1, counts;
-    DevBuf<lmgpu_hit> hit;
-    DevBuf<float4> shc;
-    DevBuf<BatchCtl> ctl;
-    BatchCtl* ctl_mirror = nullptr;   // pinned host copies of ctl, one per chunk parity
-    DevBuf<unsigned long long> next;
-    // per-run tables: jobs, work segments, per-job completion counters and clocks
-    DevBuf<JobDev> djobs;
-    DevBuf<WorkSeg> dsegs;
-    DevBuf<unsigned long long> job_done, job_time;
-    ~Wavefront() {
-        if (ctl_mirror) (void)hipHostFree(ctl_mirror);
-    }
-    int ensure_pool(int cap);
-    // Uploads the jobs and the work segments (queue order) of one run.
-    int set_work(const std::vector<JobDev>& jobs, const std::vector<WorkSeg>& segs, hipStream_t s);
-    // Renders every work item of the tables set last into film_sum / film_sq
-    // (FP64 atomics; film_sq may be null) and waits for the end.
-    int run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
-            double* film_sum, double* film_sq, double* elapsed, int64_t* iters, hipStream_t s);
-};
-
 struct Batch {
     std::vector<BatchJob> jobs;
     // arena (valid for arena_width / arena_leaf)
@@ -676,19 +591,10 @@ int Wavefront::set_work(const std::vector<JobDev>& jobs, const std::vector<WorkS
 int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
                    double* film_sum, double* fsq, double* elapsed, int64_t* iters, hipStream_t s) {
     const int nsegs = int(dsegs.n);
-    hipLaunchKernelGGL(k_b_fill_i32, dim3(nblk(cap, 256)), dim3(256), 0, s, pstate.p, int64_t(cap), kFree);
-    hipLaunchKernelGGL(k_b_init, dim3(1), dim3(64), 0, s, ctl.p, total, counts.p);
-
     PoolDev P{pjob.p, ppix.p, psample.p, pstate.p, pthr.p, pL.p, pprev.p, pinfo.p};
     BatchQ q{{ext0.p, ext1.p}, {path0.p, path1.p}, hit.p, sh.p, shc.p, counts.p};
     const int refill = 32;
-    // two iterations (queue parities 0 and 1) captured as one graph
-    hipStream_t cs;
-    LMGPU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    hipGraph_t g;
-    LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int par = 0; par < 2; par++) {
-        const int in_q = par;
+    auto iteration = [&](hipStream_t cs, int in_q) {
         // at most kRegenMaxIter strides of the pool per block
         const unsigned rg = unsigned(std::max<int64_t>(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)),
                                                        nblk(cap, block * kRegenMaxIter)));
@@ -721,7 +627,21 @@ int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int bloc
         else
             hipLaunchKernelGGL(k_b_connect<8>, dim3(occupancy_grid(k_b_connect<8>, kBlock)), dim3(kBlock), 0, cs,
                                static_cast<const TravNode<8>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
-    }
+    };
+    return loop(iteration, total, max_verts, elapsed, iters, s);
+}
+
+int Wavefront::loop(const std::function<void(hipStream_t, int)>& iteration, int64_t total, int max_verts,
+                    double* elapsed, int64_t* iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_fill_i32, dim3(nblk(cap, 256)), dim3(256), 0, s, pstate.p, int64_t(cap), kFree);
+    hipLaunchKernelGGL(k_b_init, dim3(1), dim3(64), 0, s, ctl.p, total, counts.p);
+    // two iterations (queue parities 0 and 1) captured as one graph
+    hipStream_t cs;
+    LMGPU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    LMGPU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    iteration(cs, 0);
+    iteration(cs, 1);
     hipError_t e = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(cs);
@@ -898,11 +818,8 @@ int Scene::render_regen(const PTConsts& c, int spp, int sample_offset, double* s
     std::vector<WorkSeg> segs(static_cast<size_t>(spp));
     for (int k = 0; k < spp; k++) segs[size_t(k)] = WorkSeg{int64_t(k) * c.npix, 0, sample_offset + k};
     // pool: LMGPU_REGEN_SLOTS_PER_PIXEL paths per pixel (the slots of a
-    // sample's short paths take the next sample's pixels), at most
-    // LMGPU_REGEN_MAX_SLOTS, at least 4096
-    int64_t want = std::min<int64_t>(total, int64_t(LMGPU_REGEN_SLOTS_PER_PIXEL) * c.npix);
-    int cap = 4096;
-    while (cap < want && cap < LMGPU_REGEN_MAX_SLOTS) cap <<= 1;
+    // sample's short paths take the next sample's pixels)
+    const int cap = Wavefront::pool_size(total, c.npix);
     LMGPU_TRY(wf->ensure_pool(cap));
     LMGPU_TRY(wf->set_work(std::vector<JobDev>{J}, segs, s));
     const void* tn = accel.width == 4 ? static_cast<const void*>(dev.nodes4)

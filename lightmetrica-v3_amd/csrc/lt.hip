@@ -33,6 +33,7 @@
 #include "lmgpu_internal.h"
 #include "pt_device.h"
 #include "pt_internal.h"
+#include "wavefront.h"
 
 namespace lmgpu {
 
@@ -59,17 +60,16 @@ __device__ __forceinline__ float sn_correction_le(const Geom& g, f3 wi, f3 wo) {
     return (wi_ns * wo_ng) / (wo_ns * wi_ng);
 }
 
-// Light subpath starts. Path state: thr = throughput, info.x = vertex count
-// of the vertex the ray will hit. Random numbers: (path, pass, 0, 0) for the
-// light selection and the position, (path, pass, 0, 1) for the direction.
-__global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState ps, Queues q, LTConsts c) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= c.npaths) return;
+// Start of light path `i` of pass `pass` (renderer_lt.cpp:78-90): its first
+// ray and weight. Random numbers: (i, pass, 0, 0) for the light selection and
+// the position, (i, pass, 0, 1) for the direction.
+__device__ __forceinline__ void lt_start(const SceneDev& sc, const LTConsts& c, uint32_t i, uint32_t pass,
+                                         lmgpu_ray& ray, f3& w) {
     const uint64_t seed = lt_seed(c);
-    const rand4 r0 = rng(uint32_t(i), c.pass, 0u, 0u, seed);
-    const rand4 r1 = rng(uint32_t(i), c.pass, 0u, 1u, seed);
+    const rand4 r0 = rng(i, pass, 0u, 0u, seed);
+    const rand4 r1 = rng(i, pass, 0u, 1u, seed);
     const int li = min(max(int(r0.u[0] * sc.num_lights), 0), sc.num_lights - 1);
-    f3 org, d, w;
+    f3 org, d;
     if (li < sc.num_area_lights) {
         const DevLight dl = sc.lights[li];
         f3 nL;
@@ -91,13 +91,22 @@ __global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState p
         // Le / (p_dir p_pos) / p_sel = Le 4 pi (pi r^2) n_lights
         w = sc.env_Le * (4.f * kPi * kPi * c.sph_r * c.sph_r * float(sc.num_lights));
     }
-    ps.thr[i] = make_float4(w.x, w.y, w.z, 0.f);
-    ps.info[i] = make_int4(2, 0, 0, 0);
-    lmgpu_ray ray;
     ray.org[0] = org.x; ray.org[1] = org.y; ray.org[2] = org.z;
     ray.tmin = kEps;
     ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
     ray.tmax = kInf;
+}
+
+// Light subpath starts of one pass. Path state: thr = throughput, info.x =
+// vertex count of the vertex the ray will hit.
+__global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState ps, Queues q, LTConsts c) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.npaths) return;
+    lmgpu_ray ray;
+    f3 w;
+    lt_start(sc, c, uint32_t(i), c.pass, ray, w);
+    ps.thr[i] = make_float4(w.x, w.y, w.z, 0.f);
+    ps.info[i] = make_int4(2, 0, 0, 0);
     q.ext[0][i] = ray;
     q.ext_path[0][i] = i;
     if (i == 0) {
@@ -106,10 +115,111 @@ __global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState p
     }
 }
 
+// Regenerating light tracing (fixed num_samples): frees the slots whose path
+// ended (state kFinishing from k_lt_shade) and gives the free slots the next
+// light paths, item j being path j mod npix of pass pass0 + j / npix (the
+// passes' numbering), their first rays appended to queue in_q. Counters are
+// updated once per block (two sweeps over the block's slots, as k_b_regen).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_lt_regen(SceneDev sc, LTConsts c, int64_t total, int npix, uint32_t pass0,
+                                                    int32_t* __restrict__ ppid, int32_t* __restrict__ ppass,
+                                                    int32_t* __restrict__ pstate, float4* __restrict__ pthr,
+                                                    int4* __restrict__ pinfo, Queues q, int in_q, int cap,
+                                                    BatchCtl* ctl) {
+    constexpr int kWaves = BLOCK / 64;
+    constexpr int kMaxIter = 32;
+    __shared__ int s_want[kMaxIter][kWaves];
+    __shared__ unsigned long long s_base;
+    __shared__ int s_qbase, s_emit, s_idle;
+    __shared__ unsigned int s_fin;
+    if (threadIdx.x == 0)
+        s_idle = __hip_atomic_load(&ctl->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __syncthreads();
+    if (s_idle) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->iterations, 1ull);
+    const int stride = gridDim.x * BLOCK;
+    const int n_iter = (cap + stride - 1) / stride;   // <= kMaxIter (host)
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_fin = 0u;
+    for (int it = 0; it < n_iter; it++) {
+        const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        bool want = false;
+        if (s < cap) {
+            const int32_t st = pstate[s];
+            want = st == kFinishing || st == kFree;
+        }
+        const unsigned long long m = __ballot(want);
+        if (lane == 0) s_want[it][wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int it = 0; it < n_iter; it++)
+            for (int w = 0; w < kWaves; w++) {
+                const int k = s_want[it][w];
+                s_want[it][w] = acc;
+                acc += k;
+            }
+        unsigned long long base = 0;
+        int emit = 0, qb = 0;
+        if (acc > 0) {
+            base = atomicAdd(&ctl->work_next, (unsigned long long)acc);
+            if (base < (unsigned long long)total) emit = int(min((unsigned long long)acc, (unsigned long long)total - base));
+            if (emit > 0) qb = atomicAdd(q.counts + in_q, emit);
+        }
+        s_base = base;
+        s_emit = emit;
+        s_qbase = qb;
+    }
+    __syncthreads();
+    const unsigned long long base = s_base;
+    const int emit_n = s_emit, qbase = s_qbase;
+    for (int it = 0; it < n_iter; it++) {
+        const int s = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        bool want = false;
+        if (s < cap) {
+            int32_t st = pstate[s];
+            if (st == kFinishing) {
+                atomicAdd(&s_fin, 1u);
+                st = kFree;
+                pstate[s] = kFree;
+            }
+            want = st == kFree;
+        }
+        const unsigned long long mask = __ballot(want);
+        if (want) {
+            const int r = s_want[it][wave] + __popcll(mask & ((1ull << lane) - 1ull));
+            if (r < emit_n) {
+                const unsigned long long j = base + (unsigned long long)r;
+                const uint32_t i = uint32_t(j % (unsigned long long)npix);
+                const uint32_t pass = pass0 + uint32_t(j / (unsigned long long)npix);
+                lmgpu_ray ray;
+                f3 w;
+                lt_start(sc, c, i, pass, ray, w);
+                pthr[s] = make_float4(w.x, w.y, w.z, 0.f);
+                pinfo[s] = make_int4(2, 0, 0, 0);
+                ppid[s] = int32_t(i);
+                ppass[s] = int32_t(pass);
+                pstate[s] = kActive;
+                q.ext[in_q][qbase + r] = ray;
+                q.ext_path[in_q][qbase + r] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fin > 0u) atomicAdd(&ctl->remaining, (unsigned long long)(-(long long)s_fin));
+}
+
 // One light-path vertex per extension-queue entry (the body of the loop of
 // renderer_lt.cpp:92-162 for the vertex the entry's ray hit). Random numbers
 // (path, pass, vertex, 0) = {RR, component, direction u0, u1}.
-__global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, Queues q, LTConsts c, int in_q) {
+// REGEN: the queue entries are slots of the regenerating loop, whose light
+// path and pass are ppid / ppass; a path that ends marks its slot kFinishing.
+template <bool REGEN>
+__global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, Queues q, LTConsts c, int in_q,
+                                                    const int32_t* __restrict__ ppid,
+                                                    const int32_t* __restrict__ ppass, int32_t* __restrict__ pstate) {
     const int n = q.counts[in_q];
     const int out_q = in_q ^ 1;
     const uint64_t seed = lt_seed(c);
@@ -133,7 +243,9 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                 const float4 thr4 = ps.thr[p];
                 f3 thr = mk(thr4.x, thr4.y, thr4.z);
                 const int nv = ps.info[p].x;
-                const rand4 r0 = rng(uint32_t(p), c.pass, uint32_t(nv), 0u, seed);
+                const uint32_t pid = REGEN ? uint32_t(ppid[p]) : uint32_t(p);
+                const uint32_t pass = REGEN ? uint32_t(ppass[p]) : c.pass;
+                const rand4 r0 = rng(pid, pass, uint32_t(nv), 0u, seed);
                 bool alive = true;
                 if (nv - 1 > 5) {
                     const float qq = fmaxf(.2f, 1.f - maxc(thr));
@@ -211,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                 }
             }
         }
+        if (REGEN && i < n && !emit_ext) pstate[p] = kFinishing;
         int se, ss;
         block_append2<kBlock>(q.counts + out_q, q.counts + 2, emit_ext, emit_sh, se, ss);
         if (emit_ext) {
@@ -226,9 +339,10 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_lt_connect(SceneDev sc, Queues q, unsigned long long* next, int refill,
-                                                       double* __restrict__ film) {
+                                                       double* __restrict__ film, int chunk_rays) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
+    if (n == 0) return;   // no connections (whole grid)
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
         const float4 a = rp[0], b = rp[1];
@@ -248,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_lt_connect(SceneDev sc, Queues q, un
         }
     };
     trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
-                                                          q.counts + 5, nullptr, refill);
+                                                          q.counts + 5, nullptr, refill, chunk_rays);
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 12), (unsigned long long)n);
 }
@@ -286,10 +400,36 @@ void lt_pass(Scene& S, const LTConsts& c, double* film, int refill, hipStream_t 
     for (int b = 0; b < c.max_verts - 1; b++) {
         const int in_q = b & 1;
         hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill);
-        hipLaunchKernelGGL(k_lt_shade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q);
-        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill, LMGPU_RAY_CHUNK);
+        hipLaunchKernelGGL(k_lt_shade<false>, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, nullptr, nullptr,
+                           nullptr);
+        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film,
+                           LMGPU_RAY_CHUNK);
     }
+}
+
+// One iteration of the regenerating light tracer on queue parity in_q.
+template <int W>
+void lt_regen_iteration(Scene& S, Wavefront& wf, const LTConsts& c, int64_t total, int npix, uint32_t pass0,
+                        double* film, int in_q, hipStream_t s) {
+    const int cap = wf.cap;
+    Queues q{{wf.ext0.p, wf.ext1.p}, wf.hit.p, {wf.path0.p, wf.path1.p}, wf.sh.p, wf.shc.p, wf.counts.p};
+    PathState ps{wf.pthr.p, wf.pL.p, wf.pprev.p, wf.pinfo.p};
+    constexpr int kRegenBlock = 128;
+    // at most 32 strides of the pool per regenerate block
+    const unsigned rg = unsigned(std::max<int64_t>(std::min<int64_t>((cap + kRegenBlock - 1) / kRegenBlock, 4096),
+                                                   (cap + kRegenBlock * 32 - 1) / (kRegenBlock * 32)));
+    const int sg = std::max(1, std::min((cap + kBlock - 1) / kBlock, 2048));
+    const int refill = 32;
+    hipLaunchKernelGGL(k_lt_regen<kRegenBlock>, dim3(rg), dim3(kRegenBlock), 0, s, S.dev, c, total, npix, pass0,
+                       wf.ppix.p, wf.psample.p, wf.pstate.p, wf.pthr.p, wf.pinfo.p, q, in_q, cap, wf.ctl.p);
+    hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, wf.next.p, wf.counts.p, in_q ^ 1);
+    hipLaunchKernelGGL(k_extend<W>, dim3(extend_grid<W>()), dim3(kBlock), 0, s, S.dev, q, in_q, wf.next.p, refill,
+                       LMGPU_REGEN_RAY_CHUNK);
+    hipLaunchKernelGGL(k_lt_shade<true>, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, wf.ppix.p, wf.psample.p,
+                       wf.pstate.p);
+    hipLaunchKernelGGL(k_lt_connect<W>, dim3(lt_connect_grid<W>()), dim3(kBlock), 0, s, S.dev, q, wf.next.p + 1, refill,
+                       film, LMGPU_REGEN_RAY_CHUNK);
 }
 
 // ---------------------------------------------------------------------------
@@ -831,21 +971,24 @@ void bd_pass(Scene& S, const BDConsts& c, double* film, int refill, LVert* lv, h
     const int cg = lt_connect_grid<W>();
     hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, 1);
     hipLaunchKernelGGL(k_bd_lgen, dim3(gb), dim3(kBlock), 0, s, S.dev, ps, q, c, lv, S.blcount.p);
-    hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+    hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film,
+                           LMGPU_RAY_CHUNK);
     for (int b = 0; b < c.max_verts - 2; b++) {   // y_1 .. y_{max_verts - 2}
         const int in_q = b & 1;
         hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill);
+        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill, LMGPU_RAY_CHUNK);
         hipLaunchKernelGGL(k_bd_lshade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, lv, S.blcount.p);
-        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film,
+                           LMGPU_RAY_CHUNK);
     }
     hipLaunchKernelGGL(k_bd_egen, dim3(gb), dim3(kBlock), 0, s, S.dev, ps, q, c);
     for (int b = 0; b < c.max_verts - 1; b++) {   // z_1 .. z_{max_verts - 1}
         const int in_q = b & 1;
         hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill);
+        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill, LMGPU_RAY_CHUNK);
         hipLaunchKernelGGL(k_bd_eshade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, lv, S.blcount.p, film);
-        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film);
+        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film,
+                           LMGPU_RAY_CHUNK);
     }
 }
 
@@ -878,6 +1021,42 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
     const f3 hi = mk(accel.root_hi[0], accel.root_hi[1], accel.root_hi[2]);
     c.sph_c = (lo + hi) * .5f;
     c.sph_r = length(hi - c.sph_c) * 1.01f;
+    if (!(p->render_time > 0)) {
+        // scheduler::spi::sample: the regenerating loop; a slot whose light
+        // path ends takes the next one (path i of pass k as in the passes),
+        // so the long paths of a pass no longer hold up the next pass
+        if (!wf) {
+            wf = new (std::nothrow) Wavefront();
+            if (!wf) return fail(LMGPU_ERR_OUT_OF_MEMORY, "wavefront");
+        }
+        const int64_t total = p->num_samples;
+        LMGPU_TRY(wf->ensure_pool(Wavefront::pool_size(total, npix)));
+        const uint32_t pass0 = uint32_t(p->pass_offset);
+        const int W = accel.width;
+        auto iteration = [&](hipStream_t cs, int in_q) {
+            if (W == 4) lt_regen_iteration<4>(*this, *wf, c, total, npix, pass0, film, in_q, cs);
+            else if (W == 8) lt_regen_iteration<8>(*this, *wf, c, total, npix, pass0, film, in_q, cs);
+            else lt_regen_iteration<2>(*this, *wf, c, total, npix, pass0, film, in_q, cs);
+        };
+        double sec = 0;
+        int64_t iters = 0;
+        LMGPU_TRY(wf->loop(iteration, total, p->max_verts, &sec, &iters, s));
+        int32_t cnt[16];
+        LMGPU_HIP(hipMemcpyAsync(cnt, wf->counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+        LMGPU_HIP(hipStreamSynchronize(s));
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            int64_t v[3];
+            std::memcpy(v, cnt + 8, sizeof(v));
+            st->samples = total;
+            st->spp = (total + npix - 1) / npix;
+            st->extension_rays = v[1];
+            st->shadow_rays = v[2];
+            st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return LMGPU_OK;
+    }
     int64_t processed = 0, passes = 0;
     uint32_t pass = uint32_t(p->pass_offset);
     const int refill = accel.refill;

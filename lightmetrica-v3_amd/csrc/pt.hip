@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathState ps, Q
                                                    double* __restrict__ splat) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
+    if (n == 0) return;   // no shadow rays (whole grid)
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(q.sh + id);
         const float4 a = rp[0], b = rp[1];
@@ -781,9 +782,9 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         // camera rays are coherent: keep 64 consecutive rays per wave
         const int pm = b == 0 ? refill_primary : refill_secondary;
         hipLaunchKernelGGL(k_bounce_begin, dim3(1), dim3(64), 0, s, next.p);
-        if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
-        else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
-        else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm);
+        if (accel.width == 4) hipLaunchKernelGGL(k_extend<4>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm, LMGPU_RAY_CHUNK);
+        else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm, LMGPU_RAY_CHUNK);
+        else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm, LMGPU_RAY_CHUNK);
         if (gp)
             hipLaunchKernelGGL(k_pt_guide_infer, dim3(std::max(1, std::min((npix + 127) / 128, 512))), dim3(256), 0, s,
                                dev, q, in_q, gp->g->Wbf.p, gp->g->P.p, box, gmix.p);

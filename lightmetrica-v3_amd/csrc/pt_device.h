@@ -522,9 +522,11 @@ __device__ __forceinline__ void shade_vertex(const SceneDev& sc, const PTConsts&
 // written as (prim or -1 / -2 on stack overflow, t, u, v); shared by the
 // path and light tracers.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill) {
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in_q, unsigned long long* next, int refill,
+                                                  int chunk_rays) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[in_q];
+    if (n == 0) return;   // an empty queue (whole grid)
     const lmgpu_ray* rays = q.ext[in_q];
     auto fetch = [&](int64_t id, f3& o, f3& d, float& tmin, float& tmax) -> int {
         const float4* rp = reinterpret_cast<const float4*>(rays + id);
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, Queues q, int in
         reinterpret_cast<float4*>(q.hit)[id] = make_float4(__int_as_float(r.overflow ? -2 : r.prim), r.t, r.u, r.v);
     };
     trace_persistent<W, false, false, kLds, kPriv, kBlock>(nodes_of<W>(sc), sc.tris, n, next, lds, fetch, emit,
-                                                           q.counts + 5, nullptr, refill);
+                                                           q.counts + 5, nullptr, refill, chunk_rays);
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 10), (unsigned long long)n);
 }
