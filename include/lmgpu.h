@@ -82,7 +82,7 @@ typedef struct lmgpu_build_opts {
     int32_t builder;     /* LMGPU_BUILDER_LBVH (1) or LMGPU_BUILDER_PLOC (0); with opts == NULL: LBVH */
     int32_t ploc_radius; /* PLOC nearest-neighbour window, 1..64 (0 = 16) */
     int32_t collapse;    /* LMGPU_COLLAPSE_SAH (0) or LMGPU_COLLAPSE_GREEDY (1) */
-    float sah_tri_cost;  /* SAH collapse: cost of a triangle test relative to a wide-node visit (0 = default) */
+    float sah_tri_cost;  /* SAH collapse: cost of a triangle test relative to a wide-node visit (0 = 0.5) */
     int32_t reserved[1];
 } lmgpu_build_opts;
 

@@ -40,9 +40,12 @@
 namespace lmgpu {
 
 // Build defaults: leaves of at most 2 triangles, and a triangle test costing
-// as much as a wide-node visit in the SAH collapse.
+// half a wide-node visit in the SAH collapse (the watertight test of a
+// prefetched triangle is cheaper than a node's 4-8 slab tests plus the sort;
+// 0.5 measured 2-3% more incoherent closest-hit rays than 1.0 on a 1.32M
+// triangle mesh, with the same hits).
 constexpr int kDefaultLeafSize = 2;
-constexpr float kDefaultSahTriCost = 1.0f;
+constexpr float kDefaultSahTriCost = 0.5f;
 // Refit levels launched between two reads of the frontier size.
 constexpr int kRefitLevelsPerSync = 6;
 

@@ -26,7 +26,10 @@ def main():
     ap.add_argument('--width', type=int, default=4)
     ap.add_argument('--leaf', type=int, default=2)
     ap.add_argument('--rays', default='incoherent')
-    ap.add_argument('--pt', action='store_true', help='also run one 1080p config-2 PT pass')
+    ap.add_argument('--pt', action='store_true',
+                    help='also run one 1080p config-2 PT pass (one-sample pass graph) and a 4-spp render of it '
+                         '(regenerating loop)')
+    ap.add_argument('--pt-width', type=int, default=8, help='BVH width of the path-tracing scene')
     ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
     ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
     ap.add_argument('--c3', type=int, default=0, help='also render the first N config-3 jobs as one batch')
@@ -61,10 +64,13 @@ def main():
     if args.pt:
         from lm_amd.renderer import GpuScene
         s2 = scenes.config2().flatten()
-        sc = GpuScene(s2, width=args.width, leaf_size=args.leaf)
+        sc = GpuScene(s2, width=args.pt_width, leaf_size=args.leaf)
         sc.render_raw(spp=1, max_verts=16, seed=1)
         torch.cuda.synchronize()
         print('1 x 1080p config-2 PT pass', flush=True)
+        _, _, st = sc.render_raw(spp=4, max_verts=16, seed=1, sample_offset=1)
+        torch.cuda.synchronize()
+        print('4-spp 1080p config-2 PT render (regenerating loop)', st, flush=True)
     if args.lt_bdpt:
         from lm_amd.renderer import GpuScene
         s2 = scenes.config2().flatten()

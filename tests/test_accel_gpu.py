@@ -75,6 +75,32 @@ def test_small_mesh_vs_brute_force(device, width, leaf):
         compare(ref, got, flags, f'{label} W{width} L{leaf}')
 
 
+@pytest.mark.parametrize('width', [2, 4, 8])
+def test_collapse_variants(device, width):
+    """The SAH collapse is a least-cost cutting of the binary tree into wide
+    nodes and leaves. With a triangle test costing one node visit (the units
+    of the reported SAH cost) its tree costs no more than any other cutting
+    of the same binary tree: the greedy collapse and the SAH collapses at
+    triangle costs 0.5 (the default) and 2. Every variant, PLOC included,
+    returns the closest hits of brute force."""
+    from lm_amd._lib import COLLAPSE_GREEDY
+    m, sph = small_mesh()
+    verts = m.triangles()
+    rays = scenes.surface_rays(sph, 20000, 13)
+    ref = orc.brute_force(verts, rays)
+    flags = orc.Accel(verts).classify(rays, ref)
+    costs = {}
+    for label, kw in (('greedy', dict(collapse=COLLAPSE_GREEDY)), ('sah1', dict(sah_tri_cost=1.0)),
+                      ('default', {}), ('sah2', dict(sah_tri_cost=2.0)), ('ploc', dict(builder=0))):
+        acc = Accel(width, 2, **kw)
+        acc.build(verts)
+        costs[label] = acc.info()['sah_cost']
+        compare(ref, gpu_cast(acc, rays_from_dict(rays), device), flags, f'{label} W{width}')
+    print(f'W{width} SAH costs', costs)
+    for other in ('greedy', 'default', 'sah2'):
+        assert costs['sah1'] <= costs[other] * (1 + 1e-4), (other, costs)
+
+
 def test_occluded_matches_closest(device):
     import torch
     m, sph = small_mesh()

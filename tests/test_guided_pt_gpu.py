@@ -105,24 +105,34 @@ def test_guided_c4_lowers_error(device):
     passes, then frozen, lowers the relative MSE against a 2048-spp plain
     image at equal samples per pixel; the untrained guide (frozen at its
     random initialisation) does not beat plain PT. Guided runs use their
-    default of no Russian roulette, and so does the plain run compared."""
+    default of no Russian roulette, and so does the plain run compared.
+
+    Each error is the median over five independent 128-spp renders. The
+    one-sample mixture of guide and BSDF bounds a vertex's weight by
+    1 / (1 - alpha) times the BSDF's, so a path through several vertices
+    where the guide misses light has a heavy tail: about one render in
+    twenty of this scene holds a single pixel whose error outweighs the
+    rest of the image, and the median of five is not moved by it."""
     flat = c4_flat()
     sc = GpuScene(flat)
     ref = c4_reference(sc)
-    spp = 128
-    ps, _, _ = sc.render_raw(spp=spp, max_verts=16, rr_depth=16, seed=21)
-    e_plain = stats.rel_mse(ps / spp, ref)
+    spp, seeds = 128, (23, 24, 25, 26, 27)
+
+    def err(**kw):
+        return float(np.median([stats.rel_mse(sc.render_raw(spp=spp, max_verts=16, seed=sd, **kw)[0] / spp, ref)
+                                for sd in seeds]))
+
+    e_plain = err(rr_depth=16)
     lo, hi = sc.bounds()
     g0 = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-    us, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=23, guide=g0, guide_opts={'steps_per_pass': 0})
-    e_untrained = stats.rel_mse(us / spp, ref)
+    e_untrained = err(guide=g0, guide_opts={'steps_per_pass': 0})
     g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
     opts = {'steps_per_pass': 4, 'batch': 16384, 'learning_rate': 5e-3}
     sc.render_raw(spp=384, max_verts=16, seed=22, guide=g, guide_opts=opts)
-    gs, _, _ = sc.render_raw(spp=spp, max_verts=16, seed=23, guide=g, guide_opts={'steps_per_pass': 0})
-    e_guided = stats.rel_mse(gs / spp, ref)
-    print(f'C4 192x108 {spp} spp rMSE: plain {e_plain:.4e}, untrained guide {e_untrained:.4e}, '
-          f'trained guide {e_guided:.4e}; gain {e_plain / e_guided:.2f}, Adam steps {g.stats()[0]}')
+    e_guided = err(guide=g, guide_opts={'steps_per_pass': 0})
+    print(f'C4 192x108 {spp} spp median rMSE of {len(seeds)} renders: plain {e_plain:.4e}, untrained guide '
+          f'{e_untrained:.4e}, trained guide {e_guided:.4e}; gain {e_plain / e_guided:.2f}, '
+          f'Adam steps {g.stats()[0]}')
     assert e_plain / e_guided > 1.15
     assert e_untrained > e_guided
 
