@@ -261,7 +261,9 @@ __device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
 // with in from LDS (bf16, pitch pin), weights Wt[N][K] bf16 (output-neuron
 // major, row pitch PW, in LDS or global memory) and N = 64. The MFMA result
 // layout (lane l, element r: row 8(r/4) + 4(l/32) + r%4, column 32t + l%32)
-// is written back by `store(row, col, value)`.
+// is written back by `store(row, col, value, e, t)`, e and t being the
+// accumulator element and the 32-column half (row = 8(e/4) + 4(l/32) + e%4,
+// col = 32t + l%32).
 template <int K, int PW, int UNROLL, class Store>
 __device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin, const uint16_t* __restrict__ Wt,
                                         const float* __restrict__ bias, int lane, Store store) {
@@ -279,15 +281,15 @@ __device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int row = 8 * (e >> 2) + 4 * h + (e & 3);
-        store(row, r, acc0[e] + bb0);
-        store(row, 32 + r, acc1[e] + bb1);
+        store(row, r, acc0[e] + bb0, e, 0);
+        store(row, 32 + r, acc1[e] + bb1, e, 1);
     }
 }
 
 // Forward pass of the guide MLP for the 32 rows of a wave tile whose encoded
 // inputs are in lx ([32][kPitchX] bf16), with the weights staged in LDS by
 // stage_weights (wl). Raw outputs land in ly ([32][kPitchY] f32, columns
-// 0..39). `save(layer, row, col, bf16)` sees every hidden activation
+// 0..39). `save(layer, e, t, bf16)` sees every hidden activation
 // (layer 1..3) for training. Every wave of the block calls this the same
 // number of times (block barriers between the layers); callers put a block
 // barrier after staging the weights and writing lx, and after this call
@@ -295,26 +297,26 @@ __device__ __forceinline__ void dense64(const uint16_t* __restrict__ in, int pin
 template <class Save>
 __device__ __forceinline__ void mlp_tile_forward(const uint16_t* wl, const float* __restrict__ P, uint16_t* lx,
                                                  uint16_t* lh, float* ly, int lane, Save save) {
-    dense64<kIn, kPW1, kIn / 16>(lx, kPitchX, wl, P + kOffB1, lane, [&](int row, int col, float v) {
+    dense64<kIn, kPW1, kIn / 16>(lx, kPitchX, wl, P + kOffB1, lane, [&](int row, int col, float v, int e, int t) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lh[row * kPitchH + col] = b;
-        save(1, row, col, b);
+        save(1, e, t, b);
     });
     __syncthreads();
-    dense64<kHid, kPWh, kHid / 16>(lh, kPitchH, wl + kWOff2, P + kOffB2, lane, [&](int row, int col, float v) {
+    dense64<kHid, kPWh, kHid / 16>(lh, kPitchH, wl + kWOff2, P + kOffB2, lane, [&](int row, int col, float v, int e, int t) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lx[row * kPitchX + col] = b;
-        save(2, row, col, b);
+        save(2, e, t, b);
     });
     __syncthreads();
-    dense64<kHid, kPWh, kHid / 16>(lx, kPitchX, wl + kWOff2 + kWOffH, P + kOffB3, lane, [&](int row, int col, float v) {
+    dense64<kHid, kPWh, kHid / 16>(lx, kPitchX, wl + kWOff2 + kWOffH, P + kOffB3, lane, [&](int row, int col, float v, int e, int t) {
         const uint16_t b = f2bf(fmaxf(v, 0.f));
         lh[row * kPitchH + col] = b;
-        save(3, row, col, b);
+        save(3, e, t, b);
     });
     __syncthreads();
     dense64<kHid, kPWh, kHid / 16>(lh, kPitchH, wl + kWOff2 + 2 * kWOffH, P + kOffB4, lane,
-                           [&](int row, int col, float v) { ly[row * kPitchY + col] = v; });
+                           [&](int row, int col, float v, int, int) { ly[row * kPitchY + col] = v; });
 }
 
 }  // namespace lmgpu

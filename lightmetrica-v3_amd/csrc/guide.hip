@@ -50,16 +50,11 @@ __device__ __forceinline__ int64_t batch_rows(const int32_t* ndev, int64_t off, 
 __device__ __forceinline__ int64_t round128(int64_t n) { return (n + 127) / 128 * 128; }
 
 // Forward over query arrays. query[i] = pos(3), nrm(3), wo(3) (stride qs
-// floats). INFER: decoded mixtures mix[i][40] = pi[8], kappa[8], mu[8][3].
-// TRAIN: saves xT[96][B], hT[3][64][B] (bf16, feature major) and raw y[i][40].
-template <bool TRAIN>
-__global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
-                                                       const float* __restrict__ query, int qs,
+// floats) -> decoded mixtures mix[i][40] = pi[8], kappa[8], mu[8][3].
+__global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n, const float* __restrict__ query, int qs,
                                                        const uint16_t* __restrict__ Wbf, const float* __restrict__ P,
-                                                       Box box, float* __restrict__ mix, uint16_t* __restrict__ xT,
-                                                       uint16_t* __restrict__ hT, float* __restrict__ yout) {
+                                                       Box box, float* __restrict__ mix) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
-    const int64_t n = batch_rows(ndev, off, n_max);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
     stage_weights(Wbf, wl, threadIdx.x, kGB);
@@ -67,13 +62,12 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
     uint16_t* lh = lx + 32 * kPitchX;
     float* ly = reinterpret_cast<float*>(lx);
     // two lanes per row: lanes r and r + 32 share row r of the wave's 32-row
-    // tile and each encode, save and decode half of it
+    // tile and each encode and decode half of it
     const int r = lane & 31, half = lane >> 5;
-    const int k_lo = half * (kIn / 2), k_hi = k_lo + kIn / 2;   // feature half for zeroing and saving
+    const int k_lo = half * (kIn / 2), k_hi = k_lo + kIn / 2;   // feature half for zeroing
     for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < n; base += int64_t(gridDim.x) * kRowsPerBlock) {
-        const int64_t i0 = base + wave * 32;
+        const int64_t i = base + wave * 32 + r;
         {
-            const int64_t i = i0 + r;
             uint16_t* row = lx + r * kPitchX;
             if (i < n) {
                 const float* q = query + i * qs;
@@ -84,42 +78,22 @@ __global__ __launch_bounds__(kGB) void k_guide_forward(int64_t n_max, const int3
             }
         }
         __syncthreads();
-        if (TRAIN) {
-            // the feature-major copy for the weight gradients, after both halves of each row are written
-            const int64_t i = i0 + r;
-            const uint16_t* row = lx + r * kPitchX;
-            if (i < B)
-                for (int k = k_lo; k < k_hi; k++) xT[int64_t(k) * B + i] = row[k];
-        }
-        mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int row, int col, uint16_t v) {
-            if (TRAIN) {
-                const int64_t i = i0 + row;
-                if (i < B) hT[(int64_t(layer - 1) * kHid + col) * B + i] = v;
-            }
-        });
+        mlp_tile_forward(wl, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
         __syncthreads();
-        {
-            const int64_t i = i0 + r;
-            if (i < n) {
-                const float* y = ly + r * kPitchY;
-                if (TRAIN) {
-                    for (int k = half * (kOut / 2); k < (half + 1) * (kOut / 2); k++) yout[i * kOut + k] = y[k];
-                } else {
-                    decode_lobes4(y, 4 * half, mix + i * kOut);
-                }
-            }
-        }
+        if (i < n) decode_lobes4(ly + r * kPitchY, 4 * half, mix + i * kOut);
         __syncthreads();
     }
 }
 
 // Target weights w_i = lum(L_i) / p_mix_i of training records
-// rec[i] = pos(3), nrm(3), wo(3), wi(3), p_mix, L(3); sum into *wsum.
+// rec[i] = pos(3), nrm(3), wo(3), wi(3), p_mix, L(3); sum into *wsum (zero
+// on entry); clears *loss, which the row kernel accumulates.
 __global__ __launch_bounds__(256) void k_guide_weights(int64_t n_max, const int32_t* ndev, int64_t off,
                                                        const float* __restrict__ rec, float* __restrict__ w,
-                                                       float* __restrict__ wsum) {
+                                                       float* __restrict__ wsum, float* __restrict__ loss) {
     __shared__ float red[4];
     const int64_t n = batch_rows(ndev, off, n_max);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.f;
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     float v = 0.f;
     if (i < n) {
@@ -135,110 +109,155 @@ __global__ __launch_bounds__(256) void k_guide_weights(int64_t n_max, const int3
     if (threadIdx.x == 0) atomicAdd(wsum, red[0] + red[1] + red[2] + red[3]);
 }
 
-// Per-sample gradient of -wt log p_guide(w_i) w.r.t. the 40 raw outputs,
-// scaled by 1/n, written feature-major as bf16 into d4T[64][B]; rows from n
-// to the end of the last 128-row tile are zeroed.
-__global__ __launch_bounds__(256) void k_guide_loss(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
-                                                    const float* __restrict__ y, const float* __restrict__ rec,
-                                                    const float* __restrict__ w, const float* __restrict__ wsum,
-                                                    float wclamp, uint16_t* __restrict__ d4T,
-                                                    float* __restrict__ loss) {
+// One training pass over 32-row wave tiles of a batch: input encoding,
+// forward (saving xT and hT, feature major, for the weight gradients), the
+// gradient of the loss -wt log p_guide(w_i) w.r.t. the 40 raw outputs of
+// every row (scaled by 1/n), and backward through layers 4..2 (saving d4T,
+// d3T, d2T, d1T). The raw outputs and the deltas stay in LDS; rows from n to
+// the end of the last 128-row tile get zero deltas. Lanes r and r + 32 share
+// row r: each encodes half of it and differentiates four of the eight lobes.
+__global__ __launch_bounds__(kGB) void k_guide_train_rows(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+                                                          const float* __restrict__ rec,
+                                                          const uint16_t* __restrict__ Wbf,
+                                                          const float* __restrict__ P,
+                                                          const uint16_t* __restrict__ WnBf,
+                                                          const float* __restrict__ zero64, Box box,
+                                                          const float* __restrict__ w, const float* __restrict__ wsum,
+                                                          float wclamp, uint16_t* __restrict__ xT,
+                                                          uint16_t* __restrict__ hT, uint16_t* __restrict__ dT,
+                                                          float* __restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
     __shared__ float red[4];
     const int64_t n = batch_rows(ndev, off, n_max);
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    float li = 0.f;
-    if (i >= n && i < round128(n))
-        for (int k = 0; k < kOut; k++) d4T[int64_t(k) * B + i] = 0;
-    if (i < n) {
-        const float mean = *wsum / float(n);
-        const float wt = mean > 0.f ? fminf(w[i] / mean, wclamp) : 0.f;
-        const float s = wt / float(n);
-        const float* yi = y + i * kOut;
-        const float* r = rec + i * kRecordFloats;
-        const f3 wi = normalize(mk(r[9], r[10], r[11]));
-        Mixture m;
-        decode_mixture(yi, m);
-        float f[kLobes];
-        float p = 0.f;
-        for (int k = 0; k < kLobes; k++) {
-            f[k] = vmf_pdf(m.kappa[k], m.mu[k], wi);
-            p += m.pi[k] * f[k];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
+    stage_weights(Wbf, wl, threadIdx.x, kGB);
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
+    uint16_t* lh = lx + 32 * kPitchX;
+    float* ly = reinterpret_cast<float*>(lx);
+    const int r = lane & 31, half = lane >> 5;
+    const int k_lo = half * (kIn / 2), k_hi = k_lo + kIn / 2;
+    const float mean = n > 0 ? *wsum / float(n) : 0.f;
+    float lsum = 0.f;
+    for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < n; base += int64_t(gridDim.x) * kRowsPerBlock) {
+        const int64_t i0 = base + wave * 32;
+        const int64_t i = i0 + r;
+        // MFMA result element e of half t of this lane is row 8(e/4) + e%4 +
+        // 4 * half, column 32t + r (dense64); its bit in a ReLU mask is 2e + t
+        // (the backward pass produces delta_{l-1} in the layout in which the
+        // forward produced h_{l-1}, so the masks stay in registers)
+        const int64_t ib = i0 + 4 * half;
+        {
+            uint16_t* row = lx + r * kPitchX;
+            if (i < n) {
+                const float* q = rec + i * kRecordFloats;
+                encode_input_half(half, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), box.lo,
+                                  box.inv, row);
+            } else {
+                for (int k = k_lo; k < k_hi; k++) row[k] = 0;
+            }
         }
-        float g[kOut];
-        for (int k = 0; k < kOut; k++) g[k] = 0.f;
-        if (s > 0.f && p > 1e-30f) {
-            li = -wt * logf(p);
-            for (int k = 0; k < kLobes; k++) {
-                const float rk = m.pi[k] * f[k] / p;
-                g[k] = s * (m.pi[k] - rk);
-                const float kap = m.kappa[k];
-                const float raw = yi[kLobes + k];
-                const float sp = softplus(raw);
-                if (sp > kKappaMin && sp < kKappaMax) {
-                    const float dlogf = 1.f / kap - 2.f / expm1f(2.f * kap) + (dot(m.mu[k], wi) - 1.f);
-                    const float sig = 1.f / (1.f + expf(-raw));
-                    g[kLobes + k] = -s * rk * dlogf * sig;
+        __syncthreads();
+        {
+            // the feature-major copy for the weight gradients, after both halves of each row are written
+            const uint16_t* row = lx + r * kPitchX;
+            if (i < B)
+                for (int k = k_lo; k < k_hi; k++) xT[int64_t(k) * B + i] = row[k];
+        }
+        uint32_t m1 = 0, m2 = 0, m3 = 0;   // ReLU masks of h1, h2, h3
+        mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int e, int t, uint16_t v) {
+            const int rc = 8 * (e >> 2) + (e & 3);
+            if (ib + rc < B) hT[(int64_t(layer - 1) * kHid + 32 * t + r) * B + ib + rc] = v;
+            const uint32_t bit = bf2f(v) > 0.f ? 1u << (2 * e + t) : 0u;
+            if (layer == 1) m1 |= bit;
+            else if (layer == 2) m2 |= bit;
+            else m3 |= bit;
+        });
+        __syncthreads();
+        // loss gradient of row r, lobes 4 * half .. 4 * half + 3
+        float g[5 * 4];   // [0..3] logits, [4..7] concentrations, [8..19] mean directions
+        for (int k = 0; k < 20; k++) g[k] = 0.f;
+        {
+            float pi[4], f[4], kap[4], raw_k[4], len[4], s = 0.f, wt = 0.f, pp = 0.f;
+            f3 mu[4], rv[4];
+            f3 wi = mk(0.f, 0.f, 1.f);
+            if (i < n) {
+                const float* y = ly + r * kPitchY;
+                const float* q = rec + i * kRecordFloats;
+                wt = mean > 0.f ? fminf(w[i] / mean, wclamp) : 0.f;
+                s = wt / float(n);
+                wi = normalize(mk(q[9], q[10], q[11]));
+                float mx = y[0];
+                for (int k = 1; k < kLobes; k++) mx = fmaxf(mx, y[k]);
+                float se = 0.f;
+                for (int k = 0; k < kLobes; k++) se += expf(y[k] - mx);
+                const float inv = 1.f / se;
+                for (int j = 0; j < 4; j++) {
+                    const int k = 4 * half + j;
+                    pi[j] = expf(y[k] - mx) * inv;
+                    raw_k[j] = y[kLobes + k];
+                    kap[j] = fminf(fmaxf(softplus(raw_k[j]), kKappaMin), kKappaMax);
+                    rv[j] = mk(y[2 * kLobes + 3 * k], y[2 * kLobes + 3 * k + 1], y[2 * kLobes + 3 * k + 2]);
+                    len[j] = length(rv[j]);
+                    mu[j] = len[j] > 1e-8f ? rv[j] / len[j] : mk(0.f, 0.f, 1.f);
+                    f[j] = vmf_pdf(kap[j], mu[j], wi);
+                    pp += pi[j] * f[j];
                 }
-                const f3 rv = mk(yi[2 * kLobes + 3 * k], yi[2 * kLobes + 3 * k + 1], yi[2 * kLobes + 3 * k + 2]);
-                const float len = length(rv);
-                if (len > 1e-8f) {
-                    const f3 gm = wi * (-s * rk * kap);
-                    const f3 gr = (gm - m.mu[k] * dot(gm, m.mu[k])) / len;
-                    g[2 * kLobes + 3 * k] = gr.x;
-                    g[2 * kLobes + 3 * k + 1] = gr.y;
-                    g[2 * kLobes + 3 * k + 2] = gr.z;
+            }
+            const float p = pp + __shfl_xor(pp, 32);
+            if (i < n && s > 0.f && p > 1e-30f) {
+                if (half == 0) lsum += -wt * logf(p);
+                for (int j = 0; j < 4; j++) {
+                    const float rk = pi[j] * f[j] / p;
+                    g[j] = s * (pi[j] - rk);
+                    const float sp = softplus(raw_k[j]);
+                    if (sp > kKappaMin && sp < kKappaMax) {
+                        const float dlogf = 1.f / kap[j] - 2.f / expm1f(2.f * kap[j]) + (dot(mu[j], wi) - 1.f);
+                        const float sig = 1.f / (1.f + expf(-raw_k[j]));
+                        g[4 + j] = -s * rk * dlogf * sig;
+                    }
+                    if (len[j] > 1e-8f) {
+                        const f3 gm = wi * (-s * rk * kap[j]);
+                        const f3 gr = (gm - mu[j] * dot(gm, mu[j])) / len[j];
+                        g[8 + 3 * j] = gr.x;
+                        g[8 + 3 * j + 1] = gr.y;
+                        g[8 + 3 * j + 2] = gr.z;
+                    }
                 }
             }
         }
-        for (int k = 0; k < kOut; k++) d4T[int64_t(k) * B + i] = f2bf(g[k]);
-    }
-    for (int sh = 32; sh > 0; sh >>= 1) li += __shfl_xor(li, sh);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = li;
-    __syncthreads();
-    if (threadIdx.x == 0 && n > 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / float(n));
-}
-
-// Backward through layers 4..2 for 32-row wave tiles: delta_{l-1} =
-// (delta_l W_l^T) * relu'(h_{l-1}); saves d3T, d2T, d1T (bf16, feature major).
-__global__ __launch_bounds__(kGB) void k_guide_backward(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
-                                                        const uint16_t* __restrict__ WnBf,
-                                                        const float* __restrict__ zero64,
-                                                        const uint16_t* __restrict__ hT, uint16_t* __restrict__ dT) {
-    constexpr int kWnLds = 3 * kHid * kPWh * 2;   // W2..W4 as [k][n], rows padded to kPWh
-    __shared__ __attribute__((aligned(16))) unsigned char lds[kWnLds + 4 * kWaveLds];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* wn = reinterpret_cast<uint16_t*>(lds);
-    for (int c = threadIdx.x; c < 3 * kHid * (kHid / 8); c += kGB) {
-        const int r = c / (kHid / 8), k = (c % (kHid / 8)) * 8;   // r over the 3 x 64 rows
-        *reinterpret_cast<uint4*>(wn + r * kPWh + k) = *reinterpret_cast<const uint4*>(WnBf + r * kHid + k);
-    }
-    __syncthreads();
-    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWnLds + wave * kWaveLds);
-    uint16_t* lh = lx + 32 * kPitchX;
-    const int64_t rows = round128(batch_rows(ndev, off, n_max));
-    for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < rows; base += int64_t(gridDim.x) * kRowsPerBlock) {
-        const int64_t i0 = base + wave * 32;
-        if (lane < 32) {
-            const int64_t i = i0 + lane;
-            for (int k = 0; k < kOutPad; k++)
-                lx[lane * kPitchX + k] = (i < B && k < kOut) ? dT[(3 * int64_t(kHid) + k) * B + i] : uint16_t(0);
+        __syncthreads();   // every lane has read the raw outputs that the deltas overwrite
+        {
+            // delta of the 64 (padded) outputs as the A operand of the backward pass, and d4T
+            uint16_t* row = lx + r * kPitchX;
+            uint16_t* d4 = dT + 3 * int64_t(kHid) * B;
+            auto put = [&](int col, float v) {
+                const uint16_t b = f2bf(v);
+                row[col] = b;
+                if (i < B) d4[int64_t(col) * B + i] = b;
+            };
+            for (int j = 0; j < 4; j++) {
+                put(4 * half + j, g[j]);
+                put(kLobes + 4 * half + j, g[4 + j]);
+                for (int c = 0; c < 3; c++) put(2 * kLobes + 12 * half + 3 * j + c, g[8 + 3 * j + c]);
+            }
+            for (int c = kOut + 12 * half; c < kOut + 12 * (half + 1); c++) row[c] = 0;
         }
         __syncthreads();
-        // layer l = 4, 3, 2: input delta in `in`, output to `out`, saved as d_{l-1}
+        // backward: delta_{l-1} = (delta_l W_l) * relu'(h_{l-1}), l = 4, 3, 2
         uint16_t* in = lx;
         uint16_t* out = lh;
         int pin = kPitchX, pout = kPitchH;
+#pragma unroll 1
         for (int l = 4; l >= 2; l--) {
-            const uint16_t* Wn = wn + (l - 2) * kHid * kPWh;
-            const uint16_t* hprev = hT + int64_t(l - 2) * kHid * B;   // h_{l-1}
-            uint16_t* dsave = dT + int64_t(l - 2) * kHid * B;          // d_{l-1}
-            dense64<kHid, kPWh, 1>(in, pin, Wn, zero64, lane, [&](int row, int col, float v) {
-                const int64_t i = i0 + row;
-                float g = 0.f;
-                if (i < B && bf2f(hprev[int64_t(col) * B + i]) > 0.f) g = v;
-                const uint16_t b = f2bf(g);
+            const uint32_t m = l == 4 ? m3 : (l == 3 ? m2 : m1);
+            uint16_t* dsave = dT + int64_t(l - 2) * kHid * B;
+            dense64<kHid, kHid, 1>(in, pin, WnBf + (l - 2) * kHid * kHid, zero64, lane,
+                                   [&](int row, int col, float v, int e, int t) {
+                const uint16_t b = f2bf((m >> (2 * e + t)) & 1u ? v : 0.f);
                 out[row * pout + col] = b;
-                if (i < B) dsave[int64_t(col) * B + i] = b;
+                const int rc = 8 * (e >> 2) + (e & 3);
+                if (ib + rc < B) dsave[int64_t(32 * t + r) * B + ib + rc] = b;
             });
             __syncthreads();
             uint16_t* t = in;
@@ -249,12 +268,19 @@ __global__ __launch_bounds__(kGB) void k_guide_backward(int64_t n_max, const int
             pout = tp;
         }
     }
+    for (int sh = 32; sh > 0; sh >>= 1) lsum += __shfl_xor(lsum, sh);
+    if (lane == 0) red[wave] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0 && n > 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / float(n));
 }
 
 // Weight gradients dWt_l[n][k] = sum_i d_l[n][i] a_{l-1}[k][i] for one
 // (layer, 32x32 tile) job per blockIdx.y over a slice of samples per
 // blockIdx.x; MFMA with the sample index as the reduction dimension,
-// accumulated into grad with one atomic per element per slice.
+// accumulated into grad with one atomic per element per slice. The jobs of
+// input tile 0 also sum their delta operand, the bias gradient
+// db_l[n] = sum_i d_l[n][i]. Block (0, 0) advances the Adam step count of a
+// non-empty batch (the Adam kernel that follows reads it).
 struct DwJob {
     int layer, nt, kt;
 };
@@ -262,20 +288,28 @@ __constant__ DwJob c_dw_jobs[18];
 
 __global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B, int slice,
                                                  const uint16_t* __restrict__ xT, const uint16_t* __restrict__ hT,
-                                                 const uint16_t* __restrict__ dT, float* __restrict__ grad) {
+                                                 const uint16_t* __restrict__ dT, float* __restrict__ grad,
+                                                 int32_t* __restrict__ stepc) {
     const DwJob job = c_dw_jobs[blockIdx.y];
     const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
     const int K = job.layer == 1 ? kIn : kHid;
     const int N = job.layer == 4 ? kOut : kHid;
     const uint16_t* A = dT + int64_t(job.layer - 1) * kHid * B + int64_t(job.nt * 32 + r) * B;
     const uint16_t* Bm = (job.layer == 1 ? xT : hT + int64_t(job.layer - 2) * kHid * B) + int64_t(job.kt * 32 + r) * B;
+    const int64_t rows = batch_rows(ndev, off, n_max);
     const int64_t i_beg = int64_t(blockIdx.x) * slice;
-    const int64_t i_end = std::min<int64_t>(round128(batch_rows(ndev, off, n_max)), i_beg + slice);
+    const int64_t i_end = std::min<int64_t>(round128(rows), i_beg + slice);
+    const bool bias = job.kt == 0;
     f32x16 acc = {};
+    float bsum = 0.f;
     for (int64_t i = i_beg; i < i_end; i += 16) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + i + 8 * h);
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bm + i + 8 * h);
         acc = mfma_bf16(a, b, acc);
+        if (bias) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) bsum += float(a[e]);
+        }
     }
     const int poff = job.layer == 1 ? kOffW1 : (job.layer == 2 ? kOffW2 : (job.layer == 3 ? kOffW3 : kOffW4));
 #pragma unroll
@@ -284,37 +318,13 @@ __global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* n
         const int k = job.kt * 32 + r;
         if (n < N && k < K) atomicAdd(&grad[poff + n * K + k], acc[e]);
     }
-}
-
-// Bias gradients: db_l[n] = sum_i d_l[n][i]; blocks (layer * 64 + n, chunk)
-// read 8 bf16 per lane and add their partial sum atomically (G starts at 0).
-__global__ __launch_bounds__(256) void k_guide_db(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
-                                                  const uint16_t* __restrict__ dT, float* __restrict__ grad) {
-    __shared__ float red[4];
-    const int layer = blockIdx.x / kHid + 1, n = blockIdx.x % kHid;
-    if (layer == 4 && n >= kOut) return;   // whole block: padded output rows
-    const uint4* d = reinterpret_cast<const uint4*>(dT + int64_t(blockIdx.x) * B);
-    const int64_t nvec = round128(batch_rows(ndev, off, n_max)) / 8;
-    float s = 0.f;
-    for (int64_t v = int64_t(blockIdx.y) * 256 + threadIdx.x; v < nvec; v += int64_t(gridDim.y) * 256) {
-        const uint4 q = d[v];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float lo, hi;
-            const uint32_t a = w[k] << 16, b = w[k] & 0xffff0000u;
-            memcpy(&lo, &a, 4);
-            memcpy(&hi, &b, 4);
-            s += lo + hi;
-        }
+    if (bias) {
+        bsum += __shfl_xor(bsum, 32);
+        const int boff = job.layer == 1 ? kOffB1 : (job.layer == 2 ? kOffB2 : (job.layer == 3 ? kOffB3 : kOffB4));
+        const int n = job.nt * 32 + r;
+        if (h == 0 && n < N && i_end > i_beg) atomicAdd(&grad[boff + n], bsum);
     }
-    for (int sh = 32; sh > 0; sh >>= 1) s += __shfl_xor(s, sh);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int poff = layer == 1 ? kOffB1 : (layer == 2 ? kOffB2 : (layer == 3 ? kOffB3 : kOffB4));
-        atomicAdd(&grad[poff + n], red[0] + red[1] + red[2] + red[3]);
-    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0 && rows > 0) *stepc += 1;
 }
 
 // Refreshes the BF16 copies: Wbf (packed, transposed weights) and WnBf
@@ -339,18 +349,22 @@ __global__ __launch_bounds__(256) void k_guide_sync(const float* __restrict__ P,
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16
-// copies. The step count t lives in device memory (k_guide_step advances it
-// afterwards); an empty batch leaves everything unchanged.
+// copies and clears the gradient and the target-weight sum for the next step
+// (both are zero between steps). The step count t lives in device memory (k_guide_dw
+// has advanced it); an empty batch leaves everything unchanged.
 __global__ __launch_bounds__(256) void k_guide_adam(int64_t n_max, const int32_t* ndev, int64_t off,
                                                     const int32_t* __restrict__ stepc, float* __restrict__ P,
                                                     float* __restrict__ M, float* __restrict__ V,
-                                                    const float* __restrict__ G, float lr, float b1, float b2, float eps,
-                                                    uint16_t* __restrict__ Wbf, uint16_t* __restrict__ WnBf) {
+                                                    float* __restrict__ G, float lr, float b1, float b2, float eps,
+                                                    uint16_t* __restrict__ Wbf, uint16_t* __restrict__ WnBf,
+                                                    float* __restrict__ wsum) {
     const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j == 0) *wsum = 0.f;
     if (j >= kNParams || batch_rows(ndev, off, n_max) == 0) return;
-    const float t = float(*stepc + 1);
+    const float t = float(*stepc);
     const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
     const float g = G[j];
+    G[j] = 0.f;
     const float m = b1 * M[j] + (1.f - b1) * g;
     const float v = b2 * V[j] + (1.f - b2) * g * g;
     M[j] = m;
@@ -358,10 +372,6 @@ __global__ __launch_bounds__(256) void k_guide_adam(int64_t n_max, const int32_t
     const float p = P[j] - lr * (m / c1) / (sqrtf(v / c2) + eps);
     P[j] = p;
     write_bf16(j, p, Wbf, WnBf);
-}
-
-__global__ void k_guide_step(int64_t n_max, const int32_t* ndev, int64_t off, int32_t* stepc) {
-    if (threadIdx.x == 0 && batch_rows(ndev, off, n_max) > 0) *stepc += 1;
 }
 
 __global__ void k_zero_f32(float* p, int64_t n) {
@@ -457,6 +467,8 @@ int Guide::set_params(const float* host) {
     LMGPU_HIP(hipMemset(M.p, 0, sizeof(float) * kNParams));
     LMGPU_HIP(hipMemset(V.p, 0, sizeof(float) * kNParams));
     LMGPU_HIP(hipMemset(stepc.p, 0, sizeof(int32_t)));
+    LMGPU_HIP(hipMemset(G.p, 0, sizeof(float) * kNParams));
+    LMGPU_HIP(hipMemset(scal.p, 0, 4 * sizeof(float)));
     hipLaunchKernelGGL(k_guide_sync, dim3(nb(kNParams, 256)), dim3(256), 0, nullptr, P.p, Wbf.p, WnBf.p);
     LMGPU_HIP(hipGetLastError());
     LMGPU_HIP(hipDeviceSynchronize());
@@ -477,19 +489,18 @@ int Guide::infer(const float* query, int qs, float* mix, int64_t n, hipStream_t 
     if (n <= 0) return LMGPU_OK;
     Box b;
     for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
-    hipLaunchKernelGGL(k_guide_forward<false>, dim3(grid_for(n)), dim3(kGB), 0, s, n, nullptr, int64_t(0), int64_t(0),
-                       query, qs, Wbf.p, P.p, b, mix, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid_for(n)), dim3(kGB), 0, s, n, query, qs, Wbf.p, P.p, b, mix);
     LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }
 
 int Guide::ensure_train(int64_t B) {
     if (B % 128) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training capacity must be a multiple of 128");
+    if (B > (int64_t(1) << 22)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training batch above 2^22 records");
     if (B <= train_cap) return LMGPU_OK;
     LMGPU_TRY(xT.alloc(size_t(96) * B));
     LMGPU_TRY(hT.alloc(size_t(3) * kHid * B));
     LMGPU_TRY(dT.alloc(size_t(4) * kHid * B));
-    LMGPU_TRY(y.alloc(size_t(kOut) * B));
     LMGPU_TRY(w.alloc(B));
     // Input rows 80..95 and output-delta rows 40..63 are read by the 32-wide
     // weight-gradient tiles and never written: they stay zero.
@@ -504,27 +515,22 @@ int Guide::ensure_train(int64_t B) {
 int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* ndev, int64_t off, float lr,
                          float* grad_host, hipStream_t s, float clamp) {
     if (B > train_cap || n > B) return fail(LMGPU_ERR_INVALID_ARGUMENT, "training batch exceeds the workspace");
-    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, scal.p, int64_t(4));
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb(kNParams, 256)), dim3(256), 0, s, G.p, int64_t(kNParams));
-    hipLaunchKernelGGL(k_guide_weights, dim3(nb(n, 256)), dim3(256), 0, s, n, ndev, off, rec, w.p, scal.p);
+    // four launches per step: target weights summed (and the loss cleared);
+    // the rows (forward, loss gradient, backward); weight and bias gradients
+    // with the step count; Adam, which also clears the gradient and the weight
+    // sum for the next step
+    hipLaunchKernelGGL(k_guide_weights, dim3(nb(n, 256)), dim3(256), 0, s, n, ndev, off, rec, w.p, scal.p,
+                       scal.p + 1);
     Box b;
     for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
-    hipLaunchKernelGGL(k_guide_forward<true>, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, rec, kRecordFloats,
-                       Wbf.p, P.p, b, nullptr, xT.p, hT.p, y.p);
-    hipLaunchKernelGGL(k_guide_loss, dim3(nb(B, 256)), dim3(256), 0, s, n, ndev, off, B, y.p, rec, w.p, scal.p,
-                       clamp > 0.f ? clamp : wclamp,
-                       dT.p + 3 * kHid * B, scal.p + 1);
-    hipLaunchKernelGGL(k_guide_backward, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, WnBf.p, zero64.p, hT.p,
-                       dT.p);
+    hipLaunchKernelGGL(k_guide_train_rows, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, rec, Wbf.p, P.p,
+                       WnBf.p, zero64.p, b, w.p, scal.p, clamp > 0.f ? clamp : wclamp, xT.p, hT.p, dT.p, scal.p + 1);
     const int slice = 4096;
     hipLaunchKernelGGL(k_guide_dw, dim3(nb(B, slice), 18), dim3(64), 0, s, n, ndev, off, B, slice, xT.p, hT.p, dT.p,
-                       G.p);
-    hipLaunchKernelGGL(k_guide_db, dim3(4 * kHid, unsigned(std::min<int64_t>(16, (B / 8 + 255) / 256))), dim3(256), 0,
-                       s, n, ndev, off, B, dT.p, G.p);
+                       G.p, stepc.p);
     if (grad_host) LMGPU_HIP(hipMemcpyAsync(grad_host, G.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_guide_adam, dim3(nb(kNParams, 256)), dim3(256), 0, s, n, ndev, off, stepc.p, P.p, M.p, V.p,
-                       G.p, lr, 0.9f, 0.999f, 1e-8f, Wbf.p, WnBf.p);
-    hipLaunchKernelGGL(k_guide_step, dim3(1), dim3(64), 0, s, n, ndev, off, stepc.p);
+                       G.p, lr, 0.9f, 0.999f, 1e-8f, Wbf.p, WnBf.p, scal.p);
     LMGPU_HIP(hipGetLastError());
     return LMGPU_OK;
 }

@@ -16,12 +16,12 @@ namespace lmgpu {
 constexpr int kRecordFloats = 16;
 
 struct Guide {
-    DevBuf<float> P, M, V, G;       // parameters, Adam moments, gradients (packed layout)
+    DevBuf<float> P, M, V, G;       // parameters, Adam moments, gradients (packed layout; G is zero between steps)
     DevBuf<uint16_t> Wbf, WnBf;     // BF16 copies: packed transposed weights; layers 2..4 as W[k][n]
     DevBuf<float> zero64, scal;     // zero bias; scalars: [0] weight sum, [1] loss of the last step
     DevBuf<int32_t> stepc;          // Adam step count
     DevBuf<uint16_t> xT, hT, dT;    // training activations / deltas, feature major
-    DevBuf<float> y, w;             // raw outputs, target weights
+    DevBuf<float> w;                // target weights
     int64_t train_cap = 0;
     // identity of the device buffers, for callers that capture them in graphs:
     // uid is unique per guide in the process, gen grows with every reallocation
