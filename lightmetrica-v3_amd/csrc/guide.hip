@@ -31,6 +31,11 @@ namespace lmgpu {
 
 namespace {
 
+// Samples per block of the weight-gradient kernel (a multiple of 128).
+#ifndef LMGPU_DW_SLICE
+#define LMGPU_DW_SLICE 1024
+#endif
+
 constexpr int kGB = 256;                  // 4 waves x 32 rows per block tile
 constexpr int kRowsPerBlock = 128;
 
@@ -109,14 +114,14 @@ __global__ __launch_bounds__(256) void k_guide_weights(int64_t n_max, const int3
     if (threadIdx.x == 0) atomicAdd(wsum, red[0] + red[1] + red[2] + red[3]);
 }
 
-// One training pass over 32-row wave tiles of a batch: input encoding,
+// One training pass over the 128-row tile of a block (32 rows per wave): input encoding,
 // forward (saving xT and hT, feature major, for the weight gradients), the
 // gradient of the loss -wt log p_guide(w_i) w.r.t. the 40 raw outputs of
 // every row (scaled by 1/n), and backward through layers 4..2 (saving d4T,
 // d3T, d2T, d1T). The raw outputs and the deltas stay in LDS; rows from n to
 // the end of the last 128-row tile get zero deltas. Lanes r and r + 32 share
 // row r: each encodes half of it and differentiates four of the eight lobes.
-__global__ __launch_bounds__(kGB) void k_guide_train_rows(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
+__global__ __launch_bounds__(kGB, 2) void k_guide_train_rows(int64_t n_max, const int32_t* ndev, int64_t off, int64_t B,
                                                           const float* __restrict__ rec,
                                                           const uint16_t* __restrict__ Wbf,
                                                           const float* __restrict__ P,
@@ -129,6 +134,10 @@ __global__ __launch_bounds__(kGB) void k_guide_train_rows(int64_t n_max, const i
     __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
     __shared__ float red[4];
     const int64_t n = batch_rows(ndev, off, n_max);
+    // one 128-row tile per block (no loop over tiles: values the tiles would
+    // share stay out of registers, so two waves fit per SIMD)
+    const int64_t base = int64_t(blockIdx.x) * kRowsPerBlock;
+    if (base >= n) return;   // whole block
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
     stage_weights(Wbf, wl, threadIdx.x, kGB);
@@ -139,135 +148,134 @@ __global__ __launch_bounds__(kGB) void k_guide_train_rows(int64_t n_max, const i
     const int k_lo = half * (kIn / 2), k_hi = k_lo + kIn / 2;
     const float mean = n > 0 ? *wsum / float(n) : 0.f;
     float lsum = 0.f;
-    for (int64_t base = int64_t(blockIdx.x) * kRowsPerBlock; base < n; base += int64_t(gridDim.x) * kRowsPerBlock) {
-        const int64_t i0 = base + wave * 32;
-        const int64_t i = i0 + r;
-        // MFMA result element e of half t of this lane is row 8(e/4) + e%4 +
-        // 4 * half, column 32t + r (dense64); its bit in a ReLU mask is 2e + t
-        // (the backward pass produces delta_{l-1} in the layout in which the
-        // forward produced h_{l-1}, so the masks stay in registers)
-        const int64_t ib = i0 + 4 * half;
-        {
-            uint16_t* row = lx + r * kPitchX;
-            if (i < n) {
-                const float* q = rec + i * kRecordFloats;
-                encode_input_half(half, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), box.lo,
-                                  box.inv, row);
-            } else {
-                for (int k = k_lo; k < k_hi; k++) row[k] = 0;
-            }
-        }
-        __syncthreads();
-        {
-            // the feature-major copy for the weight gradients, after both halves of each row are written
-            const uint16_t* row = lx + r * kPitchX;
-            if (i < B)
-                for (int k = k_lo; k < k_hi; k++) xT[int64_t(k) * B + i] = row[k];
-        }
-        uint32_t m1 = 0, m2 = 0, m3 = 0;   // ReLU masks of h1, h2, h3
-        mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int e, int t, uint16_t v) {
-            const int rc = 8 * (e >> 2) + (e & 3);
-            if (ib + rc < B) hT[(int64_t(layer - 1) * kHid + 32 * t + r) * B + ib + rc] = v;
-            const uint32_t bit = bf2f(v) > 0.f ? 1u << (2 * e + t) : 0u;
-            if (layer == 1) m1 |= bit;
-            else if (layer == 2) m2 |= bit;
-            else m3 |= bit;
-        });
-        __syncthreads();
-        // loss gradient of row r, lobes 4 * half .. 4 * half + 3
-        float g[5 * 4];   // [0..3] logits, [4..7] concentrations, [8..19] mean directions
-        for (int k = 0; k < 20; k++) g[k] = 0.f;
-        {
-            float pi[4], f[4], kap[4], raw_k[4], len[4], s = 0.f, wt = 0.f, pp = 0.f;
-            f3 mu[4], rv[4];
-            f3 wi = mk(0.f, 0.f, 1.f);
-            if (i < n) {
-                const float* y = ly + r * kPitchY;
-                const float* q = rec + i * kRecordFloats;
-                wt = mean > 0.f ? fminf(w[i] / mean, wclamp) : 0.f;
-                s = wt / float(n);
-                wi = normalize(mk(q[9], q[10], q[11]));
-                float mx = y[0];
-                for (int k = 1; k < kLobes; k++) mx = fmaxf(mx, y[k]);
-                float se = 0.f;
-                for (int k = 0; k < kLobes; k++) se += expf(y[k] - mx);
-                const float inv = 1.f / se;
-                for (int j = 0; j < 4; j++) {
-                    const int k = 4 * half + j;
-                    pi[j] = expf(y[k] - mx) * inv;
-                    raw_k[j] = y[kLobes + k];
-                    kap[j] = fminf(fmaxf(softplus(raw_k[j]), kKappaMin), kKappaMax);
-                    rv[j] = mk(y[2 * kLobes + 3 * k], y[2 * kLobes + 3 * k + 1], y[2 * kLobes + 3 * k + 2]);
-                    len[j] = length(rv[j]);
-                    mu[j] = len[j] > 1e-8f ? rv[j] / len[j] : mk(0.f, 0.f, 1.f);
-                    f[j] = vmf_pdf(kap[j], mu[j], wi);
-                    pp += pi[j] * f[j];
-                }
-            }
-            const float p = pp + __shfl_xor(pp, 32);
-            if (i < n && s > 0.f && p > 1e-30f) {
-                if (half == 0) lsum += -wt * logf(p);
-                for (int j = 0; j < 4; j++) {
-                    const float rk = pi[j] * f[j] / p;
-                    g[j] = s * (pi[j] - rk);
-                    const float sp = softplus(raw_k[j]);
-                    if (sp > kKappaMin && sp < kKappaMax) {
-                        const float dlogf = 1.f / kap[j] - 2.f / expm1f(2.f * kap[j]) + (dot(mu[j], wi) - 1.f);
-                        const float sig = 1.f / (1.f + expf(-raw_k[j]));
-                        g[4 + j] = -s * rk * dlogf * sig;
-                    }
-                    if (len[j] > 1e-8f) {
-                        const f3 gm = wi * (-s * rk * kap[j]);
-                        const f3 gr = (gm - mu[j] * dot(gm, mu[j])) / len[j];
-                        g[8 + 3 * j] = gr.x;
-                        g[8 + 3 * j + 1] = gr.y;
-                        g[8 + 3 * j + 2] = gr.z;
-                    }
-                }
-            }
-        }
-        __syncthreads();   // every lane has read the raw outputs that the deltas overwrite
-        {
-            // delta of the 64 (padded) outputs as the A operand of the backward pass, and d4T
-            uint16_t* row = lx + r * kPitchX;
-            uint16_t* d4 = dT + 3 * int64_t(kHid) * B;
-            auto put = [&](int col, float v) {
-                const uint16_t b = f2bf(v);
-                row[col] = b;
-                if (i < B) d4[int64_t(col) * B + i] = b;
-            };
-            for (int j = 0; j < 4; j++) {
-                put(4 * half + j, g[j]);
-                put(kLobes + 4 * half + j, g[4 + j]);
-                for (int c = 0; c < 3; c++) put(2 * kLobes + 12 * half + 3 * j + c, g[8 + 3 * j + c]);
-            }
-            for (int c = kOut + 12 * half; c < kOut + 12 * (half + 1); c++) row[c] = 0;
-        }
-        __syncthreads();
-        // backward: delta_{l-1} = (delta_l W_l) * relu'(h_{l-1}), l = 4, 3, 2
-        uint16_t* in = lx;
-        uint16_t* out = lh;
-        int pin = kPitchX, pout = kPitchH;
-#pragma unroll 1
-        for (int l = 4; l >= 2; l--) {
-            const uint32_t m = l == 4 ? m3 : (l == 3 ? m2 : m1);
-            uint16_t* dsave = dT + int64_t(l - 2) * kHid * B;
-            dense64<kHid, kHid, 1>(in, pin, WnBf + (l - 2) * kHid * kHid, zero64, lane,
-                                   [&](int row, int col, float v, int e, int t) {
-                const uint16_t b = f2bf((m >> (2 * e + t)) & 1u ? v : 0.f);
-                out[row * pout + col] = b;
-                const int rc = 8 * (e >> 2) + (e & 3);
-                if (ib + rc < B) dsave[int64_t(32 * t + r) * B + ib + rc] = b;
-            });
-            __syncthreads();
-            uint16_t* t = in;
-            in = out;
-            out = t;
-            const int tp = pin;
-            pin = pout;
-            pout = tp;
+    const int64_t i0 = base + wave * 32;
+    const int64_t i = i0 + r;
+    // MFMA result element e of half t of this lane is row 8(e/4) + e%4 +
+    // 4 * half, column 32t + r (dense64); its bit in a ReLU mask is 2e + t
+    // (the backward pass produces delta_{l-1} in the layout in which the
+    // forward produced h_{l-1}, so the masks stay in registers)
+    const int64_t ib = i0 + 4 * half;
+    {
+        uint16_t* row = lx + r * kPitchX;
+        if (i < n) {
+            const float* q = rec + i * kRecordFloats;
+            encode_input_half(half, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), box.lo,
+                              box.inv, row);
+        } else {
+            for (int k = k_lo; k < k_hi; k++) row[k] = 0;
         }
     }
+    __syncthreads();
+    {
+        // the feature-major copy for the weight gradients, after both halves of each row are written
+        const uint16_t* row = lx + r * kPitchX;
+        if (i < B)
+            for (int k = k_lo; k < k_hi; k++) xT[int64_t(k) * B + i] = row[k];
+    }
+    uint32_t m1 = 0, m2 = 0, m3 = 0;   // ReLU masks of h1, h2, h3
+    mlp_tile_forward(wl, P, lx, lh, ly, lane, [&](int layer, int e, int t, uint16_t v) {
+        const int rc = 8 * (e >> 2) + (e & 3);
+        if (ib + rc < B) hT[(int64_t(layer - 1) * kHid + 32 * t + r) * B + ib + rc] = v;
+        const uint32_t bit = bf2f(v) > 0.f ? 1u << (2 * e + t) : 0u;
+        if (layer == 1) m1 |= bit;
+        else if (layer == 2) m2 |= bit;
+        else m3 |= bit;
+    });
+    __syncthreads();
+    // loss gradient of row r, lobes 4 * half .. 4 * half + 3
+    float g[5 * 4];   // [0..3] logits, [4..7] concentrations, [8..19] mean directions
+    for (int k = 0; k < 20; k++) g[k] = 0.f;
+    {
+        float pi[4], f[4], kap[4], raw_k[4], len[4], s = 0.f, wt = 0.f, pp = 0.f;
+        f3 mu[4], rv[4];
+        f3 wi = mk(0.f, 0.f, 1.f);
+        if (i < n) {
+            const float* y = ly + r * kPitchY;
+            const float* q = rec + i * kRecordFloats;
+            wt = mean > 0.f ? fminf(w[i] / mean, wclamp) : 0.f;
+            s = wt / float(n);
+            wi = normalize(mk(q[9], q[10], q[11]));
+            float mx = y[0];
+            for (int k = 1; k < kLobes; k++) mx = fmaxf(mx, y[k]);
+            float se = 0.f;
+            for (int k = 0; k < kLobes; k++) se += expf(y[k] - mx);
+            const float inv = 1.f / se;
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * half + j;
+                pi[j] = expf(y[k] - mx) * inv;
+                raw_k[j] = y[kLobes + k];
+                kap[j] = fminf(fmaxf(softplus(raw_k[j]), kKappaMin), kKappaMax);
+                rv[j] = mk(y[2 * kLobes + 3 * k], y[2 * kLobes + 3 * k + 1], y[2 * kLobes + 3 * k + 2]);
+                len[j] = length(rv[j]);
+                mu[j] = len[j] > 1e-8f ? rv[j] / len[j] : mk(0.f, 0.f, 1.f);
+                f[j] = vmf_pdf(kap[j], mu[j], wi);
+                pp += pi[j] * f[j];
+            }
+        }
+        const float p = pp + __shfl_xor(pp, 32);
+        if (i < n && s > 0.f && p > 1e-30f) {
+            if (half == 0) lsum += -wt * logf(p);
+            for (int j = 0; j < 4; j++) {
+                const float rk = pi[j] * f[j] / p;
+                g[j] = s * (pi[j] - rk);
+                const float sp = softplus(raw_k[j]);
+                if (sp > kKappaMin && sp < kKappaMax) {
+                    const float dlogf = 1.f / kap[j] - 2.f / expm1f(2.f * kap[j]) + (dot(mu[j], wi) - 1.f);
+                    const float sig = 1.f / (1.f + expf(-raw_k[j]));
+                    g[4 + j] = -s * rk * dlogf * sig;
+                }
+                if (len[j] > 1e-8f) {
+                    const f3 gm = wi * (-s * rk * kap[j]);
+                    const f3 gr = (gm - mu[j] * dot(gm, mu[j])) / len[j];
+                    g[8 + 3 * j] = gr.x;
+                    g[8 + 3 * j + 1] = gr.y;
+                    g[8 + 3 * j + 2] = gr.z;
+                }
+            }
+        }
+    }
+    __syncthreads();   // every lane has read the raw outputs that the deltas overwrite
+    {
+        // delta of the 64 (padded) outputs as the A operand of the backward pass, and d4T
+        uint16_t* row = lx + r * kPitchX;
+        uint16_t* d4 = dT + 3 * int64_t(kHid) * B;
+        auto put = [&](int col, float v) {
+            const uint16_t b = f2bf(v);
+            row[col] = b;
+            if (i < B) d4[int64_t(col) * B + i] = b;
+        };
+        for (int j = 0; j < 4; j++) {
+            put(4 * half + j, g[j]);
+            put(kLobes + 4 * half + j, g[4 + j]);
+            for (int c = 0; c < 3; c++) put(2 * kLobes + 12 * half + 3 * j + c, g[8 + 3 * j + c]);
+        }
+        for (int c = kOut + 12 * half; c < kOut + 12 * (half + 1); c++) row[c] = 0;
+    }
+    __syncthreads();
+    // backward: delta_{l-1} = (delta_l W_l) * relu'(h_{l-1}), l = 4, 3, 2
+    uint16_t* in = lx;
+    uint16_t* out = lh;
+    int pin = kPitchX, pout = kPitchH;
+#pragma unroll 1
+    for (int l = 4; l >= 2; l--) {
+        const uint32_t m = l == 4 ? m3 : (l == 3 ? m2 : m1);
+        uint16_t* dsave = dT + int64_t(l - 2) * kHid * B;
+        dense64<kHid, kHid, 1>(in, pin, WnBf + (l - 2) * kHid * kHid, zero64, lane,
+                               [&](int row, int col, float v, int e, int t) {
+            const uint16_t b = f2bf((m >> (2 * e + t)) & 1u ? v : 0.f);
+            out[row * pout + col] = b;
+            const int rc = 8 * (e >> 2) + (e & 3);
+            if (ib + rc < B) dsave[int64_t(32 * t + r) * B + ib + rc] = b;
+        });
+        __syncthreads();
+        uint16_t* t = in;
+        in = out;
+        out = t;
+        const int tp = pin;
+        pin = pout;
+        pout = tp;
+    }
+
     for (int sh = 32; sh > 0; sh >>= 1) lsum += __shfl_xor(lsum, sh);
     if (lane == 0) red[wave] = lsum;
     __syncthreads();
@@ -523,9 +531,9 @@ int Guide::enqueue_train(const float* rec, int64_t n, int64_t B, const int32_t* 
                        scal.p + 1);
     Box b;
     for (int a = 0; a < 3; a++) { b.lo[a] = box_lo[a]; b.inv[a] = box_inv[a]; }
-    hipLaunchKernelGGL(k_guide_train_rows, dim3(grid_for(B)), dim3(kGB), 0, s, n, ndev, off, B, rec, Wbf.p, P.p,
+    hipLaunchKernelGGL(k_guide_train_rows, dim3(unsigned(B / kRowsPerBlock)), dim3(kGB), 0, s, n, ndev, off, B, rec, Wbf.p, P.p,
                        WnBf.p, zero64.p, b, w.p, scal.p, clamp > 0.f ? clamp : wclamp, xT.p, hT.p, dT.p, scal.p + 1);
-    const int slice = 4096;
+    const int slice = LMGPU_DW_SLICE;
     hipLaunchKernelGGL(k_guide_dw, dim3(nb(B, slice), 18), dim3(64), 0, s, n, ndev, off, B, slice, xT.p, hT.p, dT.p,
                        G.p, stepc.p);
     if (grad_host) LMGPU_HIP(hipMemcpyAsync(grad_host, G.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, s));
