@@ -339,7 +339,9 @@ int lmgpu_lt_render(lmgpu_scene* scene, const lmgpu_lt_params* params, double* f
  * weighted by the power heuristic over all strategies of its full path
  * (Path::eval_mis_weight) and splatted at its raster position. Same film and
  * scheduler contract as lmgpu_lt_render (film value = film * (w*h) /
- * stats->samples; stats->spp counts the passes, of at most 2^18 samples).
+ * stats->samples; stats->spp counts the passes, of min(w*h, 2^k) samples with
+ * 2^k the largest power of two in [2^18, 2^21] for which 2^k (max_verts - 1)
+ * stored light vertices of 96 B fit in 2 GiB).
  * An environment light is a non-connectable light endpoint (bidir.h:245-258,
  * 399-411, 585-588): light subpaths may start on it and eye subpaths may end
  * on it (strategy s = 0), but no connection reaches it.

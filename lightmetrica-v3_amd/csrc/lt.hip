@@ -465,6 +465,15 @@ constexpr int kFlagSpec = 1 << 8;        // this vertex's component is specular 
 constexpr int kFlagSpecPrev = 1 << 9;    // the previous vertex's component is specular (or it cannot be connected)
 constexpr int kFlagInfPrev = 1 << 10;    // path state: the previous vertex is the environment, at infinity
 
+// Samples per BDPT pass for max_verts - 1 = lstride stored light vertices
+// per path: 2^k, k in [18, 21], the largest with 2^k * lstride * 96 B <= 2 GiB.
+inline int64_t bdpt_pass_size(int lstride) {
+    const int64_t cap = (int64_t(1) << 31) / (int64_t(lstride) * 96);
+    int k = 21;
+    while (k > 18 && (int64_t(1) << k) > cap) k--;
+    return int64_t(1) << k;
+}
+
 struct BDConsts {
     int max_verts;
     int min_verts;    // strategies with fewer vertices are skipped (Renderer_Path_Base, renderer_bdpt.cpp:49)
@@ -1105,10 +1114,11 @@ int Scene::bdpt_render(const lmgpu_lt_params* p, double* film_in, bool on_device
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
     LMGPU_TRY(ensure_workspace(npix));
-    // samples per pass: bounded so that the stored light vertices and the
-    // connection queue stay modest (P (max_verts - 1) entries each)
-    const int P = std::min(npix, 1 << 18);
+    // samples per pass: the largest power of two in [2^18, 2^21] whose stored
+    // light vertices (P (max_verts - 1) LVerts of 96 B) fit in 2 GiB, at most
+    // w*h; the connection queue has as many entries
     const int lstride = p->max_verts - 1;
+    const int P = int(std::min<int64_t>(npix, bdpt_pass_size(lstride)));
     const size_t nv = size_t(P) * lstride;
     if (blverts.n < nv * sizeof(LVert) / sizeof(float4)) LMGPU_TRY(blverts.alloc(nv * sizeof(LVert) / sizeof(float4)));
     if (bsh.n < nv) {

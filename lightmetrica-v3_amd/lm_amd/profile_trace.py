@@ -82,7 +82,7 @@ def main():
         f0.camera.aspect = 1920 / 1080
         sb = GpuScene(f0, width=args.width, leaf_size=args.leaf)
         _, st = sb.render_bdpt(num_samples=2 << 18, max_verts=8, seed=1)
-        print('2 x 2^18-sample config-0 BDPT passes at 1080p', st, flush=True)
+        print('2^19 samples of config-0 BDPT at 1080p', st, flush=True)
     if args.c3:
         from lm_amd.batch import Batch
         jobs = scenes.config3_jobs()[:args.c3]

@@ -257,7 +257,8 @@ class GpuScene:
     def render_bdpt(self, num_samples: int = 0, max_verts: int = 8, seed: int = 0, pass_offset: int = 0,
                     render_time: float = 0.0, min_verts: int = 2, film=None, stream=None):
         """lmgpu_bdpt_render: splat sums of `num_samples` bidirectional
-        samples (passes of min(w*h, 2^18)); same film contract as render_lt."""
+        samples (passes of min(w*h, 2^k) samples, 2^k from 2^21 down to 2^18 as max_verts
+        grows); same film contract as render_lt."""
         p = _lib.LTParams()
         p.num_samples = int(num_samples)
         p.max_verts = int(max_verts)

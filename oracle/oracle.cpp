@@ -1675,8 +1675,12 @@ int orc_bdpt_render(const OrcScene* scene, void* accel, const OrcLTParams* P, do
     const int env_index = int(S.lights.size());
     const int M = P->max_verts;
     const int min_verts = P->min_verts > 0 ? P->min_verts : 2;
-    // samples per pass, as the GPU renderer: min(w*h, 2^18)
-    const int64_t per_pass = std::min<int64_t>(npix, int64_t(1) << 18);
+    // samples per pass, as the GPU renderer: min(w*h, 2^k), k in [18, 21] the
+    // largest with 2^k (max_verts - 1) stored light vertices of 96 B in 2 GiB
+    const int64_t vcap = (int64_t(1) << 31) / (int64_t(M - 1) * 96);
+    int pk = 21;
+    while (pk > 18 && (int64_t(1) << pk) > vcap) pk--;
+    const int64_t per_pass = std::min<int64_t>(npix, int64_t(1) << pk);
     if (P->nthreads > 0) omp_set_num_threads(P->nthreads);
     int64_t n_ext = 0, n_shadow = 0, n_vert = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : n_ext, n_shadow, n_vert)

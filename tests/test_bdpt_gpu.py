@@ -62,6 +62,26 @@ def test_bdpt_same_stream_matches_oracle(device, name):
     assert frac >= 0.97 and rel < 2e-3
 
 
+def test_bdpt_same_stream_pass_size(device):
+    """Passes of 2^20 samples (max_verts 12: 2^21 paths would store more
+    than 2 GiB of light vertices) on a 1200x900 image: sample i of a render
+    is path i mod 2^20 of pass i div 2^20 on the GPU and in the oracle alike,
+    so a render of 2^20 + 4096 samples stays on the same streams."""
+    flat = scenes.config0().flatten()
+    flat.w, flat.h = 1200, 900
+    flat.camera.aspect = 1200 / 900
+    n, M, seed = (1 << 20) + 4096, 12, 77
+    s, st = GpuScene(flat).render_bdpt(num_samples=n, max_verts=M, seed=seed)
+    assert st['passes'] == 2
+    ref = orc.bdpt_render(flat, n, M, seed=seed)
+    b = ref['sum']
+    close = np.abs(s - b) <= 1e-3 * (1.0 + np.abs(b))
+    frac = close.all(-1).mean()
+    rel = abs(s.sum() - b.sum()) / b.sum()
+    print(f'BDPT 1200x900 M=12 same-stream: pixels within 1e-3 {frac:.4f}, total rel diff {rel:.2e}; gpu {st}')
+    assert frac >= 0.97 and rel < 2e-3
+
+
 def test_bdpt_furnace_closed_form(device):
     rho, Le, M = 0.5, 1.0, 8
     flat = furnace_scene(rho, Le).flatten()

@@ -91,6 +91,36 @@ def test_gradient_matches_autograd(device):
             assert cos > 0.999 and rel < 3e-2, (li, name, cos, rel)
 
 
+def test_gradient_ragged_batch_after_a_step(device):
+    """A batch that ends inside a 128-row tile, right after another step:
+    the gradient is that batch's alone (the buffer is cleared between steps)
+    and matches autograd; the step count advances once per step."""
+    import torch
+    rng = np.random.default_rng(4)
+    g = G.Guide(BOX_LO, BOX_HI, seed=11)
+    p0 = g.get_params()
+    g.train(torch.from_numpy(random_records(2048, rng)).to(device), lr=0.0)
+    rec = random_records(1000, rng)
+    loss, grad = g.train(torch.from_numpy(rec).to(device), lr=0.0, want_grad=True)
+    assert g.stats()[0] == 2
+    assert np.array_equal(g.get_params(), p0)
+    layers = ref.unpack(p0)
+    for W, b in layers:
+        W.requires_grad_(True)
+        b.requires_grad_(True)
+    lt = ref.loss_fn(layers, torch.from_numpy(rec), BOX_LO, BOX_HI)
+    lt.backward()
+    assert abs(loss - lt.item()) < 1e-3 * max(1.0, abs(lt.item()))
+    got = G.unpack_params(grad)
+    for li, ((Wg, bg), (W, b)) in enumerate(zip(got, layers)):
+        for a, e in ((Wg, W.grad.numpy()), (bg, b.grad.numpy())):
+            a = np.asarray(a, dtype=np.float64).ravel()
+            e = np.asarray(e, dtype=np.float64).ravel()
+            cos = a @ e / (np.linalg.norm(a) * np.linalg.norm(e) + 1e-30)
+            rel = np.linalg.norm(a - e) / (np.linalg.norm(e) + 1e-30)
+            assert cos > 0.999 and rel < 3e-2, (li, cos, rel)
+
+
 def test_adam_step_matches_torch(device):
     import torch
     rng = np.random.default_rng(2)
