@@ -837,10 +837,11 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (image && gp) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering uses the pixel sampling mode");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
-    // fixed-spp pixel-mode renders of 2 or more samples without a guide go
-    // through the regenerating wavefront; 1-spp calls, the time scheduler,
-    // the image mode and guided rendering replay the one-sample pass graph
-    const bool regen = !image && !gp && p->render_time <= 0 && p->spp >= 2;
+    // pixel-mode renders without a guide go through the regenerating
+    // wavefront: fixed-spp renders of 2 or more samples, and the time
+    // scheduler in chunks of whole samples per pixel; 1-spp calls, the image
+    // mode and guided rendering replay the one-sample pass graph
+    const bool regen = !image && !gp && (p->render_time > 0 || p->spp >= 2);
     if (!regen) LMGPU_TRY(ensure_workspace(npix));
     GuidedPass gpv;
     if (gp) {
@@ -874,7 +875,33 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     c.image = image ? 1 : 0;
     if (regen) {
         int64_t ext = 0, shd = 0;
-        LMGPU_TRY(render_regen(c, p->spp, p->sample_offset, sum, sumsq, &ext, &shd, s));
+        int64_t done = 0;
+        if (p->render_time > 0) {
+            // scheduler::spp::time: chunks of k samples per pixel until the
+            // budget is spent; k is what the time left allows at the measured
+            // time per sample (2 to start, at most 256), so the render ends
+            // within about two samples' time of the budget, as the one-sample
+            // passes end within one
+            double t_spp = 0.0;
+            while (true) {
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (done > 0 && el > p->render_time) break;
+                int k = 2;
+                if (t_spp > 0.0) k = int(std::min(256.0, std::max(2.0, std::floor((p->render_time - el) / t_spp))));
+                if (int64_t(p->sample_offset) + done + k > int64_t(INT32_MAX))
+                    return fail(LMGPU_ERR_INVALID_ARGUMENT, "sample index beyond 2^31");
+                int64_t e1 = 0, s1 = 0;
+                LMGPU_TRY(render_regen(c, k, int(p->sample_offset + done), sum, sumsq, &e1, &s1, s));
+                ext += e1;
+                shd += s1;
+                done += k;
+                const double el2 = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                t_spp = (el2 - el) / k;
+            }
+        } else {
+            LMGPU_TRY(render_regen(c, p->spp, p->sample_offset, sum, sumsq, &ext, &shd, s));
+            done = p->spp;
+        }
         if (!on_device) {
             LMGPU_HIP(hipMemcpyAsync(sum_in, sum, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
             if (sumsq_in) LMGPU_HIP(hipMemcpyAsync(sumsq_in, sumsq, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
@@ -882,8 +909,8 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
         LMGPU_HIP(hipStreamSynchronize(s));
         if (st) {
             std::memset(st, 0, sizeof(*st));
-            st->samples = int64_t(npix) * p->spp;
-            st->spp = p->spp;
+            st->samples = int64_t(npix) * done;
+            st->spp = done;
             st->extension_rays = ext;
             st->shadow_rays = shd;
             st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -926,7 +953,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     }
     int64_t passes = 0;
     if (p->render_time > 0) {
-        // scheduler::spp::time (spi::time in image mode): passes of w*h samples until the budget is spent
+        // scheduler::spi::time (image mode; guided rendering): passes of w*h samples until the budget is spent
         while (true) {
             LMGPU_HIP(hipGraphLaunch(graph, s));
             passes++;

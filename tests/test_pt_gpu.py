@@ -184,6 +184,12 @@ def test_time_scheduler_and_device_film(device):
     r = RendererPT(sc, film, max_verts=8, seed=1, scheduler='time', render_time=0.2)
     out = r.render()
     assert out['processed'] >= 1 and out['elapsed'] >= 0.2
+    # the time scheduler renders chunks of whole samples per pixel on consecutive
+    # sample indices: the film equals a fixed-spp render of as many samples
+    t, tq, st = sc.render_raw(max_verts=8, seed=4, render_time=0.3)
+    assert st['spp'] >= 2 and st['samples'] == st['spp'] * flat.w * flat.h and st['elapsed'] >= 0.3
+    f, fq, _ = sc.render_raw(spp=st['spp'], max_verts=8, seed=4)
+    assert np.allclose(t, f, rtol=1e-9, atol=1e-12) and np.allclose(tq, fq, rtol=1e-9, atol=1e-12)
 
 
 def test_invalid_render_args(device):
