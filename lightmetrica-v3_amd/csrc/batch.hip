@@ -32,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "guide_internal.h"
 #include "lmgpu_internal.h"
 #include "pt_device.h"
 #include "pt_internal.h"
@@ -230,13 +231,13 @@ __global__ __launch_bounds__(kBlock) void k_b_extend(const typename TravNode<W>:
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->ext_rays, (unsigned long long)n);
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jobs, PoolDev P, BatchQ q, int in_q) {
+template <int BLOCK, bool GUIDED>
+__global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jobs, PoolDev P, BatchQ q, int in_q,
+                                                   GuideDev gd, int cap) {
     const int n = q.counts[in_q];
     const int out_q = in_q ^ 1;
     const int stride = gridDim.x * BLOCK;
     const int n_iter = (n + stride - 1) / stride;
-    const GuideDev nog{};
     for (int it = 0; it < n_iter; it++) {
         const int i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
         VertexOut out;
@@ -256,9 +257,11 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
             c.rr_depth = J.rr_depth;
             c.seed_lo = J.seed_lo;
             c.seed_hi = J.seed_hi;
-            c.npix = J.npix;
-            shade_vertex<false>(J.sc, c, uint32_t(P.pixel[s]), uint32_t(P.sample[s]), job_seed(J), s,
-                                mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), h, R, nog, nullptr, out);
+            // guided: the records of slot s sit at (vertex - 1) * cap + s
+            c.npix = GUIDED ? cap : J.npix;
+            shade_vertex<GUIDED>(J.sc, c, uint32_t(P.pixel[s]), uint32_t(P.sample[s]), job_seed(J), s,
+                                 mk(ra.x, ra.y, ra.z), mk(rb.x, rb.y, rb.z), h, R, gd,
+                                 GUIDED ? gd.mix + int64_t(i) * kOut : nullptr, out);
             if (out.ext) {
                 P.prev[s] = R.prev;
                 P.info[s] = R.info;
@@ -278,15 +281,93 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
         if (out.sh) {
             q.sh[ss] = out.sh_ray;
             q.sh_contrib[ss] = make_float4(out.sh_C.x, out.sh_C.y, out.sh_C.z, __int_as_float(s));
+            if (GUIDED) gd.sh_rec[ss] = out.sh_slot;
         }
     }
 }
 
-template <int W>
+// Training records of the paths that finished in the last iteration (slots
+// in kFinishing, before k_b_regen frees them): record d of slot s becomes
+// (pos, nrm, wo, wi, p_mix, (L_final - e) f) when its target weight is
+// positive, appended to train[tcap][16] while there is room, and every
+// record of the slot is cleared for the slot's next path. Slots are visited
+// in the order j * kScramble mod cap (an odd multiplier: a bijection on the
+// power-of-two pool), so that a buffer that fills up holds paths from all
+// over the film. The finished paths are counted in *nfin.
+__global__ __launch_bounds__(256) void k_b_gcollect(PoolDev P, int cap, GRec* __restrict__ rec, int rec_depth,
+                                                    float* __restrict__ train, int tcap, int32_t* __restrict__ tcount,
+                                                    int32_t* __restrict__ nfin,
+                                                    unsigned long long* __restrict__ recorded) {
+    constexpr uint32_t kScramble = 2654435761u;
+    const int stride = gridDim.x * 256;
+    const int n_iter = (cap + stride - 1) / stride;
+    for (int it = 0; it < n_iter; it++) {
+        const int j = it * stride + blockIdx.x * 256 + threadIdx.x;
+        const int s = int((uint32_t(j) * kScramble) & uint32_t(cap - 1));
+        const bool fin = j < cap && P.state[s] == kFinishing;
+        const unsigned long long fmask = __ballot(fin);
+        if (fmask && __lane_id() == __ffsll(fmask) - 1) atomicAdd(nfin, __popcll(fmask));
+        const float4 Lf = fin ? P.L[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int d = 0; d < rec_depth; d++) {
+            bool keep = false;
+            float4 o0, o1, o2, o3;
+            if (fin) {
+                GRec* rp = rec + int64_t(d) * cap + s;
+                const float4 a = rp->a;
+                if (a.w > 0.f) {
+                    const GRec r = *rp;
+                    const float lx = fmaxf(0.f, Lf.x - r.e.x) * r.f.x, ly = fmaxf(0.f, Lf.y - r.e.y) * r.f.y,
+                                lz = fmaxf(0.f, Lf.z - r.e.z) * r.f.z;
+                    const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / r.a.w;
+                    if (wgt > 0.f && wgt < 3e38f) {
+                        keep = true;
+                        o0 = make_float4(r.a.x, r.a.y, r.a.z, r.b.x);
+                        o1 = make_float4(r.b.y, r.b.z, r.c.x, r.c.y);
+                        o2 = make_float4(r.c.z, r.d.x, r.d.y, r.d.z);
+                        o3 = make_float4(r.a.w, lx, ly, lz);
+                    }
+                    rp->a = make_float4(a.x, a.y, a.z, 0.f);
+                }
+            }
+            const unsigned long long mask = __ballot(keep);
+            if (mask && __lane_id() == __ffsll(mask) - 1) atomicAdd(recorded, (unsigned long long)__popcll(mask));
+            const int k = wave_append(tcount, keep);
+            if (keep && k < tcap) {
+                float4* o = reinterpret_cast<float4*>(train + int64_t(k) * kRecordFloats);
+                o[0] = o0;
+                o[1] = o1;
+                o[2] = o2;
+                o[3] = o3;
+            }
+        }
+    }
+}
+
+// Once a sample per pixel's worth of paths (npix) has finished since the
+// last release, releases the buffered records to the training steps that
+// follow (*tready = min(count, tcap): `steps` batches, the last ones partial
+// or empty, as after a one-sample pass) and restarts the buffer; the
+// finished-path count keeps its excess (at most another npix), so releases
+// keep pace with the samples. Otherwise the steps get no rows (*tready = 0)
+// and return at entry.
+__global__ void k_b_ggate(int32_t* tcount, int32_t* tready, int32_t* nfin, int npix, int tcap) {
+    if (threadIdx.x == 0) {
+        if (*nfin >= npix) {
+            *tready = min(*tcount, tcap);
+            *tcount = 0;
+            *nfin = min(*nfin - npix, npix);
+        } else {
+            *tready = 0;
+        }
+    }
+}
+
+template <int W, bool GUIDED>
 __global__ __launch_bounds__(kBlock) void k_b_connect(const typename TravNode<W>::type* __restrict__ nodes,
                                                       const TriV* __restrict__ tris,
                                                       const JobDev* __restrict__ jobs, PoolDev P, BatchQ q,
-                                                      unsigned long long* next, int refill, BatchCtl* ctl) {
+                                                      unsigned long long* next, int refill, BatchCtl* ctl,
+                                                      GRec* __restrict__ rec, const int32_t* __restrict__ sh_rec) {
     __shared__ int lds[kLds][kBlock];
     const int n = q.counts[2];
     if (n == 0) return;   // no shadow rays this iteration (whole grid)
@@ -308,6 +389,17 @@ __global__ __launch_bounds__(kBlock) void k_b_connect(const typename TravNode<W>
             L.y += c.y;
             L.z += c.z;
             P.L[s] = L;
+            if (GUIDED) {
+                // the vertex's NEE belongs to its record's radiance at sampling time
+                const int slot = sh_rec[id];
+                if (slot >= 0) {
+                    float4 e = rec[slot].e;
+                    e.x += c.x;
+                    e.y += c.y;
+                    e.z += c.z;
+                    rec[slot].e = e;
+                }
+            }
         }
     };
     trace_persistent<W, true, false, kLds, kPriv, kBlock>(nodes, tris, n, next, lds, fetch, emit, q.counts + 5, nullptr,
@@ -588,13 +680,52 @@ int Wavefront::set_work(const std::vector<JobDev>& jobs, const std::vector<WorkS
     return LMGPU_OK;
 }
 
+int Wavefront::ensure_guided(int rec_depth, int64_t train_cap) {
+    const size_t nrec = size_t(std::max(rec_depth, 1)) * size_t(cap);
+    if (grec.n < nrec) {
+        LMGPU_TRY(grec.alloc(nrec));
+        LMGPU_HIP(hipMemset(grec.p, 0, sizeof(GRec) * nrec));
+    }
+    if (gmix.n < size_t(cap) * kOut) LMGPU_TRY(gmix.alloc(size_t(cap) * kOut));
+    if (gshrec.n < size_t(cap)) LMGPU_TRY(gshrec.alloc(cap));
+    if (gtrain.n < size_t(std::max<int64_t>(train_cap, 1)) * kRecordFloats)
+        LMGPU_TRY(gtrain.alloc(size_t(std::max<int64_t>(train_cap, 1)) * kRecordFloats));
+    if (!gcnt.p) LMGPU_TRY(gcnt.alloc(3));
+    if (!grecorded.p) LMGPU_TRY(grecorded.alloc(1));
+    return LMGPU_OK;
+}
+
 int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
-                   double* film_sum, double* fsq, double* elapsed, int64_t* iters, hipStream_t s) {
+                   double* film_sum, double* fsq, double* elapsed, int64_t* iters, hipStream_t s,
+                   const RegenGuide* G) {
     const int nsegs = int(dsegs.n);
     PoolDev P{pjob.p, ppix.p, psample.p, pstate.p, pthr.p, pL.p, pprev.p, pinfo.p};
     BatchQ q{{ext0.p, ext1.p}, {path0.p, path1.p}, hit.p, sh.p, shc.p, counts.p};
     const int refill = 32;
+    GuideDev gd{};
+    const bool train = G && G->steps > 0 && G->rec_depth > 0;
+    const int tcap = G ? G->batch * G->steps : 0;
+    if (G) {
+        gd.mix = gmix.p;
+        gd.rec = G->rec_depth > 0 ? grec.p : nullptr;
+        gd.sh_rec = gshrec.p;
+        gd.alpha = G->alpha;
+        gd.rec_depth = G->rec_depth;
+        hipLaunchKernelGGL(k_b_fill_i32, dim3(1), dim3(64), 0, s, gcnt.p, int64_t(3), 0);
+        hipLaunchKernelGGL(k_b_fill_u64, dim3(1), dim3(64), 0, s, grecorded.p, int64_t(1));
+    }
     auto iteration = [&](hipStream_t cs, int in_q) {
+        if (train) {
+            // online training: the records of the paths that finished in the
+            // last iteration; after every sample per pixel's worth of finished
+            // paths, `steps` Adam steps on the buffered records
+            hipLaunchKernelGGL(k_b_gcollect, dim3(unsigned(std::min<int64_t>(nblk(cap, 256), 4096))), dim3(256), 0, cs, P,
+                               cap, grec.p, G->rec_depth, gtrain.p, tcap, gcnt.p, gcnt.p + 2, grecorded.p);
+            hipLaunchKernelGGL(k_b_ggate, dim3(1), dim3(64), 0, cs, gcnt.p, gcnt.p + 1, gcnt.p + 2, G->npix, tcap);
+            for (int t = 0; t < G->steps; t++)
+                (void)G->g->enqueue_train(gtrain.p + int64_t(t) * G->batch * kRecordFloats, G->batch, G->batch,
+                                          gcnt.p + 1, int64_t(t) * G->batch, G->lr, nullptr, cs, G->wclamp);
+        }
         // at most kRegenMaxIter strides of the pool per block
         const unsigned rg = unsigned(std::max<int64_t>(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 4096)),
                                                        nblk(cap, block * kRegenMaxIter)));
@@ -615,18 +746,46 @@ int Wavefront::run(const void* trav_nodes, const TriV* tris, int width, int bloc
             hipLaunchKernelGGL(k_b_extend<8>, dim3(occupancy_grid(k_b_extend<8>, kBlock)), dim3(kBlock), 0, cs,
                                static_cast<const TravNode<8>::type*>(trav_nodes), tris, djobs.p, P, q, in_q, next.p, refill, ctl.p);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, cs, counts.p, in_q ^ 1);
+        if (G)
+            hipLaunchKernelGGL(k_pt_guide_infer<BatchQ>, dim3(unsigned(std::max(1, std::min(cap / 128, 512)))), dim3(256),
+                               0, cs, G->sc, q, in_q, G->g->Wbf.p, G->g->P.p, G->box, gmix.p);
         const unsigned sg = unsigned(std::max<int64_t>(1, std::min<int64_t>(nblk(cap, block), 2048 * 256 / block)));
-        if (block == 128) hipLaunchKernelGGL(k_b_shade<128>, dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q);
-        else hipLaunchKernelGGL(k_b_shade<256>, dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q);
-        if (width == 2)
-            hipLaunchKernelGGL(k_b_connect<2>, dim3(occupancy_grid(k_b_connect<2>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<2>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
-        else if (width == 4)
-            hipLaunchKernelGGL(k_b_connect<4>, dim3(occupancy_grid(k_b_connect<4>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<4>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
-        else
-            hipLaunchKernelGGL(k_b_connect<8>, dim3(occupancy_grid(k_b_connect<8>, kBlock)), dim3(kBlock), 0, cs,
-                               static_cast<const TravNode<8>::type*>(trav_nodes), tris, djobs.p, P, q, next.p + 1, refill, ctl.p);
+        if (G) {
+            if (block == 128)
+                hipLaunchKernelGGL((k_b_shade<128, true>), dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q, gd, cap);
+            else
+                hipLaunchKernelGGL((k_b_shade<256, true>), dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q, gd, cap);
+        } else {
+            if (block == 128)
+                hipLaunchKernelGGL((k_b_shade<128, false>), dim3(sg), dim3(128), 0, cs, djobs.p, P, q, in_q, gd, cap);
+            else
+                hipLaunchKernelGGL((k_b_shade<256, false>), dim3(sg), dim3(256), 0, cs, djobs.p, P, q, in_q, gd, cap);
+        }
+        if (width == 2) {
+            const auto* nd = static_cast<const TravNode<2>::type*>(trav_nodes);
+            if (G)
+                hipLaunchKernelGGL((k_b_connect<2, true>), dim3(occupancy_grid(k_b_connect<2, true>, kBlock)), dim3(kBlock), 0,
+                                   cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+            else
+                hipLaunchKernelGGL((k_b_connect<2, false>), dim3(occupancy_grid(k_b_connect<2, false>, kBlock)), dim3(kBlock),
+                                   0, cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+        } else if (width == 4) {
+            const auto* nd = static_cast<const TravNode<4>::type*>(trav_nodes);
+            if (G)
+                hipLaunchKernelGGL((k_b_connect<4, true>), dim3(occupancy_grid(k_b_connect<4, true>, kBlock)), dim3(kBlock), 0,
+                                   cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+            else
+                hipLaunchKernelGGL((k_b_connect<4, false>), dim3(occupancy_grid(k_b_connect<4, false>, kBlock)), dim3(kBlock),
+                                   0, cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+        } else {
+            const auto* nd = static_cast<const TravNode<8>::type*>(trav_nodes);
+            if (G)
+                hipLaunchKernelGGL((k_b_connect<8, true>), dim3(occupancy_grid(k_b_connect<8, true>, kBlock)), dim3(kBlock), 0,
+                                   cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+            else
+                hipLaunchKernelGGL((k_b_connect<8, false>), dim3(occupancy_grid(k_b_connect<8, false>, kBlock)), dim3(kBlock),
+                                   0, cs, nd, tris, djobs.p, P, q, next.p + 1, refill, ctl.p, gd.rec, gd.sh_rec);
+        }
     };
     return loop(iteration, total, max_verts, elapsed, iters, s);
 }
@@ -798,7 +957,7 @@ int Batch::run_once(int width, int block, int spp_cap, bool keep_results, bool w
 void wavefront_destroy(Wavefront* w) { delete w; }
 
 int Scene::render_regen(const PTConsts& c, int spp, int sample_offset, double* sum, double* sumsq, int64_t* ext_rays,
-                        int64_t* shadow_rays, hipStream_t s) {
+                        int64_t* shadow_rays, hipStream_t s, const GuidedPass* gp, int64_t* guide_records) {
     if (!wf) {
         wf = new (std::nothrow) Wavefront();
         if (!wf) return fail(LMGPU_ERR_OUT_OF_MEMORY, "wavefront");
@@ -819,18 +978,44 @@ int Scene::render_regen(const PTConsts& c, int spp, int sample_offset, double* s
     for (int k = 0; k < spp; k++) segs[size_t(k)] = WorkSeg{int64_t(k) * c.npix, 0, sample_offset + k};
     // pool: LMGPU_REGEN_SLOTS_PER_PIXEL paths per pixel (the slots of a
     // sample's short paths take the next sample's pixels)
-    const int cap = Wavefront::pool_size(total, c.npix);
+    const int cap = gp ? Wavefront::pool_size(total, c.npix, LMGPU_REGEN_GUIDED_SLOTS_PER_PIXEL,
+                                              LMGPU_REGEN_GUIDED_MAX_SLOTS)
+                       : Wavefront::pool_size(total, c.npix);
     LMGPU_TRY(wf->ensure_pool(cap));
+    RegenGuide G;
+    if (gp) {
+        G.g = gp->g;
+        G.sc = dev;
+        for (int a = 0; a < 3; a++) {
+            G.box.lo[a] = gp->g->box_lo[a];
+            G.box.inv[a] = gp->g->box_inv[a];
+        }
+        G.alpha = gp->alpha;
+        G.lr = gp->lr;
+        G.wclamp = gp->wclamp;
+        G.batch = gp->batch;
+        G.steps = gp->steps;
+        G.rec_depth = gp->rec_depth;
+        G.npix = c.npix;
+        if (G.steps > 0) LMGPU_TRY(gp->g->ensure_train(G.batch));
+        LMGPU_TRY(wf->ensure_guided(G.rec_depth, int64_t(G.batch) * G.steps));
+    }
     LMGPU_TRY(wf->set_work(std::vector<JobDev>{J}, segs, s));
     const void* tn = accel.width == 4 ? static_cast<const void*>(dev.nodes4)
                      : accel.width == 8 ? static_cast<const void*>(dev.nodes8) : static_cast<const void*>(dev.nodes2);
     double sec = 0;
     int64_t iters = 0;
-    LMGPU_TRY(wf->run(tn, accel.tris.p, accel.width, 128, total, c.max_verts, sum, sumsq, &sec, &iters, s));
+    LMGPU_TRY(wf->run(tn, accel.tris.p, accel.width, 128, total, c.max_verts, sum, sumsq, &sec, &iters, s,
+                      gp ? &G : nullptr));
     BatchCtl hc;
     LMGPU_HIP(hipMemcpy(&hc, wf->ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
     *ext_rays = int64_t(hc.ext_rays);
     *shadow_rays = int64_t(hc.shadow_rays);
+    if (guide_records) {
+        unsigned long long r = 0;
+        if (gp) LMGPU_HIP(hipMemcpy(&r, wf->grecorded.p, sizeof(r), hipMemcpyDeviceToHost));
+        *guide_records = int64_t(r);
+    }
     return LMGPU_OK;
 }
 

@@ -260,53 +260,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathState ps, Que
     }
 }
 
-// Guide inference over the extension queue of a bounce: the query of entry
-// i is its hit point, shading normal and the direction back along the ray;
-// the decoded mixture goes to mix[i]. Entries without a hit get a zero query.
-__global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Queues q, int in_q, const uint16_t* __restrict__ Wbf,
-                                                        const float* __restrict__ P, GBox box, float* __restrict__ mix) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
-    stage_weights(Wbf, wl, threadIdx.x, 256);
-    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
-    uint16_t* lh = lx + 32 * kPitchX;
-    float* ly = reinterpret_cast<float*>(lx);
-    const int n = q.counts[in_q];
-    // two lanes per row: lanes r and r + 32 share row r of the wave's 32-row
-    // tile and each encode and decode half of it
-    const int r = lane & 31, half = lane >> 5;
-    for (int base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
-        const int i0 = base + wave * 32;
-        {
-            const int i = i0 + r;
-            uint16_t* row = lx + r * kPitchX;
-            const float4 h = i < n ? reinterpret_cast<const float4*>(q.hit)[i] : make_float4(__int_as_float(-1), 0, 0, 0);
-            const int tri = __float_as_int(h.x);
-            if (tri >= 0) {
-                const ShadeTri st = sc.stri[tri];
-                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
-                const f3 pos = mk(st.p0.x, st.p0.y, st.p0.z) * b0 + mk(st.p1.x, st.p1.y, st.p1.z) * b1 +
-                               mk(st.p2.x, st.p2.y, st.p2.z) * b2;
-                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
-                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
-                const lmgpu_ray& ray = q.ext[in_q][i];
-                encode_input_half(half, pos, nsh, mk(-ray.dir[0], -ray.dir[1], -ray.dir[2]), box.lo, box.inv, row);
-            } else {
-                for (int k = half * (kIn / 2); k < (half + 1) * (kIn / 2); k++) row[k] = 0;
-            }
-        }
-        __syncthreads();
-        mlp_tile_forward(wl, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
-        __syncthreads();
-        {
-            const int i = i0 + r;
-            if (i < n) decode_lobes4(ly + r * kPitchY, 4 * half, mix + int64_t(i) * kOut);
-        }
-        __syncthreads();
-    }
-}
-
 // After a pass: turns the records written in this pass (the slots listed in
 // rec_list) into training records (pos, nrm, wo, wi, p_mix, f |cos| L_i
 // estimate) appended to train[cap][16] when their target weight is
@@ -786,7 +739,7 @@ int Scene::enqueue_pass(const PTConsts& c, double* sum, double* sumsq, const Gui
         else if (accel.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm, LMGPU_RAY_CHUNK);
         else hipLaunchKernelGGL(k_extend<2>, dim3(tg), dim3(kBlock), 0, s, dev, q, in_q, next.p, pm, LMGPU_RAY_CHUNK);
         if (gp)
-            hipLaunchKernelGGL(k_pt_guide_infer, dim3(std::max(1, std::min((npix + 127) / 128, 512))), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_pt_guide_infer<Queues>, dim3(std::max(1, std::min((npix + 127) / 128, 512))), dim3(256), 0, s,
                                dev, q, in_q, gp->g->Wbf.p, gp->g->P.p, box, gmix.p);
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, counts.p, in_q ^ 1);
         if (gp) hipLaunchKernelGGL(k_shade<true>, dim3(sg), dim3(kBlock), 0, s, dev, ps, q, c, in_q, gd);
@@ -837,14 +790,14 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (image && gp) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering uses the pixel sampling mode");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
-    // pixel-mode renders without a guide go through the regenerating
-    // wavefront: fixed-spp renders of 2 or more samples, and the time
-    // scheduler in chunks of whole samples per pixel; 1-spp calls, the image
-    // mode and guided rendering replay the one-sample pass graph
-    const bool regen = !image && !gp && (p->render_time > 0 || p->spp >= 2);
+    // pixel-mode renders go through the regenerating wavefront: fixed-spp
+    // renders of 2 or more samples, and the time scheduler in chunks of whole
+    // samples per pixel, guided or not; 1-spp calls and the image mode replay
+    // the one-sample pass graph
+    const bool regen = !image && (p->render_time > 0 || p->spp >= 2);
     if (!regen) LMGPU_TRY(ensure_workspace(npix));
     GuidedPass gpv;
-    if (gp) {
+    if (gp && !regen) {
         LMGPU_TRY(ensure_guided(npix, *gp));
         // buffers as they are after ensure_guided (it may grow the guide's training workspace)
         gpv = *gp;
@@ -874,7 +827,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     c.npix = npix;
     c.image = image ? 1 : 0;
     if (regen) {
-        int64_t ext = 0, shd = 0;
+        int64_t ext = 0, shd = 0, grecs = 0;
         int64_t done = 0;
         if (p->render_time > 0) {
             // scheduler::spp::time: chunks of k samples per pixel until the
@@ -890,16 +843,17 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
                 if (t_spp > 0.0) k = int(std::min(256.0, std::max(2.0, std::floor((p->render_time - el) / t_spp))));
                 if (int64_t(p->sample_offset) + done + k > int64_t(INT32_MAX))
                     return fail(LMGPU_ERR_INVALID_ARGUMENT, "sample index beyond 2^31");
-                int64_t e1 = 0, s1 = 0;
-                LMGPU_TRY(render_regen(c, k, int(p->sample_offset + done), sum, sumsq, &e1, &s1, s));
+                int64_t e1 = 0, s1 = 0, g1 = 0;
+                LMGPU_TRY(render_regen(c, k, int(p->sample_offset + done), sum, sumsq, &e1, &s1, s, gp, &g1));
                 ext += e1;
                 shd += s1;
+                grecs += g1;
                 done += k;
                 const double el2 = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 t_spp = (el2 - el) / k;
             }
         } else {
-            LMGPU_TRY(render_regen(c, p->spp, p->sample_offset, sum, sumsq, &ext, &shd, s));
+            LMGPU_TRY(render_regen(c, p->spp, p->sample_offset, sum, sumsq, &ext, &shd, s, gp, &grecs));
             done = p->spp;
         }
         if (!on_device) {
@@ -913,6 +867,7 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
             st->spp = done;
             st->extension_rays = ext;
             st->shadow_rays = shd;
+            st->guide_records = grecs;
             st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
         return LMGPU_OK;

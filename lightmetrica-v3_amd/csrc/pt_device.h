@@ -560,4 +560,52 @@ inline int extend_grid() {
     return grid;
 }
 
+// Guide inference over the extension queue of a bounce: the query of entry
+// i is its hit point, shading normal and the direction back along the ray;
+// the decoded mixture goes to mix[i]. Entries without a hit get a zero query.
+template <class Q>
+__global__ __launch_bounds__(256) void k_pt_guide_infer(SceneDev sc, Q q, int in_q, const uint16_t* __restrict__ Wbf,
+                                                        const float* __restrict__ P, GBox box, float* __restrict__ mix) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kWeightLds + 4 * kWaveLds];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* wl = reinterpret_cast<uint16_t*>(lds);
+    stage_weights(Wbf, wl, threadIdx.x, 256);
+    uint16_t* lx = reinterpret_cast<uint16_t*>(lds + kWeightLds + wave * kWaveLds);
+    uint16_t* lh = lx + 32 * kPitchX;
+    float* ly = reinterpret_cast<float*>(lx);
+    const int n = q.counts[in_q];
+    // two lanes per row: lanes r and r + 32 share row r of the wave's 32-row
+    // tile and each encode and decode half of it
+    const int r = lane & 31, half = lane >> 5;
+    for (int base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
+        const int i0 = base + wave * 32;
+        {
+            const int i = i0 + r;
+            uint16_t* row = lx + r * kPitchX;
+            const float4 h = i < n ? reinterpret_cast<const float4*>(q.hit)[i] : make_float4(__int_as_float(-1), 0, 0, 0);
+            const int tri = __float_as_int(h.x);
+            if (tri >= 0) {
+                const ShadeTri st = sc.stri[tri];
+                const float b1 = h.z, b2 = h.w, b0 = 1.f - b1 - b2;
+                const f3 pos = mk(st.p0.x, st.p0.y, st.p0.z) * b0 + mk(st.p1.x, st.p1.y, st.p1.z) * b1 +
+                               mk(st.p2.x, st.p2.y, st.p2.z) * b2;
+                const f3 nsh = normalize(mk(st.n0.x, st.n0.y, st.n0.z) * b0 + mk(st.n1.x, st.n1.y, st.n1.z) * b1 +
+                                         mk(st.n2.x, st.n2.y, st.n2.z) * b2);
+                const lmgpu_ray& ray = q.ext[in_q][i];
+                encode_input_half(half, pos, nsh, mk(-ray.dir[0], -ray.dir[1], -ray.dir[2]), box.lo, box.inv, row);
+            } else {
+                for (int k = half * (kIn / 2); k < (half + 1) * (kIn / 2); k++) row[k] = 0;
+            }
+        }
+        __syncthreads();
+        mlp_tile_forward(wl, P, lx, lh, ly, lane, [](int, int, int, uint16_t) {});
+        __syncthreads();
+        {
+            const int i = i0 + r;
+            if (i < n) decode_lobes4(ly + r * kPitchY, 4 * half, mix + int64_t(i) * kOut);
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace lmgpu

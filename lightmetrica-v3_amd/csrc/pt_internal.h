@@ -82,8 +82,13 @@ struct Scene {
     // finished sample to the film at once (FP64 atomics), so paths of later samples
     // fill the slots that short paths free instead of each bounce waiting for a
     // pass's longest paths. Same Philox streams and per-vertex code as the passes.
+    // With a guide (gp), directions at diffuse and glossy vertices come from
+    // the one-sample mixture of guide and BSDF and the guide trains online:
+    // the records of paths that finished are collected every iteration and
+    // `steps` Adam steps run whenever batch * steps of them are buffered.
     int render_regen(const PTConsts& c, int spp, int sample_offset, double* sum, double* sumsq, int64_t* ext_rays,
-                     int64_t* shadow_rays, hipStream_t s);
+                     int64_t* shadow_rays, hipStream_t s, const GuidedPass* gp = nullptr,
+                     int64_t* guide_records = nullptr);
     // renderer::lt and renderer::bdpt (lt.hip)
     DevBuf<float4> blverts;            // BDPT light vertices, [samples per pass][max_verts - 1]
     DevBuf<lmgpu_ray> bsh;             // BDPT connection queue

@@ -26,6 +26,16 @@
 // Rays a wave takes per atomic on the traversal's ray counter in the
 // regenerating loops, whose queues stay long (the one-sample passes use
 // LMGPU_RAY_CHUNK: their late bounces queue few rays).
+// Path pool of a guided render, whose slots also hold training records
+// (record_depth x 96 B each): 4 slots per pixel, at most 2^23. With fewer
+// paths in flight an iteration finishes about half a sample per pixel's
+// worth of paths, so the guide can train after every such sample.
+#ifndef LMGPU_REGEN_GUIDED_SLOTS_PER_PIXEL
+#define LMGPU_REGEN_GUIDED_SLOTS_PER_PIXEL 4
+#endif
+#ifndef LMGPU_REGEN_GUIDED_MAX_SLOTS
+#define LMGPU_REGEN_GUIDED_MAX_SLOTS (1 << 23)
+#endif
 #ifndef LMGPU_REGEN_RAY_CHUNK
 #define LMGPU_REGEN_RAY_CHUNK 128
 #endif
@@ -73,6 +83,19 @@ struct BatchCtl {
 };
 
 
+struct Guide;
+
+// Guided path tracing in the regenerating loop: the guide, its options and
+// the scene the inference reads hit points from.
+struct RegenGuide {
+    Guide* g = nullptr;
+    SceneDev sc{};
+    GBox box{};
+    float alpha = 0.5f, lr = 1e-3f, wclamp = 50.f;
+    int batch = 0, steps = 0, rec_depth = 0;
+    int npix = 0;   // the guide trains after every npix finished paths
+};
+
 // Path pool, work queue and the loop that drains it (regenerate -> extend ->
 // shade -> connect, two iterations per graph replay), shared by the batch
 // and by lmgpu_pt_render's regenerating path.
@@ -92,27 +115,44 @@ struct Wavefront {
     DevBuf<JobDev> djobs;
     DevBuf<WorkSeg> dsegs;
     DevBuf<unsigned long long> job_done, job_time;
+    // guided runs: training records per (vertex, slot), decoded mixtures per
+    // extension-queue entry, the record slot of each shadow-queue entry, the
+    // training buffer with its counters ([0] appended, [1] released to the
+    // training steps, [2] paths finished since the last release) and the
+    // count of training records
+    DevBuf<GRec> grec;
+    DevBuf<float> gmix, gtrain;
+    DevBuf<int32_t> gshrec, gcnt;
+    DevBuf<unsigned long long> grecorded;
     ~Wavefront() {
         if (ctl_mirror) (void)hipHostFree(ctl_mirror);
     }
     int ensure_pool(int cap);
+    // Guided buffers for the current pool (records zeroed when allocated).
+    int ensure_guided(int rec_depth, int64_t train_cap);
     // Uploads the jobs and the work segments (queue order) of one run.
     int set_work(const std::vector<JobDev>& jobs, const std::vector<WorkSeg>& segs, hipStream_t s);
     // Renders every work item of the tables set last into film_sum / film_sq
     // (FP64 atomics; film_sq may be null) and waits for the end.
+    // With a guide, the directions at diffuse and glossy vertices come from
+    // the one-sample mixture of guide and BSDF, and the guide trains online
+    // on the records of finished paths.
     int run(const void* trav_nodes, const TriV* tris, int width, int block, int64_t total, int max_verts,
-            double* film_sum, double* film_sq, double* elapsed, int64_t* iters, hipStream_t s);
+            double* film_sum, double* film_sq, double* elapsed, int64_t* iters, hipStream_t s,
+            const RegenGuide* guide = nullptr);
     // The loop itself: frees the pool, sets the run's counters for `total`
     // items, captures iteration(stream, 0) and iteration(stream, 1) (the
     // kernels of one iteration on queue parity 0 / 1) as one graph and
     // replays it until every item is done; max_verts bounds the iterations.
     int loop(const std::function<void(hipStream_t, int)>& iteration, int64_t total, int max_verts, double* elapsed,
              int64_t* iters, hipStream_t s);
-    // Slot count for `total` items over `npix` pixels.
-    static int pool_size(int64_t total, int npix) {
-        const int64_t want = std::min<int64_t>(total, int64_t(LMGPU_REGEN_SLOTS_PER_PIXEL) * npix);
+    // Slot count for `total` items over `npix` pixels: per_pixel slots per
+    // pixel, a power of two in [4096, max_slots].
+    static int pool_size(int64_t total, int npix, int per_pixel = LMGPU_REGEN_SLOTS_PER_PIXEL,
+                         int max_slots = LMGPU_REGEN_MAX_SLOTS) {
+        const int64_t want = std::min<int64_t>(total, int64_t(per_pixel) * npix);
         int c = 4096;
-        while (c < want && c < LMGPU_REGEN_MAX_SLOTS) c <<= 1;
+        while (c < want && c < max_slots) c <<= 1;
         return c;
     }
 };

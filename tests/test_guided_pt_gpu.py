@@ -55,6 +55,34 @@ def test_guided_furnace_known_answer(device, mode, steps):
         assert st['guide_records'] > 0
 
 
+@pytest.mark.parametrize('steps', [1, 0])
+def test_guided_furnace_one_sample_passes(device, steps):
+    """The one-sample pass graph (renders of one sample per pixel, called
+    with consecutive sample offsets; the guide trains after every pass): the
+    furnace closed form E[L] = Le sum_k rho^k."""
+    import torch
+    rho, Le, M, spp = 0.5, 1.0, 8, 48
+    flat = furnace_scene(rho, Le).flatten()
+    sc = GpuScene(flat)
+    lo, hi = sc.bounds()
+    g = Guide(lo - 1e-3, hi + 1e-3, seed=4)
+    f = torch.zeros((flat.h, flat.w, 3), dtype=torch.float64, device=device)
+    q = torch.zeros_like(f)
+    for k in range(spp):
+        _, _, st = sc.render_raw(spp=1, max_verts=M, seed=5, sample_offset=k, film_sum=f, film_sumsq=q, guide=g,
+                                 guide_opts={'steps_per_pass': steps, 'batch': 2048})
+    assert st['spp'] == 1
+    if steps:
+        assert g.stats()[0] == spp
+    n = flat.w * flat.h * spp
+    mean = f[..., 0].sum().item() / n
+    var = q[..., 0].sum().item() / n - mean ** 2
+    expected = Le * sum(rho ** k for k in range(M - 1))
+    z = (mean - expected) / math.sqrt(var / n)
+    print(f'guided furnace, one-sample passes, steps {steps}: mean {mean:.6f} expected {expected:.6f} z {z:.2f}')
+    assert abs(z) < 5
+
+
 def test_guided_c0_welch_vs_oracle(device):
     """Config 0 at 128x128, 64 spp, guided with online training, against the
     FP64 oracle of plain renderer::pt (independent seeds)."""
@@ -71,7 +99,7 @@ def test_guided_c0_welch_vs_oracle(device):
           f'{stats.rel_mse(s / spp, ref["image"]):.3e}; adam steps {steps} last loss {loss:.4f}; '
           f'records {st["guide_records"]}')
     assert frac <= bound
-    assert steps == spp and math.isfinite(loss)
+    assert steps > 0 and math.isfinite(loss)
 
 
 def c4_flat(w=192, h=108):
@@ -101,8 +129,9 @@ def test_guided_c4_welch_vs_plain(device):
 
 def test_guided_c4_lowers_error(device):
     """What the trained network gains over the method without it: on the
-    slit-lit room (config 4 at 192x108) a guide trained online for 384
-    passes, then frozen, lowers the relative MSE against a 2048-spp plain
+    slit-lit room (config 4 at 192x108) a guide trained online over a
+    384-spp render (4 Adam steps of up to 4096 records after every sample per
+    pixel of finished paths), then frozen, lowers the relative MSE against a 2048-spp plain
     image at equal samples per pixel; the untrained guide (frozen at its
     random initialisation) does not beat plain PT. Guided runs use their
     default of no Russian roulette, and so does the plain run compared.
@@ -127,7 +156,7 @@ def test_guided_c4_lowers_error(device):
     g0 = Guide(lo - 1e-3, hi + 1e-3, seed=2)
     e_untrained = err(guide=g0, guide_opts={'steps_per_pass': 0})
     g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-    opts = {'steps_per_pass': 4, 'batch': 16384, 'learning_rate': 5e-3}
+    opts = {'steps_per_pass': 4, 'batch': 4096, 'learning_rate': 5e-3}
     sc.render_raw(spp=384, max_verts=16, seed=22, guide=g, guide_opts=opts)
     e_guided = err(guide=g, guide_opts={'steps_per_pass': 0})
     print(f'C4 192x108 {spp} spp median rMSE of {len(seeds)} renders: plain {e_plain:.4e}, untrained guide '
