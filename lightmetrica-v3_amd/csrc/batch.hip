@@ -293,53 +293,104 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
 // record of the slot is cleared for the slot's next path. Slots are visited
 // in the order j * kScramble mod cap (an odd multiplier: a bijection on the
 // power-of-two pool), so that a buffer that fills up holds paths from all
-// over the film. The finished paths are counted in *nfin.
+// over the film. A first sweep over a slot's records counts what it keeps,
+// one atomic per block and stride reserves the block's entries, a second
+// sweep writes them; the finished paths (*nfin) and kept records
+// (*recorded) are added once per block.
+__device__ __forceinline__ bool grec_target(const GRec* rp, float4 Lf, float4 a, float& lx, float& ly, float& lz) {
+    const float4 e = rp->e, f = rp->f;
+    lx = fmaxf(0.f, Lf.x - e.x) * f.x;
+    ly = fmaxf(0.f, Lf.y - e.y) * f.y;
+    lz = fmaxf(0.f, Lf.z - e.z) * f.z;
+    const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / a.w;
+    return wgt > 0.f && wgt < 3e38f;
+}
+
 __global__ __launch_bounds__(256) void k_b_gcollect(PoolDev P, int cap, GRec* __restrict__ rec, int rec_depth,
                                                     float* __restrict__ train, int tcap, int32_t* __restrict__ tcount,
                                                     int32_t* __restrict__ nfin,
                                                     unsigned long long* __restrict__ recorded) {
     constexpr uint32_t kScramble = 2654435761u;
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    __shared__ unsigned long long s_rec[4];
+    __shared__ int s_fin[4];
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
     const int stride = gridDim.x * 256;
     const int n_iter = (cap + stride - 1) / stride;
+    unsigned long long my_rec = 0;
+    int my_fin = 0;
     for (int it = 0; it < n_iter; it++) {
         const int j = it * stride + blockIdx.x * 256 + threadIdx.x;
         const int s = int((uint32_t(j) * kScramble) & uint32_t(cap - 1));
         const bool fin = j < cap && P.state[s] == kFinishing;
-        const unsigned long long fmask = __ballot(fin);
-        if (fmask && __lane_id() == __ffsll(fmask) - 1) atomicAdd(nfin, __popcll(fmask));
         const float4 Lf = fin ? P.L[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int d = 0; d < rec_depth; d++) {
-            bool keep = false;
-            float4 o0, o1, o2, o3;
-            if (fin) {
+        int cnt = 0;
+        if (fin) {
+            for (int d = 0; d < rec_depth; d++) {
+                const GRec* rp = rec + int64_t(d) * cap + s;
+                const float4 a = rp->a;
+                float lx, ly, lz;
+                if (a.w > 0.f && grec_target(rp, Lf, a, lx, ly, lz)) cnt++;
+            }
+        }
+        my_fin += fin ? 1 : 0;
+        my_rec += cnt;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int w = 0; w < 4; w++) {
+                const int x = s_wave[w];
+                s_wave[w] = acc;
+                acc += x;
+            }
+            s_base = acc ? atomicAdd(tcount, acc) : 0;
+        }
+        __syncthreads();
+        int k = s_base + s_wave[wave] + incl - cnt;
+        __syncthreads();   // the next stride rewrites s_wave and s_base
+        if (fin) {
+            for (int d = 0; d < rec_depth; d++) {
                 GRec* rp = rec + int64_t(d) * cap + s;
                 const float4 a = rp->a;
                 if (a.w > 0.f) {
-                    const GRec r = *rp;
-                    const float lx = fmaxf(0.f, Lf.x - r.e.x) * r.f.x, ly = fmaxf(0.f, Lf.y - r.e.y) * r.f.y,
-                                lz = fmaxf(0.f, Lf.z - r.e.z) * r.f.z;
-                    const float wgt = (0.2126f * lx + 0.7152f * ly + 0.0722f * lz) / r.a.w;
-                    if (wgt > 0.f && wgt < 3e38f) {
-                        keep = true;
-                        o0 = make_float4(r.a.x, r.a.y, r.a.z, r.b.x);
-                        o1 = make_float4(r.b.y, r.b.z, r.c.x, r.c.y);
-                        o2 = make_float4(r.c.z, r.d.x, r.d.y, r.d.z);
-                        o3 = make_float4(r.a.w, lx, ly, lz);
+                    float lx, ly, lz;
+                    if (grec_target(rp, Lf, a, lx, ly, lz)) {
+                        if (k < tcap) {
+                            const float4 b = rp->b, c = rp->c, dd = rp->d;
+                            float4* o = reinterpret_cast<float4*>(train + int64_t(k) * kRecordFloats);
+                            o[0] = make_float4(a.x, a.y, a.z, b.x);
+                            o[1] = make_float4(b.y, b.z, c.x, c.y);
+                            o[2] = make_float4(c.z, dd.x, dd.y, dd.z);
+                            o[3] = make_float4(a.w, lx, ly, lz);
+                        }
+                        k++;
                     }
                     rp->a = make_float4(a.x, a.y, a.z, 0.f);
                 }
             }
-            const unsigned long long mask = __ballot(keep);
-            if (mask && __lane_id() == __ffsll(mask) - 1) atomicAdd(recorded, (unsigned long long)__popcll(mask));
-            const int k = wave_append(tcount, keep);
-            if (keep && k < tcap) {
-                float4* o = reinterpret_cast<float4*>(train + int64_t(k) * kRecordFloats);
-                o[0] = o0;
-                o[1] = o1;
-                o[2] = o2;
-                o[3] = o3;
-            }
         }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        my_rec += __shfl_xor(my_rec, o);
+        my_fin += __shfl_xor(my_fin, o);
+    }
+    if (lane == 0) {
+        s_rec[wave] = my_rec;
+        s_fin[wave] = my_fin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long r = s_rec[0] + s_rec[1] + s_rec[2] + s_rec[3];
+        const int f = s_fin[0] + s_fin[1] + s_fin[2] + s_fin[3];
+        if (r) atomicAdd(recorded, r);
+        if (f) atomicAdd(nfin, f);
     }
 }
 

@@ -305,8 +305,10 @@ __global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* n
     const uint16_t* A = dT + int64_t(job.layer - 1) * kHid * B + int64_t(job.nt * 32 + r) * B;
     const uint16_t* Bm = (job.layer == 1 ? xT : hT + int64_t(job.layer - 2) * kHid * B) + int64_t(job.kt * 32 + r) * B;
     const int64_t rows = batch_rows(ndev, off, n_max);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0 && rows > 0) *stepc += 1;
     const int64_t i_beg = int64_t(blockIdx.x) * slice;
     const int64_t i_end = std::min<int64_t>(round128(rows), i_beg + slice);
+    if (i_end <= i_beg) return;   // a block past the batch (whole block)
     const bool bias = job.kt == 0;
     f32x16 acc = {};
     float bsum = 0.f;
@@ -330,9 +332,8 @@ __global__ __launch_bounds__(64) void k_guide_dw(int64_t n_max, const int32_t* n
         bsum += __shfl_xor(bsum, 32);
         const int boff = job.layer == 1 ? kOffB1 : (job.layer == 2 ? kOffB2 : (job.layer == 3 ? kOffB3 : kOffB4));
         const int n = job.nt * 32 + r;
-        if (h == 0 && n < N && i_end > i_beg) atomicAdd(&grad[boff + n], bsum);
+        if (h == 0 && n < N) atomicAdd(&grad[boff + n], bsum);
     }
-    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0 && rows > 0) *stepc += 1;
 }
 
 // Refreshes the BF16 copies: Wbf (packed, transposed weights) and WnBf
