@@ -290,10 +290,11 @@ __global__ __launch_bounds__(BLOCK) void k_b_shade(const JobDev* __restrict__ jo
 // in kFinishing, before k_b_regen frees them): record d of slot s becomes
 // (pos, nrm, wo, wi, p_mix, (L_final - e) f) when its target weight is
 // positive, appended to train[tcap][16] while there is room, and every
-// record of the slot is cleared for the slot's next path. Slots are visited
-// in the order j * kScramble mod cap (an odd multiplier: a bijection on the
-// power-of-two pool), so that a buffer that fills up holds paths from all
-// over the film. A first sweep over a slot's records counts what it keeps,
+// record of the slot is cleared for the slot's next path. Each wave takes a
+// run of 64 consecutive slots (coalesced loads), the runs in the order
+// w * kScramble mod (cap / 64) (an odd multiplier: a bijection on the
+// power-of-two count of runs), so that a buffer that fills up holds paths
+// from all over the film. A first sweep over a slot's records counts what it keeps,
 // one atomic per block and stride reserves the block's entries, a second
 // sweep writes them; the finished paths (*nfin) and kept records
 // (*recorded) are added once per block.
@@ -322,7 +323,8 @@ __global__ __launch_bounds__(256) void k_b_gcollect(PoolDev P, int cap, GRec* __
     int my_fin = 0;
     for (int it = 0; it < n_iter; it++) {
         const int j = it * stride + blockIdx.x * 256 + threadIdx.x;
-        const int s = int((uint32_t(j) * kScramble) & uint32_t(cap - 1));
+        const uint32_t run = (uint32_t(j >> 6) * kScramble) & uint32_t((cap >> 6) - 1);
+        const int s = int(run << 6) + lane;
         const bool fin = j < cap && P.state[s] == kFinishing;
         const float4 Lf = fin ? P.L[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         int cnt = 0;

@@ -99,6 +99,7 @@ __global__ __launch_bounds__(256) void k_guide_weights(int64_t n_max, const int3
     __shared__ float red[4];
     const int64_t n = batch_rows(ndev, off, n_max);
     if (blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.f;
+    if (int64_t(blockIdx.x) * 256 >= n) return;   // a block past the batch (whole block)
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     float v = 0.f;
     if (i < n) {
