@@ -246,7 +246,9 @@ typedef struct lmgpu_pt_params {
     uint64_t seed;            /* Philox key */
     int32_t sample_offset;    /* index of the first sample (progressive rendering) */
     int32_t guiding;          /* must be 0: guided rendering is lmgpu_pt_render_guided */
-    double render_time;       /* > 0: scheduler::spp::time, render 1-spp passes for this many seconds */
+    double render_time;       /* > 0: scheduler::spp::time, render whole samples per pixel for this many
+                                 seconds (pixel mode: the regenerating loop in chunks of samples; image
+                                 mode: passes of w*h samples) */
     int32_t primary_mode;     /* LMGPU_PRIMARY_PIXEL or LMGPU_PRIMARY_IMAGE (primary_ray_sampling_mode) */
     int32_t reserved0;
     int64_t num_samples;      /* image mode, scheduler::spi::sample: samples over the whole image */
@@ -446,18 +448,22 @@ int lmgpu_guide_stats(const lmgpu_guide* guide, int64_t* steps, float* last_loss
 /* Guided renderer::pt. At diffuse and glossy vertices the next direction is
  * drawn from the guide with probability alpha and from the BSDF otherwise;
  * the mixture density is used in the throughput and in all MIS weights, so
- * the estimate is unbiased for any guide. After every 1-spp pass the
- * radiance that arrived along the sampled directions of the first
- * record_depth surface vertices is turned into training records (pos, nrm,
- * wo, wi, p_mix, f|cos|L_i), up to batch * steps_per_pass of them in a
- * film- and depth-spread order, and the guide takes steps_per_pass Adam
- * steps; the next pass samples the updated guide. The whole pass, training
- * included, is one captured hipGraph. */
+ * the estimate is unbiased for any guide. The radiance that arrived along
+ * the sampled directions of the first record_depth surface vertices of a
+ * path is turned into training records (pos, nrm, wo, wi, p_mix,
+ * f|cos|L_i) when the path ends, up to batch * steps_per_pass of them per
+ * sample per pixel in a film-spread order, and after every sample per
+ * pixel's worth of finished paths the guide takes steps_per_pass Adam steps
+ * (the last batches partial or empty); paths started later sample the
+ * updated guide. Renders of 2 or more samples per pixel and the time
+ * scheduler run in the regenerating loop (the guide's inference, records
+ * and training steps are kernels of its captured iteration); a 1-spp call
+ * is one captured one-sample pass followed by its training steps. */
 typedef struct lmgpu_guide_opts {
     float alpha;              /* guide sampling probability (0.65) */
     float learning_rate;      /* Adam learning rate (1e-2) */
-    int32_t batch;            /* records per Adam step, rounded up to 128 (2^16) */
-    int32_t steps_per_pass;   /* Adam steps per pass (8); 0 = frozen guide, nothing recorded */
+    int32_t batch;            /* records per Adam step, rounded up to 128 (2^14) */
+    int32_t steps_per_pass;   /* Adam steps per sample per pixel (4); 0 = frozen guide, nothing recorded */
     int32_t record_depth;     /* recording surface vertices per path (8, at most 16) */
     float weight_clamp;       /* clamp of training weights relative to the batch mean (0 = 50) */
     int32_t reserved[6];

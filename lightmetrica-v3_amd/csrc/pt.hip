@@ -1046,8 +1046,8 @@ int lmgpu_pt_render_guided(lmgpu_scene* scene, const lmgpu_pt_params* params, lm
     lmgpu_guide_opts o{};
     o.alpha = 0.65f;
     o.learning_rate = 1e-2f;
-    o.batch = 1 << 16;
-    o.steps_per_pass = 8;
+    o.batch = 1 << 14;
+    o.steps_per_pass = 4;
     o.record_depth = 8;
     if (opts) o = *opts;
     if (!(o.alpha >= 0.f && o.alpha <= 1.f)) return fail(LMGPU_ERR_INVALID_ARGUMENT, "alpha must be in [0, 1]");

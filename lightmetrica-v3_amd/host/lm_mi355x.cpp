@@ -478,8 +478,8 @@ Renderer_PT_MI355X::Renderer_PT_MI355X(const Scene* scene, Film* film, const Pro
     }
     g_alpha_ = float(prop_or<double>(p, "guide_alpha", 0.65));
     g_lr_ = float(prop_or<double>(p, "guide_lr", 1e-2));
-    g_batch_ = prop_or<int>(p, "guide_batch", 1 << 16);
-    g_steps_ = prop_or<int>(p, "guide_steps", 8);
+    g_batch_ = prop_or<int>(p, "guide_batch", 1 << 14);
+    g_steps_ = prop_or<int>(p, "guide_steps", 4);
     g_depth_ = prop_or<int>(p, "guide_depth", 8);
     acc_width_ = prop_or<int>(p, "accel_width", 4);
     acc_leaf_ = prop_or<int>(p, "accel_leaf_size", 2);

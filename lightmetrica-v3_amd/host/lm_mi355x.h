@@ -304,8 +304,8 @@ public:
 //   "time"): spp (pixel mode, scheduler::spp::sample), num_samples (image
 //   mode, scheduler::spi::sample) or render_time (seconds),
 //   seed (int), rr_depth (int; 5, or max_verts when guiding), guiding
-//   (bool) with guide_alpha (0.65), guide_lr (1e-2), guide_batch (2^16),
-//   guide_steps (8), guide_depth (8);
+//   (bool) with guide_alpha (0.65), guide_lr (1e-2), guide_batch (2^14),
+//   guide_steps (4), guide_depth (8);
 //   accel_width, accel_leaf_size (BVH of the renderer's device scene).
 class Renderer_PT_MI355X final : public Renderer {
 public:

@@ -312,7 +312,7 @@ class RendererRaycast:
 
 
 # lmgpu_guide_opts defaults: alpha, learning_rate, batch, steps_per_pass, record_depth
-GUIDE_DEFAULTS = (0.65, 1e-2, 1 << 16, 8, 8)
+GUIDE_DEFAULTS = (0.65, 1e-2, 1 << 14, 4, 8)
 
 
 class RendererPT:
