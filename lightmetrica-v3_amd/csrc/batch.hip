@@ -327,9 +327,12 @@ __global__ __launch_bounds__(256) void k_b_gcollect(PoolDev P, int cap, GRec* __
         const int s = int(run << 6) + lane;
         const bool fin = j < cap && P.state[s] == kFinishing;
         const float4 Lf = fin ? P.L[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // a vertex that sampled a direction left the path's vertex count at
+        // its own count + 1, so only records d < count - 1 can have been written
+        const int dmax = fin ? min(rec_depth, P.info[s].x - 1) : 0;
         int cnt = 0;
         if (fin) {
-            for (int d = 0; d < rec_depth; d++) {
+            for (int d = 0; d < dmax; d++) {
                 const GRec* rp = rec + int64_t(d) * cap + s;
                 const float4 a = rp->a;
                 float lx, ly, lz;
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(256) void k_b_gcollect(PoolDev P, int cap, GRec* __
         int k = s_base + s_wave[wave] + incl - cnt;
         __syncthreads();   // the next stride rewrites s_wave and s_base
         if (fin) {
-            for (int d = 0; d < rec_depth; d++) {
+            for (int d = 0; d < dmax; d++) {
                 GRec* rp = rec + int64_t(d) * cap + s;
                 const float4 a = rp->a;
                 if (a.w > 0.f) {

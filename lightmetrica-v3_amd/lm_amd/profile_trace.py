@@ -2,7 +2,7 @@
 incoherent rays, preceded by a plain device copy of known size that
 calibrates the HBM byte counters; optionally one 1080p config-2
 path-tracing pass (--pt), 4 + 4 plain (roulette from vertex 5 / none) and 4 guided (no
-roulette) 1080p passes of config 4 (--c4), a config-3 batch (--c3 N), light-tracing / BDPT passes (--lt-bdpt) and
+roulette) 1080p renders of config 4 (--c4), a config-3 batch (--c3 N), light-tracing / BDPT passes (--lt-bdpt) and
 the guide MLP's training and inference kernels (--guide).
 
 Run under rocprofv3, e.g.
@@ -30,7 +30,7 @@ def main():
                     help='also run one 1080p config-2 PT pass (one-sample pass graph) and a 4-spp render of it '
                          '(regenerating loop)')
     ap.add_argument('--pt-width', type=int, default=8, help='BVH width of the path-tracing scene')
-    ap.add_argument('--c4', action='store_true', help='also run 4 plain + 4 guided 1080p config-4 passes')
+    ap.add_argument('--c4', action='store_true', help='also render 1080p config 4 at 32 spp: plain, plain without roulette, guided')
     ap.add_argument('--no-trace', action='store_true', help='skip the config-1 closest-hit launches')
     ap.add_argument('--c3', type=int, default=0, help='also render the first N config-3 jobs as one batch')
     ap.add_argument('--guide', action='store_true',
@@ -97,17 +97,17 @@ def main():
         s4 = scenes.config4().flatten()
         sc = GpuScene(s4, width=args.width, leaf_size=args.leaf)
         fs = torch.zeros((s4.h, s4.w, 3), dtype=torch.float64, device=dev)
-        _, _, st = sc.render_raw(spp=4, max_verts=16, seed=1, film_sum=fs)
+        _, _, st = sc.render_raw(spp=32, max_verts=16, seed=1, film_sum=fs)
         torch.cuda.synchronize()
-        print('4 x 1080p config-4 plain PT passes (roulette from vertex 5)', st, flush=True)
-        _, _, st = sc.render_raw(spp=4, max_verts=16, rr_depth=16, seed=1, film_sum=fs)
+        print('32-spp 1080p config-4 plain PT render (roulette from vertex 5)', st, flush=True)
+        _, _, st = sc.render_raw(spp=32, max_verts=16, rr_depth=16, seed=1, film_sum=fs)
         torch.cuda.synchronize()
-        print('4 x 1080p config-4 plain PT passes (no roulette)', st, flush=True)
+        print('32-spp 1080p config-4 plain PT render (no roulette)', st, flush=True)
         lo, hi = sc.bounds()
         g = Guide(lo - 1e-3, hi + 1e-3, seed=2)
-        _, _, st = sc.render_raw(spp=4, max_verts=16, rr_depth=16, seed=2, film_sum=fs, guide=g)
+        _, _, st = sc.render_raw(spp=32, max_verts=16, rr_depth=16, seed=2, film_sum=fs, guide=g)
         torch.cuda.synchronize()
-        print('4 x 1080p config-4 guided PT passes (default guide options, no roulette)', st, flush=True)
+        print('32-spp 1080p config-4 guided PT render (default guide options, no roulette)', st, flush=True)
     if args.guide:
         guide_workload(dev)
 
