@@ -97,24 +97,6 @@ __device__ __forceinline__ void lt_start(const SceneDev& sc, const LTConsts& c, 
     ray.tmax = kInf;
 }
 
-// Light subpath starts of one pass. Path state: thr = throughput, info.x =
-// vertex count of the vertex the ray will hit.
-__global__ __launch_bounds__(kBlock) void k_lt_generate(SceneDev sc, PathState ps, Queues q, LTConsts c) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= c.npaths) return;
-    lmgpu_ray ray;
-    f3 w;
-    lt_start(sc, c, uint32_t(i), c.pass, ray, w);
-    ps.thr[i] = make_float4(w.x, w.y, w.z, 0.f);
-    ps.info[i] = make_int4(2, 0, 0, 0);
-    q.ext[0][i] = ray;
-    q.ext_path[0][i] = i;
-    if (i == 0) {
-        q.counts[0] = c.npaths;
-        atomicAdd(reinterpret_cast<unsigned long long*>(q.counts + 8), (unsigned long long)c.npaths);
-    }
-}
-
 // Regenerating light tracing (fixed num_samples): frees the slots whose path
 // ended (state kFinishing from k_lt_shade) and gives the free slots the next
 // light paths, item j being path j mod npix of pass pass0 + j / npix (the
@@ -386,26 +368,6 @@ int lt_connect_grid() {
         grid = std::max(1, cus) * std::max(1, a);
     }
     return grid;
-}
-
-template <int W>
-void lt_pass(Scene& S, const LTConsts& c, double* film, int refill, hipStream_t s) {
-    PathState ps{S.thr.p, S.L.p, S.prev.p, S.info.p};
-    Queues q{{S.ext0.p, S.ext1.p}, S.hit.p, {S.path0.p, S.path1.p}, S.sh.p, S.shc.p, S.counts.p};
-    hipLaunchKernelGGL(k_lt_generate, dim3(unsigned((c.npaths + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, S.dev, ps,
-                       q, c);
-    const int sg = std::max(1, std::min((c.npaths + kBlock - 1) / kBlock, 2048));
-    const int tg = extend_grid<W>();
-    const int cg = lt_connect_grid<W>();
-    for (int b = 0; b < c.max_verts - 1; b++) {
-        const int in_q = b & 1;
-        hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, S.next.p, S.counts.p, in_q ^ 1);
-        hipLaunchKernelGGL(k_extend<W>, dim3(tg), dim3(kBlock), 0, s, S.dev, q, in_q, S.next.p, refill, LMGPU_RAY_CHUNK);
-        hipLaunchKernelGGL(k_lt_shade<false>, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, nullptr, nullptr,
-                           nullptr);
-        hipLaunchKernelGGL(k_lt_connect<W>, dim3(cg), dim3(kBlock), 0, s, S.dev, q, S.next.p + 1, refill, film,
-                           LMGPU_RAY_CHUNK);
-    }
 }
 
 // One iteration of the regenerating light tracer on queue parity in_q.
@@ -1030,18 +992,20 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
     const f3 hi = mk(accel.root_hi[0], accel.root_hi[1], accel.root_hi[2]);
     c.sph_c = (lo + hi) * .5f;
     c.sph_r = length(hi - c.sph_c) * 1.01f;
-    if (!(p->render_time > 0)) {
-        // scheduler::spi::sample: the regenerating loop; a slot whose light
-        // path ends takes the next one (path i of pass k as in the passes),
-        // so the long paths of a pass no longer hold up the next pass
-        if (!wf) {
-            wf = new (std::nothrow) Wavefront();
-            if (!wf) return fail(LMGPU_ERR_OUT_OF_MEMORY, "wavefront");
-        }
-        const int64_t total = p->num_samples;
+    // scheduler::spi::sample and spi::time: the regenerating loop; a slot
+    // whose light path ends takes the next one (path i of pass k as in the
+    // passes), so the long paths of a pass no longer hold up the next pass.
+    // The time scheduler runs it in chunks of whole passes (k passes of w*h
+    // paths, k what the time left allows at the measured time per pass, at
+    // most 256), on consecutive pass indices.
+    if (!wf) {
+        wf = new (std::nothrow) Wavefront();
+        if (!wf) return fail(LMGPU_ERR_OUT_OF_MEMORY, "wavefront");
+    }
+    const int W = accel.width;
+    int64_t ext = 0, shd = 0;
+    auto run_loop = [&](int64_t total, uint32_t pass0) -> int {
         LMGPU_TRY(wf->ensure_pool(Wavefront::pool_size(total, npix)));
-        const uint32_t pass0 = uint32_t(p->pass_offset);
-        const int W = accel.width;
         auto iteration = [&](hipStream_t cs, int in_q) {
             if (W == 4) lt_regen_iteration<4>(*this, *wf, c, total, npix, pass0, film, in_q, cs);
             else if (W == 8) lt_regen_iteration<8>(*this, *wf, c, total, npix, pass0, film, in_q, cs);
@@ -1051,53 +1015,42 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
         int64_t iters = 0;
         LMGPU_TRY(wf->loop(iteration, total, p->max_verts, &sec, &iters, s));
         int32_t cnt[16];
-        LMGPU_HIP(hipMemcpyAsync(cnt, wf->counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-        if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
-        LMGPU_HIP(hipStreamSynchronize(s));
-        if (st) {
-            std::memset(st, 0, sizeof(*st));
-            int64_t v[3];
-            std::memcpy(v, cnt + 8, sizeof(v));
-            st->samples = total;
-            st->spp = (total + npix - 1) / npix;
-            st->extension_rays = v[1];
-            st->shadow_rays = v[2];
-            st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        }
-        return LMGPU_OK;
-    }
-    int64_t processed = 0, passes = 0;
-    uint32_t pass = uint32_t(p->pass_offset);
-    const int refill = accel.refill;
-    while (true) {
-        const int64_t left = p->render_time > 0 ? int64_t(npix) : p->num_samples - processed;
-        if (left <= 0) break;
-        c.npaths = int(std::min<int64_t>(left, npix));
-        c.pass = pass++;
-        if (accel.width == 4) lt_pass<4>(*this, c, film, refill, s);
-        else if (accel.width == 8) lt_pass<8>(*this, c, film, refill, s);
-        else lt_pass<2>(*this, c, film, refill, s);
-        LMGPU_HIP(hipGetLastError());
-        processed += c.npaths;
-        passes++;
-        if (p->render_time > 0) {
-            LMGPU_HIP(hipStreamSynchronize(s));
-            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= p->render_time) break;
-        }
-    }
-    int32_t cnt[16];
-    LMGPU_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
-    LMGPU_HIP(hipStreamSynchronize(s));
-    if (cnt[5]) return fail(LMGPU_ERR_STACK_OVERFLOW, "BVH traversal stack capacity exceeded during light tracing");
-    if (st) {
-        std::memset(st, 0, sizeof(*st));
+        LMGPU_HIP(hipMemcpy(cnt, wf->counts.p, sizeof(cnt), hipMemcpyDeviceToHost));
         int64_t v[3];
         std::memcpy(v, cnt + 8, sizeof(v));
-        st->samples = processed;
+        ext += v[1];
+        shd += v[2];
+        return LMGPU_OK;
+    };
+    int64_t total = 0, passes = 0;
+    if (!(p->render_time > 0)) {
+        total = p->num_samples;
+        LMGPU_TRY(run_loop(total, uint32_t(p->pass_offset)));
+        passes = (total + npix - 1) / npix;
+    } else {
+        double t_pass = 0.0;
+        while (true) {
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (passes > 0 && el >= p->render_time) break;
+            int k = 1;
+            if (t_pass > 0.0) k = int(std::min(256.0, std::max(1.0, std::floor((p->render_time - el) / t_pass))));
+            if (int64_t(p->pass_offset) + passes + k > int64_t(UINT32_MAX))
+                return fail(LMGPU_ERR_INVALID_ARGUMENT, "pass index beyond 2^32");
+            LMGPU_TRY(run_loop(int64_t(k) * npix, uint32_t(p->pass_offset + passes)));
+            passes += k;
+            total += int64_t(k) * npix;
+            const double el2 = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            t_pass = (el2 - el) / k;
+        }
+    }
+    if (!on_device) LMGPU_HIP(hipMemcpyAsync(film_in, film, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost, s));
+    LMGPU_HIP(hipStreamSynchronize(s));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->samples = total;
         st->spp = passes;
-        st->extension_rays = v[1];
-        st->shadow_rays = v[2];
+        st->extension_rays = ext;
+        st->shadow_rays = shd;
         st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return LMGPU_OK;

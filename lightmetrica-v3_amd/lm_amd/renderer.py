@@ -232,7 +232,8 @@ class GpuScene:
     def render_lt(self, num_samples: int = 0, max_verts: int = 8, seed: int = 0, pass_offset: int = 0,
                   render_time: float = 0.0, film=None, stream=None):
         """lmgpu_lt_render: splat sums of `num_samples` light paths (or of
-        w*h-path passes for `render_time` s). A torch CUDA tensor `film`
+        chunks of whole w*h-path passes for `render_time` s), both in the
+        regenerating loop. A torch CUDA tensor `film`
         ([h,w,3] float64) is accumulated into on the device; otherwise a fresh
         host array is returned. The film value is sum * w*h / samples."""
         p = _lib.LTParams()

@@ -137,6 +137,12 @@ def test_renderer_lt_api(device):
     assert np.allclose(f.cpu().numpy(), both, rtol=1e-9, atol=1e-12)
     t = RendererLT(sc, film, max_verts=6, seed=7, scheduler='time', render_time=0.2).render()
     assert t['processed'] % 1024 == 0 and t['processed'] >= 1024
+    # the time scheduler renders chunks of whole passes on consecutive pass indices:
+    # its film equals a num_samples render of as many light paths
+    ts, tst = sc.render_lt(max_verts=6, seed=8, render_time=0.3)
+    assert tst['samples'] == tst['passes'] * 1024 and tst['elapsed'] >= 0.3
+    fs_, _ = sc.render_lt(num_samples=tst['samples'], max_verts=6, seed=8)
+    assert np.allclose(ts, fs_, rtol=1e-9, atol=1e-12)
     with pytest.raises(_lib.LmError):
         sc.render_lt(num_samples=10, max_verts=1)
     with pytest.raises(_lib.LmError):
