@@ -1025,6 +1025,8 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
     int64_t total = 0, passes = 0;
     if (!(p->render_time > 0)) {
         total = p->num_samples;
+        if (int64_t(p->pass_offset) + (total - 1) / npix > int64_t(UINT32_MAX))
+            return fail(LMGPU_ERR_INVALID_ARGUMENT, "pass index beyond 2^32");
         LMGPU_TRY(run_loop(total, uint32_t(p->pass_offset)));
         passes = (total + npix - 1) / npix;
     } else {

@@ -785,6 +785,8 @@ int Scene::render(const lmgpu_pt_params* p, double* sum_in, double* sumsq_in, bo
     if (image && p->render_time <= 0 && p->num_samples <= 0)
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "image sampling mode needs num_samples or render_time");
     if (p->sample_offset < 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "sample_offset must be >= 0");
+    if (!image && p->render_time <= 0 && int64_t(p->sample_offset) + p->spp > int64_t(INT32_MAX))
+        return fail(LMGPU_ERR_INVALID_ARGUMENT, "sample index beyond 2^31");
     if (image && sumsq_in)
         return fail(LMGPU_ERR_INVALID_ARGUMENT, "image sampling mode has no per-pixel sum of squares (pass NULL)");
     if (image && gp) return fail(LMGPU_ERR_UNSUPPORTED, "guided rendering uses the pixel sampling mode");

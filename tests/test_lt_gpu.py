@@ -147,5 +147,7 @@ def test_renderer_lt_api(device):
         sc.render_lt(num_samples=10, max_verts=1)
     with pytest.raises(_lib.LmError):
         sc.render_lt(num_samples=0, max_verts=6)
+    with pytest.raises(_lib.LmError, match='beyond'):
+        sc.render_lt(num_samples=1024 * 2**32 + 1, max_verts=6)
     with pytest.raises(ValueError):
         RendererLT(sc, film, max_verts=6, scheduler='sample')

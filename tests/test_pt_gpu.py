@@ -202,3 +202,5 @@ def test_invalid_render_args(device):
         sc.render_raw(spp=0, max_verts=8)
     with pytest.raises(LmError):
         sc.render_raw(spp=1, max_verts=8, sampling_mode=7)
+    with pytest.raises(LmError, match='beyond'):
+        sc.render_raw(spp=4, max_verts=8, sample_offset=2**31 - 2)
