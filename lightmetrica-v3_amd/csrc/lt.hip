@@ -975,14 +975,14 @@ int Scene::lt_render(const lmgpu_lt_params* p, double* film_in, bool on_device, 
     if (dev.num_lights == 0) return fail(LMGPU_ERR_INVALID_ARGUMENT, "light tracing needs a light");
     const auto t0 = std::chrono::steady_clock::now();
     const int npix = w * h;
-    LMGPU_TRY(ensure_workspace(npix));
+    // the loop's queues, counters and path state live in the Wavefront pool;
+    // the per-pixel buffers of the one-sample passes are not needed
     double* film = film_in;
     if (!on_device) {
         if (fsum.n != size_t(npix) * 3) LMGPU_TRY(fsum.alloc(size_t(npix) * 3));
         film = fsum.p;
         LMGPU_HIP(hipMemsetAsync(film, 0, sizeof(double) * 3 * size_t(npix), s));
     }
-    LMGPU_HIP(hipMemsetAsync(counts.p, 0, 16 * sizeof(int32_t), s));
     LTConsts c{};
     c.max_verts = p->max_verts;
     c.seed_lo = uint32_t(p->seed);
