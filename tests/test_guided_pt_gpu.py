@@ -83,6 +83,32 @@ def test_guided_furnace_one_sample_passes(device, steps):
     assert abs(z) < 5
 
 
+def test_guided_furnace_time_scheduler(device):
+    """Guided rendering under the time scheduler (chunks of whole samples per
+    pixel in the regenerating loop, one guide trained across the chunks):
+    the furnace closed form E[L] = Le sum_k rho^k, and the stats count whole
+    samples per pixel."""
+    rho, Le, M = 0.5, 1.0, 8
+    flat = furnace_scene(rho, Le).flatten()
+    sc = GpuScene(flat)
+    lo, hi = sc.bounds()
+    g = Guide(lo - 1e-3, hi + 1e-3, seed=6)
+    s, q, st = sc.render_raw(render_time=0.3, max_verts=M, seed=9, guide=g,
+                             guide_opts={'batch': 4096, 'steps_per_pass': 2})
+    spp = st['spp']
+    assert spp >= 2 and st['samples'] == flat.w * flat.h * spp and st['elapsed'] >= 0.3
+    assert g.stats()[0] > 0 and st['guide_records'] > 0
+    n = flat.w * flat.h * spp
+    mean = s[..., 0].sum() / n
+    var = q[..., 0].sum() / n - mean ** 2
+    expected = Le * sum(rho ** k for k in range(M - 1))
+    z = (mean - expected) / math.sqrt(var / n)
+    print(f'guided furnace, time scheduler: {spp} spp, mean {mean:.6f} expected {expected:.6f} z {z:.2f}, '
+          f'{g.stats()[0]} Adam steps')
+    assert abs(z) < 5
+    assert np.isfinite(s).all()
+
+
 def test_guided_c0_welch_vs_oracle(device):
     """Config 0 at 128x128, 64 spp, guided with online training, against the
     FP64 oracle of plain renderer::pt (independent seeds)."""
