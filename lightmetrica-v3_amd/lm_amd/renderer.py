@@ -243,7 +243,9 @@ class GpuScene:
         p.seed = int(seed)
         p.render_time = float(render_time)
         st = _lib.PTStats()
-        if film is not None and hasattr(film, 'data_ptr'):
+        if film is not None and not hasattr(film, 'data_ptr'):
+            raise ValueError('film must be a torch CUDA tensor ([h,w,3] float64) or None')
+        if film is not None:
             _lib.call('lmgpu_lt_render', self._h, ctypes.byref(p), ctypes.c_void_p(self._film(film, 'film')), 1,
                       ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
             s = film
@@ -268,7 +270,9 @@ class GpuScene:
         p.render_time = float(render_time)
         p.min_verts = int(min_verts)
         st = _lib.PTStats()
-        if film is not None and hasattr(film, 'data_ptr'):
+        if film is not None and not hasattr(film, 'data_ptr'):
+            raise ValueError('film must be a torch CUDA tensor ([h,w,3] float64) or None')
+        if film is not None:
             _lib.call('lmgpu_bdpt_render', self._h, ctypes.byref(p), ctypes.c_void_p(self._film(film, 'film')), 1,
                       ctypes.byref(st), ctypes.c_void_p(_stream_ptr(stream)))
             s = film

@@ -99,3 +99,17 @@ def test_films_rejected_before_render(monkeypatch, film, what):
     with pytest.raises(ValueError, match=what):
         s.render_bdpt(num_samples=1, film=film)
     assert calls.names == []
+
+
+def test_host_array_films_rejected(monkeypatch):
+    """LT and BDPT accumulate only into device films: a host array is an
+    error, not silently replaced by a fresh film."""
+    import numpy as np
+    calls = _Calls(monkeypatch)
+    s = _scene()
+    film = np.zeros((6, 8, 3))
+    with pytest.raises(ValueError, match='torch CUDA tensor'):
+        s.render_lt(num_samples=1, film=film)
+    with pytest.raises(ValueError, match='torch CUDA tensor'):
+        s.render_bdpt(num_samples=1, film=film)
+    assert calls.names == []
