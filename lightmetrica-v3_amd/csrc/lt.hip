@@ -41,8 +41,6 @@ namespace {
 
 struct LTConsts {
     int max_verts;
-    int npaths;       // light paths of this pass
-    uint32_t pass;    // pass index (second Philox counter word)
     uint32_t seed_lo, seed_hi;
     f3 sph_c;         // bounding sphere of the scene (envconst rays)
     float sph_r;
@@ -196,9 +194,8 @@ __global__ __launch_bounds__(BLOCK) void k_lt_regen(SceneDev sc, LTConsts c, int
 // One light-path vertex per extension-queue entry (the body of the loop of
 // renderer_lt.cpp:92-162 for the vertex the entry's ray hit). Random numbers
 // (path, pass, vertex, 0) = {RR, component, direction u0, u1}.
-// REGEN: the queue entries are slots of the regenerating loop, whose light
-// path and pass are ppid / ppass; a path that ends marks its slot kFinishing.
-template <bool REGEN>
+// The queue entries are slots of the regenerating loop, whose light path and
+// pass are ppid / ppass; a path that ends marks its slot kFinishing.
 __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, Queues q, LTConsts c, int in_q,
                                                     const int32_t* __restrict__ ppid,
                                                     const int32_t* __restrict__ ppass, int32_t* __restrict__ pstate) {
@@ -225,8 +222,8 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                 const float4 thr4 = ps.thr[p];
                 f3 thr = mk(thr4.x, thr4.y, thr4.z);
                 const int nv = ps.info[p].x;
-                const uint32_t pid = REGEN ? uint32_t(ppid[p]) : uint32_t(p);
-                const uint32_t pass = REGEN ? uint32_t(ppass[p]) : c.pass;
+                const uint32_t pid = uint32_t(ppid[p]);
+                const uint32_t pass = uint32_t(ppass[p]);
                 const rand4 r0 = rng(pid, pass, uint32_t(nv), 0u, seed);
                 bool alive = true;
                 if (nv - 1 > 5) {
@@ -305,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_lt_shade(SceneDev sc, PathState ps, 
                 }
             }
         }
-        if (REGEN && i < n && !emit_ext) pstate[p] = kFinishing;
+        if (i < n && !emit_ext) pstate[p] = kFinishing;
         int se, ss;
         block_append2<kBlock>(q.counts + out_q, q.counts + 2, emit_ext, emit_sh, se, ss);
         if (emit_ext) {
@@ -388,7 +385,7 @@ void lt_regen_iteration(Scene& S, Wavefront& wf, const LTConsts& c, int64_t tota
     hipLaunchKernelGGL(k_lt_begin, dim3(1), dim3(64), 0, s, wf.next.p, wf.counts.p, in_q ^ 1);
     hipLaunchKernelGGL(k_extend<W>, dim3(extend_grid<W>()), dim3(kBlock), 0, s, S.dev, q, in_q, wf.next.p, refill,
                        LMGPU_REGEN_RAY_CHUNK);
-    hipLaunchKernelGGL(k_lt_shade<true>, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, wf.ppix.p, wf.psample.p,
+    hipLaunchKernelGGL(k_lt_shade, dim3(sg), dim3(kBlock), 0, s, S.dev, ps, q, c, in_q, wf.ppix.p, wf.psample.p,
                        wf.pstate.p);
     hipLaunchKernelGGL(k_lt_connect<W>, dim3(lt_connect_grid<W>()), dim3(kBlock), 0, s, S.dev, q, wf.next.p + 1, refill,
                        film, LMGPU_REGEN_RAY_CHUNK);
