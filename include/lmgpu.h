@@ -312,10 +312,13 @@ int lmgpu_raycast_render(lmgpu_scene* scene, const lmgpu_raycast_params* params,
  * envconst: light_envconst.cpp:47-74) connected to the pinhole camera at every
  * surface vertex, splatted into film[h][w][3] (FP64, y = 0 the bottom row;
  * device films accumulate, host films are overwritten). `num_samples` light
- * paths (scheduler::spi::sample), traced in passes of at most w*h; with
- * render_time > 0, passes of w*h paths until that many seconds have passed.
- * The reference's film value is film * (w*h) / stats->samples
- * (renderer_lt.cpp:166); stats->spp counts the passes. Light paths of pass k
+ * paths (scheduler::spi::sample); with render_time > 0, whole passes of w*h
+ * paths until that many seconds have passed. Both run in a regenerating
+ * wavefront loop: a slot whose light path ends takes the next one. Item j is
+ * light path j mod (w*h) of pass j / (w*h), so a render equals the passes it
+ * spans. The reference's film value is film * (w*h) / stats->samples
+ * (renderer_lt.cpp:166); stats->spp counts the passes (begun ones included).
+ * Rejects renders whose pass index would pass 2^32. Light paths of pass k
  * (pass_offset + k) draw Philox counters (path, pass, vertex, block), so
  * progressive calls continue with pass_offset += the previous spp. Paths that
  * hit the emitter directly seen by the camera (two vertices) carry no
